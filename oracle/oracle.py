@@ -1,0 +1,288 @@
+"""ctypes view of the CPU oracle (liboracle.so). TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module, and only as the checker or as the timed CPU baseline.  The
+product package (wgnet_amd) never imports it.
+
+Function-by-function reference citations are in wave_oracle.h.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_f32p = ctypes.POINTER(ctypes.c_float)
+_f64p = ctypes.POINTER(ctypes.c_double)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+
+
+class Problem(ctypes.Structure):
+    _fields_ = [
+        ("nvx", ctypes.c_int32),
+        ("nvz", ctypes.c_int32),
+        ("pml", ctypes.c_int32),
+        ("nt", ctypes.c_int32),
+        ("h", ctypes.c_float),
+        ("dt", ctypes.c_float),
+        ("vmin", ctypes.c_float),
+        ("vmax", ctypes.c_float),
+        ("peak_hz", ctypes.c_float),
+        ("src_x", ctypes.c_float),
+        ("src_z", ctypes.c_float),
+        ("nrec", ctypes.c_int32),
+        ("rec_xz", _f32p),
+    ]
+
+
+class Dims(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in ("pad", "nx", "nz", "cells", "region", "region_adj")]
+
+
+def build() -> str:
+    """Compile liboracle.so with the committed Makefile."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+_LIB = None
+
+
+def lib() -> ctypes.CDLL:
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.POINTER(Problem)
+        sig = {
+            "or_dims_of": (None, [P, ctypes.POINTER(Dims)]),
+            "or_ricker_f32": (None, [ctypes.c_int32, ctypes.c_float, ctypes.c_float, _f32p]),
+            "or_ricker_f64": (None, [ctypes.c_int32, ctypes.c_double, ctypes.c_double, _f64p]),
+            "or_cpml_f64": (None, [ctypes.c_int32, ctypes.c_float, ctypes.c_float, ctypes.c_int32, ctypes.c_float, ctypes.c_float, _f64p, _f64p, _f64p]),
+            "or_cpml_f32": (None, [ctypes.c_int32, ctypes.c_float, ctypes.c_float, ctypes.c_int32, ctypes.c_float, ctypes.c_float, _f32p, _f32p, _f32p]),
+            "or_corners": (None, [ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _i32p, _f32p]),
+            "or_pad_velocity": (None, [P, _f32p, _f32p, _f32p]),
+            "or_fold": (None, [P, _f32p, _f32p]),
+            "or_forward_f32": (None, [P, _f32p, _f32p, _f32p, _f32p]),
+            "or_forward_f64": (None, [P, _f32p, _f64p, _f64p, _f64p]),
+            "or_adjoint_f32": (None, [P, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p]),
+            "or_adjoint_f64": (None, [P, _f32p, _f64p, _f64p, _f64p, _f64p, _f64p]),
+            "or_born_f64": (None, [P, _f32p, _f64p, _f64p, _f64p]),
+            "or_misfit_f32": (ctypes.c_float, [ctypes.c_int64, _f32p, _f32p, ctypes.c_float, _f32p]),
+            "or_energy": (ctypes.c_double, [ctypes.c_int64, _f32p]),
+            "or_evaluate_adjoint_f32": (ctypes.c_int32, [P, ctypes.c_int32, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p]),
+            "or_evaluate_adjoint_f64": (ctypes.c_int32, [P, ctypes.c_int32, _f32p, _f32p, _f64p, _f64p, _f64p, _f64p]),
+            "or_batch_fwd_adj_f32": (None, [P, ctypes.c_int32, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, ctypes.c_int32]),
+            "or_batch_forward_f32": (None, [P, ctypes.c_int32, _f32p, _f32p, _f32p, _f32p, ctypes.c_int32]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def _p(a: np.ndarray | None, dtype):
+    if a is None:
+        return None
+    assert a.dtype == dtype and a.flags.c_contiguous, (a.dtype, a.flags)
+    return a.ctypes.data_as(_f32p if dtype == np.float32 else _f64p)
+
+
+@dataclass
+class Geometry:
+    """One (model, shot) problem.  Receivers default to one per surface cell."""
+
+    nvx: int = 70
+    nvz: int = 70
+    pml: int = 20
+    nt: int = 1000
+    h: float = 10.0
+    dt: float = 0.001
+    vmin: float = 1000.0
+    vmax: float = 5000.0
+    peak_hz: float = 15.0
+    src_x: float = 0.0
+    src_z: float = 10.0
+    rec_xz: np.ndarray | None = None
+
+    @property
+    def nrec(self) -> int:
+        return self.nvx if self.rec_xz is None else int(self.rec_xz.shape[0])
+
+    def problem(self) -> Problem:
+        self._rec = None if self.rec_xz is None else np.ascontiguousarray(self.rec_xz, np.float32)
+        return Problem(
+            self.nvx, self.nvz, self.pml, self.nt, self.h, self.dt, self.vmin, self.vmax,
+            self.peak_hz, self.src_x, self.src_z, self.nrec,
+            None if self._rec is None else self._rec.ctypes.data_as(_f32p),
+        )
+
+    def dims(self) -> Dims:
+        d = Dims()
+        lib().or_dims_of(ctypes.byref(self.problem()), ctypes.byref(d))
+        return d
+
+
+def ricker_f32(n: int, dt: float, f: float) -> np.ndarray:
+    out = np.zeros(n, np.float32)
+    lib().or_ricker_f32(n, dt, f, _p(out, np.float32))
+    return out
+
+
+def ricker_f64(n: int, dt: float, f: float) -> np.ndarray:
+    out = np.zeros(n, np.float64)
+    lib().or_ricker_f64(n, dt, f, _p(out, np.float64))
+    return out
+
+
+def cpml_f64(n, h, dt, width, max_vel, freq):
+    a, b, db = (np.zeros(n, np.float64) for _ in range(3))
+    lib().or_cpml_f64(n, h, dt, width, max_vel, freq, _p(a, np.float64), _p(b, np.float64), _p(db, np.float64))
+    return a, b, db
+
+
+def cpml_f32(n, h, dt, width, max_vel, freq):
+    a, b, db = (np.zeros(n, np.float32) for _ in range(3))
+    lib().or_cpml_f32(n, h, dt, width, max_vel, freq, _p(a, np.float32), _p(b, np.float32), _p(db, np.float32))
+    return a, b, db
+
+
+def corners(x, z, h, pad, nx, nz):
+    cells = np.zeros(4, np.int32)
+    w = np.zeros(4, np.float32)
+    lib().or_corners(x, z, h, pad, nx, nz, cells.ctypes.data_as(_i32p), _p(w, np.float32))
+    return cells, w
+
+
+def pad_velocity(g: Geometry, vel_vis: np.ndarray):
+    d = g.dims()
+    padded = np.zeros(d.cells, np.float32)
+    mask = np.zeros(d.cells, np.float32)
+    lib().or_pad_velocity(ctypes.byref(g.problem()), _p(np.ascontiguousarray(vel_vis, np.float32).reshape(-1), np.float32),
+                          _p(padded, np.float32), _p(mask, np.float32))
+    return padded, mask
+
+
+def fold(g: Geometry, padded: np.ndarray) -> np.ndarray:
+    out = np.zeros(g.nvx * g.nvz, np.float32)
+    lib().or_fold(ctypes.byref(g.problem()), _p(np.ascontiguousarray(padded, np.float32), np.float32), _p(out, np.float32))
+    return out
+
+
+def forward_f32(g: Geometry, vel_vis: np.ndarray, wavelet: np.ndarray, snaps: bool = False):
+    d = g.dims()
+    tr = np.zeros((g.nt, g.nrec), np.float32)
+    sn = np.zeros((g.nt, d.cells), np.float32) if snaps else None
+    lib().or_forward_f32(ctypes.byref(g.problem()), _p(np.ascontiguousarray(vel_vis, np.float32).reshape(-1), np.float32),
+                         _p(np.ascontiguousarray(wavelet, np.float32), np.float32), _p(tr, np.float32), _p(sn, np.float32))
+    return (tr, sn) if snaps else tr
+
+
+def forward_f64(g: Geometry, vel_vis: np.ndarray, wavelet: np.ndarray, snaps: bool = False):
+    d = g.dims()
+    tr = np.zeros((g.nt, g.nrec), np.float64)
+    sn = np.zeros((g.nt, d.cells), np.float64) if snaps else None
+    lib().or_forward_f64(ctypes.byref(g.problem()), _p(np.ascontiguousarray(vel_vis, np.float32).reshape(-1), np.float32),
+                         _p(np.ascontiguousarray(wavelet, np.float64), np.float64), _p(tr, np.float64), _p(sn, np.float64))
+    return (tr, sn) if snaps else tr
+
+
+def adjoint_f32(g: Geometry, vel_vis, wavelet, snaps, resid, want_dwav=False):
+    d = g.dims()
+    grad = np.zeros(d.cells, np.float32)
+    dwav = np.zeros(g.nt, np.float32) if want_dwav else None
+    lib().or_adjoint_f32(ctypes.byref(g.problem()), _p(np.ascontiguousarray(vel_vis, np.float32).reshape(-1), np.float32),
+                         _p(np.ascontiguousarray(wavelet, np.float32), np.float32), _p(snaps, np.float32),
+                         _p(np.ascontiguousarray(resid, np.float32), np.float32), _p(grad, np.float32), _p(dwav, np.float32))
+    return (grad, dwav) if want_dwav else grad
+
+
+def adjoint_f64(g: Geometry, vel_vis, wavelet, snaps, resid, want_dwav=False):
+    d = g.dims()
+    grad = np.zeros(d.cells, np.float64)
+    dwav = np.zeros(g.nt, np.float64) if want_dwav else None
+    lib().or_adjoint_f64(ctypes.byref(g.problem()), _p(np.ascontiguousarray(vel_vis, np.float32).reshape(-1), np.float32),
+                         _p(np.ascontiguousarray(wavelet, np.float64), np.float64), _p(snaps, np.float64),
+                         _p(np.ascontiguousarray(resid, np.float64), np.float64), _p(grad, np.float64), _p(dwav, np.float64))
+    return (grad, dwav) if want_dwav else grad
+
+
+def born_f64(g: Geometry, vel_vis, dvel_vis, wavelet) -> np.ndarray:
+    out = np.zeros((g.nt, g.nrec), np.float64)
+    lib().or_born_f64(ctypes.byref(g.problem()), _p(np.ascontiguousarray(vel_vis, np.float32).reshape(-1), np.float32),
+                      _p(np.ascontiguousarray(dvel_vis, np.float64).reshape(-1), np.float64),
+                      _p(np.ascontiguousarray(wavelet, np.float64), np.float64), _p(out, np.float64))
+    return out
+
+
+def misfit_f32(pred: np.ndarray, obs: np.ndarray, scale: float):
+    pred = np.ascontiguousarray(pred, np.float32).reshape(-1)
+    obs = np.ascontiguousarray(obs, np.float32).reshape(-1)
+    res = np.zeros_like(pred)
+    loss = lib().or_misfit_f32(pred.size, _p(pred, np.float32), _p(obs, np.float32), scale, _p(res, np.float32))
+    return float(np.float32(loss)), res
+
+
+def energy(obs: np.ndarray) -> float:
+    obs = np.ascontiguousarray(obs, np.float32).reshape(-1)
+    return float(lib().or_energy(obs.size, _p(obs, np.float32)))
+
+
+def evaluate_adjoint_f32(g: Geometry, src_xz: np.ndarray, vel_vis, wavelet, obs):
+    src = np.ascontiguousarray(src_xz, np.float32)
+    out = np.zeros(g.nvx * g.nvz, np.float32)
+    loss = np.zeros(1, np.float32)
+    rc = lib().or_evaluate_adjoint_f32(ctypes.byref(g.problem()), src.shape[0], _p(src, np.float32),
+                                       _p(np.ascontiguousarray(vel_vis, np.float32).reshape(-1), np.float32),
+                                       _p(np.ascontiguousarray(wavelet, np.float32), np.float32),
+                                       _p(np.ascontiguousarray(obs, np.float32).reshape(-1), np.float32),
+                                       _p(loss, np.float32), _p(out, np.float32))
+    if rc != 0:
+        raise ValueError("Observed traces have zero energy")
+    return float(loss[0]), out.reshape(g.nvz, g.nvx)
+
+
+def evaluate_adjoint_f64(g: Geometry, src_xz: np.ndarray, vel_vis, wavelet, obs):
+    src = np.ascontiguousarray(src_xz, np.float32)
+    out = np.zeros(g.nvx * g.nvz, np.float64)
+    loss = np.zeros(1, np.float64)
+    rc = lib().or_evaluate_adjoint_f64(ctypes.byref(g.problem()), src.shape[0], _p(src, np.float32),
+                                       _p(np.ascontiguousarray(vel_vis, np.float32).reshape(-1), np.float32),
+                                       _p(np.ascontiguousarray(wavelet, np.float64), np.float64),
+                                       _p(np.ascontiguousarray(obs, np.float64).reshape(-1), np.float64),
+                                       _p(loss, np.float64), _p(out, np.float64))
+    if rc != 0:
+        raise ValueError("Observed traces have zero energy")
+    return float(loss[0]), out.reshape(g.nvz, g.nvx)
+
+
+def batch_fwd_adj_f32(g: Geometry, src_xz, vel_vis, wavelet, obs, threads: int):
+    units = src_xz.shape[0]
+    d = g.dims()
+    losses = np.zeros(units, np.float32)
+    grads = np.zeros((units, d.cells), np.float32)
+    lib().or_batch_fwd_adj_f32(ctypes.byref(g.problem()), units, _p(np.ascontiguousarray(src_xz, np.float32), np.float32),
+                               _p(np.ascontiguousarray(vel_vis, np.float32), np.float32),
+                               _p(np.ascontiguousarray(wavelet, np.float32), np.float32),
+                               _p(np.ascontiguousarray(obs, np.float32), np.float32),
+                               _p(losses, np.float32), _p(grads, np.float32), threads)
+    return losses, grads
+
+
+def batch_forward_f32(g: Geometry, src_xz, vel_vis, wavelet, threads: int):
+    units = src_xz.shape[0]
+    traces = np.zeros((units, g.nt, g.nrec), np.float32)
+    lib().or_batch_forward_f32(ctypes.byref(g.problem()), units, _p(np.ascontiguousarray(src_xz, np.float32), np.float32),
+                               _p(np.ascontiguousarray(vel_vis, np.float32), np.float32),
+                               _p(np.ascontiguousarray(wavelet, np.float32), np.float32), _p(traces, np.float32), threads)
+    return traces
