@@ -1,0 +1,154 @@
+/*
+ * wgnet_hip: C-ABI of the MI355X wave engine (forward, exact discrete adjoint,
+ * misfit and gradient) that replaces the PyO3 module `wave_gauss_rs` of the
+ * WGNet reference (crates/wave-gauss-rs/src/lib.rs).
+ *
+ * Conventions
+ *  - Every entry point returns 0 on success and a negative WG_E_* code on
+ *    failure; wg_last_error() then returns a message (thread-local).
+ *  - Host-pointer calls (wg_simulate, wg_evaluate_adjoint) are synchronous,
+ *    as the reference's `simulate` / `evaluate_adjoint` are (lib.rs:108-137).
+ *  - *_dev calls take device pointers and a hipStream_t (passed as void*;
+ *    NULL = the null stream).  They enqueue work and return without waiting.
+ *  - Layouts follow the reference: velocity [nvz, nvx] row-major in m/s,
+ *    gathers [model, shot, t, receiver] (lib.rs:115), gradients [nvz, nvx]
+ *    in d(loss)/d(v in m/s) (lib.rs:118-119).
+ *  - One engine must not be used by two threads at once; calls on one engine
+ *    are serialised by an internal mutex.
+ */
+#ifndef WGNET_HIP_H
+#define WGNET_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WG_OK 0
+#define WG_E_INVALID (-1)   /* bad argument: maps to ValueError */
+#define WG_E_DEVICE (-2)    /* HIP runtime failure: maps to RuntimeError */
+#define WG_E_TOO_LARGE (-3) /* grid does not fit the resident kernel */
+#define WG_E_ZERO_ENERGY (-4) /* observed traces have zero energy (ValueError) */
+
+typedef struct wg_engine wg_engine;
+
+/* Engine parameters.  Defaults (wg_config_default) are those of
+ * PyWaveEngine(shots=5, nt=2001, dt=0.0005, peak_hz=15.0, pml=20,
+ * vmin=1000.0, vmax=5000.0) (lib.rs:52) on the 70 x 70 grid at h = 10 m with
+ * one receiver per surface cell (wave_wgpu.rs:175-215). */
+typedef struct {
+    int32_t shots;
+    int32_t nt;
+    float dt;
+    float peak_hz;
+    int32_t pml;
+    float vmin;
+    float vmax;
+    int32_t nvx; /* visible grid columns (reference: 70) */
+    int32_t nvz; /* visible grid rows (reference: 70) */
+    float h;     /* spacing in metres (reference: 10) */
+} wg_config;
+
+/* Fills the reference defaults (lib.rs:52, geology-core lib.rs:5-9). */
+void wg_config_default(wg_config *cfg);
+
+/* Thread-local message of the last failing call ("" if none). */
+const char *wg_last_error(void);
+
+/* Replaces PyWaveEngine::new / NativeWaveEngine::new_async
+ * (lib.rs:51-57, wave_wgpu.rs:156-712): geometry (default sources at
+ * x_k = xmax*k/(S-1), z = h, wave_wgpu.rs:204-214), C-PML profiles,
+ * Ricker wavelet, device buffers.  Uses the current HIP device. */
+int32_t wg_engine_create(const wg_config *cfg, wg_engine **out);
+void wg_engine_destroy(wg_engine *e);
+
+/* Getters of lib.rs:59-72 plus the padded-grid sizes. */
+int32_t wg_engine_shots(const wg_engine *e);
+int32_t wg_engine_nt(const wg_engine *e);
+int32_t wg_engine_nrec(const wg_engine *e);
+int32_t wg_engine_dims(const wg_engine *e, int32_t *nx, int32_t *nz,
+                       int32_t *pad);
+
+/* Source positions [shots, 2] (x, z) in metres: lib.rs:74-90,
+ * wave_wgpu.rs:761-785 (bilinear corners recomputed). */
+int32_t wg_get_sources(const wg_engine *e, float *xz, int32_t n);
+int32_t wg_set_sources(wg_engine *e, const float *xz, int32_t n);
+
+/* Source wavelet [nt]: lib.rs:92-106, wave_wgpu.rs:788-795. */
+int32_t wg_get_wavelet(const wg_engine *e, float *w, int32_t n);
+int32_t wg_set_wavelet(wg_engine *e, const float *w, int32_t n);
+
+/* PyWaveEngine.simulate (lib.rs:108-116, wave_wgpu.rs:799-844): host
+ * velocity [nvz*nvx] m/s -> host gathers [shots, nt, nrec].  Synchronous. */
+int32_t wg_simulate(wg_engine *e, const float *vel_host, float *traces_host);
+
+/* PyWaveEngine.evaluate_adjoint (lib.rs:118-137, wave_wgpu.rs:848-988):
+ * loss = 0.5 ||p - d||^2 / ||d||^2 and grad [nvz*nvx] = dloss/dv (m/s).
+ * Returns WG_E_ZERO_ENERGY when sum(obs^2) == 0 (the reference panics with
+ * "Observed traces have zero energy", wave_wgpu.rs:853).  Synchronous. */
+int32_t wg_evaluate_adjoint(wg_engine *e, const float *vel_host,
+                            const float *obs_host, float *loss_host,
+                            float *grad_host);
+
+/* Batched device path of WavePhysics.simulate (physics.py:59-64): B models
+ * vel_dev [B, nvz, nvx] -> traces_dev [B, shots, nt, nrec]. */
+int32_t wg_simulate_dev(wg_engine *e, int32_t B, const float *vel_dev,
+                        float *traces_dev, void *stream);
+
+/* Batched device path of WavePhysics.loss_and_grad (physics.py:66-76) in
+ * m/s units: per model b, loss_dev[b] and grad_dev [B, nvz, nvx].
+ * traces_dev [B, shots, nt, nrec] receives the predicted gathers (may be
+ * NULL).  Models whose observed energy is zero get loss = NaN and a NaN
+ * gradient; energy_dev [B] (may be NULL) receives sum(obs^2) per model in
+ * double so the caller can raise the reference's error. */
+int32_t wg_loss_grad_dev(wg_engine *e, int32_t B, const float *vel_dev,
+                         const float *obs_dev, float *loss_dev,
+                         float *grad_dev, float *traces_dev,
+                         double *energy_dev, void *stream);
+
+/* Transpose of the linearised engine at vel_dev, for the dot-product test
+ * and gradient checks (no reference counterpart: the reference exposes the
+ * adjoint only through evaluate_adjoint, splatAdjoint.wgsl:58-131).
+ *   resid_dev [B, shots, nt, nrec]: trace-space vector y (already scaled)
+ *   grad_dev  [B, nvz, nvx]: fold(mask * sum_s J_s^T y)    (may be NULL)
+ *   dwav_dev  [B, shots, nt]: d<traces, y>/d(wavelet)       (may be NULL)
+ *   traces_dev [B, shots, nt, nrec]: forward gathers       (may be NULL) */
+int32_t wg_adjoint_apply_dev(wg_engine *e, int32_t B, const float *vel_dev,
+                             const float *resid_dev, float *grad_dev,
+                             float *dwav_dev, float *traces_dev, void *stream);
+
+/* Tangent-linear (Born) forward: dtraces_dev [B, shots, nt, nrec] =
+ * J(vel) dvel, dvel_dev [B, nvz, nvx] (no reference counterpart; the other
+ * half of the dot-product test). */
+int32_t wg_born_dev(wg_engine *e, int32_t B, const float *vel_dev,
+                    const float *dvel_dev, float *dtraces_dev, void *stream);
+
+/* Wavelet-only forward used with dwav of wg_adjoint_apply_dev: gathers for
+ * per-(model, shot) wavelets wav_dev [B, shots, nt] (no reference
+ * counterpart). */
+int32_t wg_simulate_wavelets_dev(wg_engine *e, int32_t B, const float *vel_dev,
+                                 const float *wav_dev, float *traces_dev,
+                                 void *stream);
+
+/* Kernel timing (no reference counterpart; the reference records no
+ * timestamps, wave_wgpu.rs:829).  When enabled, HIP events are recorded on the
+ * launch stream around every forward and adjoint kernel launch; wg_timing_read
+ * waits for them and returns the summed durations (ms) and launch counts since
+ * the last read, then clears them. */
+int32_t wg_timing_enable(wg_engine *e, int32_t on);
+int32_t wg_timing_read(wg_engine *e, double *fwd_ms, int64_t *fwd_launches,
+                       double *adj_ms, int64_t *adj_launches);
+
+/* Reference Ricker (geology-core lib.rs:12-21), float32 with expf. */
+void wg_ricker_wavelet(int32_t samples, float dt, float peak_hz, float *out);
+
+/* Reference C-PML profiles (geology-core lib.rs:25-68), f64 inside. */
+void wg_cpml_profiles(int32_t n, float h, float dt, int32_t width,
+                      float max_vel, float freq, float *a, float *b,
+                      float *db);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,275 @@
+"""Pins the CPU oracle against the reference's own known-answer tests, closed
+forms and invariants a plausible mistake would break (CPU only).
+
+  ricker_wavelet   geology-core lib.rs:228-232 KAT + closed-form peak
+  cpml_profiles    lib.rs:234-239 KAT + closed-form sigma0, symmetry, db
+  forward          lib.rs:243-251 KAT, FP32 vs FP64, mirror symmetry,
+                   linearity and time invariance in the wavelet, arrival time
+  adjoint          FP64 dot-product tests (wavelet and Born), central FD
+  pad / fold       exact adjoint pair; clamp mask
+  misfit           closed form 0.5 ||p-d||^2 / ||d||^2
+  band-limit       unit DC gain, symmetric taps, band identity (verify.py:76-84)
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from oracle import bandlimit as BL
+
+
+def geom(O, **kw):
+    base = dict(nvx=24, nvz=20, nt=260, dt=0.001, src_x=87.0, src_z=10.0)
+    base.update(kw)
+    return O.Geometry(**base)
+
+
+def layered(nvz, nvx, v=(1800.0, 2600.0, 3400.0), depths=(7, 13)):
+    m = np.full((nvz, nvx), v[0], np.float32)
+    m[depths[0]:] = v[1]
+    m[depths[1]:] = v[2]
+    return m
+
+
+# ---------------------------------------------------------------- wavelet
+def test_ricker_reference_kat(orc):
+    w = orc.ricker_f32(2001, 0.0005, 15.0)
+    assert abs(w[200] - 1.0) < 1e-6  # lib.rs:231
+    assert abs(w[0]) < 1e-3  # lib.rs:232
+
+
+def test_ricker_closed_form(orc):
+    w = orc.ricker_f32(1000, 0.001, 15.0)
+    assert w[100] == np.float32(1.0)  # peak at 1.5 / (f dt) = 100 exactly
+    t = np.arange(1000) * 0.001 - 0.1
+    a = (math.pi * 15.0 * t) ** 2
+    np.testing.assert_allclose(w, (1 - 2 * a) * np.exp(-a), atol=2e-6)
+    assert np.argmax(w) == 100
+
+
+# ---------------------------------------------------------------- C-PML
+def test_cpml_reference_kat(orc):
+    a, b, db = orc.cpml_f32(114, 10.0, 0.0005, 20, 5000.0, 15.0)
+    assert a[55] == 0.0 and b[55] == 0.0  # lib.rs:237-238
+    assert a[5] > 0.0 and b[5] < 0.0  # lib.rs:239
+
+
+def test_cpml_closed_form(orc):
+    n, h, dt, width, vmax, f = 114, 10.0, 0.001, 20, 5000.0, 15.0
+    a, b, db = orc.cpml_f64(n, h, dt, width, vmax, f)
+    dt = float(np.float32(dt))  # the reference passes dt as float32 (lib.rs:28)
+    sigma0 = -3.0 * vmax * math.log(1e-3) / (2 * width * h)
+    assert abs(sigma0 - 259.0408229) < 1e-6
+    # frac = 1 at x <= 2: alpha = 0, b = exp(-sigma0 dt) - 1
+    assert abs(b[0] - (math.exp(-sigma0 * dt) - 1.0)) < 1e-15
+    for x in (5, 11, 19):
+        frac = (22 - x) / 20.0
+        s, al = sigma0 * frac * frac, math.pi * f * (1 - frac)
+        assert abs(a[x] - math.exp(-(s + al) * dt)) < 1e-15
+    np.testing.assert_array_equal(a[22:92], 0.0)  # frac = 0 in the interior
+    np.testing.assert_allclose(a, a[::-1], rtol=0, atol=1e-15)  # symmetric frame
+    np.testing.assert_allclose(b, b[::-1], rtol=0, atol=1e-15)
+    x = 9
+    assert abs(db[x] - ((2 / 3) * (b[x + 1] - b[x - 1]) - (1 / 12) * (b[x + 2] - b[x - 2])) / h) < 1e-15
+
+
+# ---------------------------------------------------------------- geometry
+def test_corners(orc):
+    cells, w = orc.corners(340.0, 10.0, 10.0, 22, 114, 114)
+    assert list(cells) == [23 * 114 + 56, 23 * 114 + 57, 24 * 114 + 56, 24 * 114 + 57]
+    assert list(w) == [1.0, 0.0, 0.0, 0.0]
+    cells, w = orc.corners(172.5, 10.0, 10.0, 22, 114, 114)  # x between nodes
+    np.testing.assert_allclose(w, [0.75, 0.25, 0.0, 0.0], atol=1e-6)
+    cells, w = orc.corners(1e6, 1e6, 10.0, 22, 114, 114)  # clamped to the last interior corner
+    assert cells[3] == 113 * 114 + 113
+
+
+def test_pad_fold_adjoint_pair(orc):
+    g = geom(orc)
+    rng = np.random.default_rng(0)
+    vel = rng.integers(1000, 5001, (g.nvz, g.nvx)).astype(np.float32)
+    vel[0, 0], vel[3, 3] = 1000.0, 5000.0
+    pad, mask = orc.pad_velocity(g, vel)
+    d = g.dims()
+    assert pad.size == d.cells
+    # clamp mask is 0 exactly at v <= vmin or v >= vmax (wave_wgpu.rs:724-728)
+    assert mask.reshape(d.nz, d.nx)[d.pad, d.pad] == 0.0 and mask.reshape(d.nz, d.nx)[d.pad + 3, d.pad + 3] == 0.0
+    x = rng.integers(-50, 50, (g.nvz, g.nvx)).astype(np.float32)
+    y = rng.integers(-50, 50, d.cells).astype(np.float32)
+    px, _ = orc.pad_velocity(orc.Geometry(nvx=g.nvx, nvz=g.nvz, vmin=-1e9, vmax=1e9), x)
+    lhs = float(np.dot(px.astype(np.float64), y))
+    rhs = float(np.dot(x.reshape(-1).astype(np.float64), orc.fold(g, y).astype(np.float64)))
+    assert lhs == rhs
+
+
+# ---------------------------------------------------------------- forward
+def test_cpu_solver_reference_kat(orc):
+    """lib.rs:243-251: homogeneous 2500 m/s, nt=400, dt=0.5 ms, src (340, 10)."""
+    g = orc.Geometry(nt=400, dt=0.0005, src_x=340.0, src_z=10.0)
+    tr = orc.forward_f32(g, np.full((70, 70), 2500.0, np.float32), orc.ricker_f32(400, 0.0005, 15.0))
+    assert float((tr.astype(np.float64) ** 2).sum()) > 1e-4
+    assert np.isfinite(tr).all()
+
+
+def test_forward_fp32_vs_fp64(orc):
+    g = geom(orc)
+    vel = layered(g.nvz, g.nvx)
+    w = orc.ricker_f32(g.nt, g.dt, g.peak_hz)
+    t32 = orc.forward_f32(g, vel, w)
+    t64 = orc.forward_f64(g, vel, w.astype(np.float64))
+    rel = np.linalg.norm(t32 - t64) / np.linalg.norm(t64)
+    assert rel < 1e-5, rel
+    assert np.all(t32[0] == 0.0)  # recorded before the first update
+
+
+def test_forward_mirror_symmetry(orc):
+    g = geom(orc, src_x=70.0)
+    vel = layered(g.nvz, g.nvx)
+    vel[:, :5] += 300.0  # break left-right symmetry of the model itself
+    w = orc.ricker_f64(g.nt, g.dt, g.peak_hz)
+    a = orc.forward_f64(g, vel, w)
+    gm = geom(orc, src_x=(g.nvx - 1) * 10.0 - 70.0)
+    b = orc.forward_f64(gm, vel[:, ::-1].copy(), w)
+    assert np.max(np.abs(a - b[:, ::-1])) <= 1e-12 * np.max(np.abs(a))
+
+
+def test_forward_linear_and_time_invariant(orc):
+    g = geom(orc)
+    vel = layered(g.nvz, g.nvx)
+    rng = np.random.default_rng(1)
+    w1, w2 = rng.standard_normal(g.nt), rng.standard_normal(g.nt)
+    f = lambda w: orc.forward_f64(g, vel, w)
+    lin = f(2.0 * w1 - 0.5 * w2) - (2.0 * f(w1) - 0.5 * f(w2))
+    assert np.max(np.abs(lin)) < 1e-10 * np.max(np.abs(f(w1)))
+    k = 17
+    ws = np.zeros(g.nt)
+    ws[k:] = w1[:-k]
+    np.testing.assert_allclose(f(ws)[k:], f(w1)[:-k], rtol=0, atol=1e-10 * np.max(np.abs(f(w1))))
+
+
+def test_direct_arrival_time(orc):
+    """Direct wave along the surface arrives at delay + distance / v."""
+    v = 2000.0
+    g = orc.Geometry(nvx=70, nvz=40, nt=500, dt=0.001, src_x=100.0, src_z=10.0)
+    tr = orc.forward_f64(g, np.full((40, 70), v, np.float32), orc.ricker_f64(500, 0.001, 15.0))
+    r = 60  # 500 m from the source
+    t_peak = np.argmax(np.abs(tr[:, r])) * 0.001
+    assert abs(t_peak - (0.1 + 500.0 / v)) < 0.012, t_peak
+
+
+# ---------------------------------------------------------------- adjoint
+def test_adjoint_dot_product_wavelet_fp64(orc):
+    g = geom(orc)
+    vel = layered(g.nvz, g.nvx)
+    rng = np.random.default_rng(2)
+    f = rng.standard_normal(g.nt)
+    tr, sn = orc.forward_f64(g, vel, f, snaps=True)
+    y = rng.standard_normal((g.nt, g.nrec))
+    _, dwav = orc.adjoint_f64(g, vel, f, sn, y, want_dwav=True)
+    lhs, rhs = float((tr * y).sum()), float((f * dwav).sum())
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+
+
+def test_adjoint_dot_product_born_fp64(orc):
+    g = geom(orc, src_x=93.0, src_z=23.0)  # off-node source: 4 weighted corners
+    vel = layered(g.nvz, g.nvx)
+    vel[2, 2], vel[5, 9] = 800.0, 5600.0  # clamped cells: mask zero
+    rng = np.random.default_rng(3)
+    w = orc.ricker_f64(g.nt, g.dt, g.peak_hz)
+    dv = rng.standard_normal((g.nvz, g.nvx))
+    y = rng.standard_normal((g.nt, g.nrec))
+    jdv = orc.born_f64(g, vel, dv, w)
+    _, sn = orc.forward_f64(g, vel, w, snaps=True)
+    gpad = orc.adjoint_f64(g, vel, w, sn, y)
+    _, mask = orc.pad_velocity(g, vel)
+    d = g.dims()
+    gm = (gpad * mask).reshape(d.nz, d.nx)
+    gv = np.zeros((g.nvz, g.nvx))
+    for z in range(d.nz):
+        for x in range(d.nx):
+            gv[min(max(z - d.pad, 0), g.nvz - 1), min(max(x - d.pad, 0), g.nvx - 1)] += gm[z, x]
+    lhs, rhs = float((jdv * y).sum()), float((dv * gv).sum())
+    assert abs(lhs - rhs) <= 1e-11 * abs(lhs)
+
+
+def test_gradient_central_fd_fp64(orc):
+    g = geom(orc)
+    true = layered(g.nvz, g.nvx)
+    cur = true.copy()
+    cur[8:12, 8:14] -= 150.0
+    src = np.array([[40.0, 10.0], [190.0, 10.0]], np.float32)
+    w = orc.ricker_f64(g.nt, g.dt, g.peak_hz)
+    obs = np.stack([orc.forward_f64(orc.Geometry(**{**g.__dict__, "src_x": s[0], "src_z": s[1]}), true, w) for s in src])
+    loss, grad = orc.evaluate_adjoint_f64(g, src, cur, w, obs)
+    rng = np.random.default_rng(4)
+    for (z, x) in [(9, 10), (3, 5), (15, 20)]:
+        h = 0.5
+        vp, vm = cur.copy(), cur.copy()
+        vp[z, x] += h
+        vm[z, x] -= h
+        lp, _ = orc.evaluate_adjoint_f64(g, src, vp, w, obs)
+        lm, _ = orc.evaluate_adjoint_f64(g, src, vm, w, obs)
+        fd = (lp - lm) / (2 * h)
+        assert abs(fd - grad[z, x]) <= 1e-5 * abs(fd) + 1e-14, (z, x, fd, grad[z, x])
+    d = rng.standard_normal((g.nvz, g.nvx)) * 0.5
+    lp, _ = orc.evaluate_adjoint_f64(g, src, (cur + d).astype(np.float32), w, obs)
+    assert lp != loss
+
+
+def test_evaluate_adjoint_f32_vs_f64(orc):
+    g = geom(orc)
+    true = layered(g.nvz, g.nvx)
+    cur = true.copy()
+    cur[8:12, 8:14] -= 150.0
+    src = np.array([[40.0, 10.0], [190.0, 10.0]], np.float32)
+    w = orc.ricker_f32(g.nt, g.dt, g.peak_hz)
+    obs = np.stack([orc.forward_f32(orc.Geometry(**{**g.__dict__, "src_x": s[0], "src_z": s[1]}), true, w) for s in src])
+    l32, g32 = orc.evaluate_adjoint_f32(g, src, cur, w, obs)
+    l64, g64 = orc.evaluate_adjoint_f64(g, src, cur, w.astype(np.float64), obs.astype(np.float64))
+    assert abs(l32 - l64) <= 1e-4 * l64
+    assert np.linalg.norm(g32 - g64) <= 1e-4 * np.linalg.norm(g64)
+    with pytest.raises(ValueError, match="zero energy"):
+        orc.evaluate_adjoint_f32(g, src, cur, w, np.zeros_like(obs))
+
+
+# ---------------------------------------------------------------- misfit
+def test_misfit_closed_form(orc):
+    rng = np.random.default_rng(5)
+    p = rng.standard_normal(70 * 300 + 17).astype(np.float32)
+    d = rng.standard_normal(p.size).astype(np.float32)
+    e = orc.energy(d)
+    assert abs(e - float((d.astype(np.float64) ** 2).sum())) <= 1e-12 * e
+    scale = np.float32(1.0 / e)
+    loss, res = orc.misfit_f32(p, d, float(scale))
+    ref = 0.5 * float(((p.astype(np.float64) - d) ** 2).sum()) / e
+    assert abs(loss - ref) <= 1e-5 * ref
+    np.testing.assert_array_equal(res, (p - d) * scale)
+
+
+# ---------------------------------------------------------------- band-limit
+def test_lowpass_kernel_properties():
+    for f, taps in [(6.0, 251), (9.0, 169), (12.0, 127), (15.0, 101)]:
+        k = BL.lowpass_kernel(f, 0.001)
+        assert len(k) == taps
+        assert abs(float(k.astype(np.float64).sum()) - 1.0) < 1e-6
+        np.testing.assert_array_equal(k, k[::-1])
+        # sigma = sqrt(ln2 / 2) / (pi f) puts the amplitude response at
+        # exp(-2 pi^2 sigma^2 f^2) = 1/2 at f
+        n = np.arange(len(k)) - len(k) // 2
+        H = abs(np.sum(k * np.exp(-2j * np.pi * f * n * 0.001)))
+        assert abs(H - 0.5) < 2e-3
+
+
+def test_band_identity_fp64(orc):
+    """verify.py:76-84: simulate(lowpass(w)) == lowpass(simulate(w))."""
+    g = geom(orc)
+    vel = layered(g.nvz, g.nvx)
+    w = orc.ricker_f64(g.nt, g.dt, g.peak_hz)
+    k = BL.lowpass_kernel(8.0, g.dt).astype(np.float64)
+    wl = np.convolve(w, k)[: g.nt]
+    low = orc.forward_f64(g, vel, wl)
+    filt = BL.causal_filter(orc.forward_f64(g, vel, w), k)
+    assert np.max(np.abs(low - filt)) <= 1e-10 * np.max(np.abs(filt))

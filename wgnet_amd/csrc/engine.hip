@@ -1,0 +1,751 @@
+// Host runtime of the wave engine: the C-ABI declared in include/wgnet_hip.h.
+// Replaces NativeWaveEngine (crates/wave-gauss-rs/src/wave_wgpu.rs) and the
+// PyO3 surface of crates/wave-gauss-rs/src/lib.rs.
+//
+// Differences from the reference engine, by design:
+//  - one kernel launch per batch of (model, shot) units runs all nt steps
+//    (the reference dispatches every step, wave_wgpu.rs:820-840);
+//  - models are batched (the reference loops over models in Python,
+//    physics.py:62-75) and the device path is asynchronous on a caller stream;
+//  - misfit, shot sum, clamp mask and fold run on the GPU.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/wgnet_hip.h"
+#include "wave_kernels.h"
+
+#pragma clang fp contract(off)
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCK(call)                                                                      \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return fail(WG_E_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr int kFdPad = 2;  // geology-core lib.rs:9
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t n = 0;
+};
+
+// Rust's saturating `f32 as usize` (NaN and negatives -> 0).
+int64_t sat_floor(float g) {
+    const float f = std::floor(g);
+    if (!(f > 0.0f)) return 0;
+    if (f > 9.0e18f) return INT64_MAX;
+    return (int64_t)f;
+}
+
+// Bilinear corners of a station (wave_wgpu.rs:217-230 and 766-780).
+void corners(float x, float z, float h, int pad, int nx, int nz, int cell[4], float w[4]) {
+    const float gx = x / h + (float)pad;
+    const float gz = z / h + (float)pad;
+    int64_t x0 = std::min<int64_t>(sat_floor(gx), nx - 2);
+    int64_t z0 = std::min<int64_t>(sat_floor(gz), nz - 2);
+    const float fx = gx - (float)x0;
+    const float fz = gz - (float)z0;
+    cell[0] = (int)(z0 * nx + x0);
+    cell[1] = (int)(z0 * nx + x0 + 1);
+    cell[2] = (int)((z0 + 1) * nx + x0);
+    cell[3] = (int)((z0 + 1) * nx + x0 + 1);
+    w[0] = (1.0f - fx) * (1.0f - fz);
+    w[1] = fx * (1.0f - fz);
+    w[2] = (1.0f - fx) * fz;
+    w[3] = fx * fz;
+}
+
+}  // namespace
+
+struct wg_engine {
+    wg_config cfg{};
+    wg::Grid g{};
+    int device = 0;
+    std::vector<float> src_xz, wavelet, prof;
+    std::vector<int> src_cell, rec_cell, band_off, ent_r;
+    std::vector<float> src_w, rec_w, ent_w;
+    int band_z0 = 0, band_rows = 0;
+    // device tables
+    float* d_prof = nullptr;
+    int* d_src_cell = nullptr;
+    float* d_src_w = nullptr;
+    int* d_rec_cell = nullptr;
+    float* d_rec_w = nullptr;
+    int* d_band_off = nullptr;
+    int* d_ent_r = nullptr;
+    float* d_ent_w = nullptr;
+    float* d_wavelet = nullptr;
+    // workspace, grown on demand
+    DevBuf snaps, traces, resid, gunit, partials, scale, energy, vel, obs, loss, grad;
+    hipStream_t stream = nullptr;  // host-pointer calls
+    hipStream_t last = nullptr;    // stream of the last enqueued call
+    bool last_valid = false;
+    size_t snap_budget = 0;
+    std::mutex mu;
+    // kernel timing: (start, stop) event pairs of forward / adjoint launches
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_fwd, ev_adj;
+
+    wg::Geometry geo() const {
+        wg::Geometry G;
+        G.prof = d_prof;
+        G.src_cell = d_src_cell;
+        G.src_w = d_src_w;
+        G.rec_cell = d_rec_cell;
+        G.rec_w = d_rec_w;
+        G.band_off = d_band_off;
+        G.ent_r = d_ent_r;
+        G.ent_w = d_ent_w;
+        G.band_z0 = band_z0;
+        G.band_rows = band_rows;
+        G.n_ent = (int)ent_r.size();
+        return G;
+    }
+};
+
+namespace {
+
+// Waits for the previous call's stream when a call arrives on another one, so
+// the shared workspace is never reused while still in flight.
+int fence(wg_engine* e, hipStream_t st) {
+    if (e->last_valid && e->last != st) HIPCK(hipStreamSynchronize(e->last));
+    e->last = st;
+    e->last_valid = true;
+    return WG_OK;
+}
+
+int ensure(wg_engine* e, DevBuf& b, size_t bytes) {
+    if (b.n >= bytes) return WG_OK;
+    if (e->last_valid) HIPCK(hipStreamSynchronize(e->last));
+    if (b.p) HIPCK(hipFree(b.p));
+    b.p = nullptr;
+    b.n = 0;
+    HIPCK(hipMalloc(&b.p, bytes));
+    b.n = bytes;
+    return WG_OK;
+}
+
+template <typename T>
+int upload(std::vector<T>& host, T** dev) {
+    if (*dev) HIPCK(hipFree(*dev));
+    *dev = nullptr;
+    const size_t n = std::max<size_t>(host.size(), 1);
+    HIPCK(hipMalloc((void**)dev, n * sizeof(T)));
+    if (!host.empty()) HIPCK(hipMemcpy(*dev, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice));
+    return WG_OK;
+}
+
+// Source corner tables from the current positions (wave_wgpu.rs:761-785).
+void build_sources(wg_engine* e) {
+    const int S = e->cfg.shots;
+    e->src_cell.assign(4 * S, 0);
+    e->src_w.assign(4 * S, 0.0f);
+    for (int s = 0; s < S; ++s)
+        corners(e->src_xz[2 * s], e->src_xz[2 * s + 1], e->cfg.h, e->g.pad, e->g.nx, e->g.nz, &e->src_cell[4 * s],
+                &e->src_w[4 * s]);
+}
+
+// Receivers at (k h, h), k < nvx (wave_wgpu.rs:215), their corners and the
+// per-cell (receiver, weight) lists with zero weights dropped, ordered by
+// (cell, receiver) (wave_wgpu.rs:232-270), stored for the rows they touch.
+void build_receivers(wg_engine* e) {
+    const wg::Grid& g = e->g;
+    const int R = g.nrec;
+    e->rec_cell.assign(4 * R, 0);
+    e->rec_w.assign(4 * R, 0.0f);
+    struct Ent {
+        int cell, r;
+        float w;
+    };
+    std::vector<Ent> ents;
+    for (int r = 0; r < R; ++r) {
+        corners((float)r * e->cfg.h, e->cfg.h, e->cfg.h, g.pad, g.nx, g.nz, &e->rec_cell[4 * r], &e->rec_w[4 * r]);
+        for (int k = 0; k < 4; ++k)
+            if (e->rec_w[4 * r + k] != 0.0f) ents.push_back({e->rec_cell[4 * r + k], r, e->rec_w[4 * r + k]});
+    }
+    std::stable_sort(ents.begin(), ents.end(),
+                     [](const Ent& a, const Ent& b) { return a.cell != b.cell ? a.cell < b.cell : a.r < b.r; });
+    e->ent_r.clear();
+    e->ent_w.clear();
+    if (ents.empty()) {
+        e->band_z0 = 0;
+        e->band_rows = 0;
+        e->band_off.assign(1, 0);
+        return;
+    }
+    const int z0 = ents.front().cell / g.nx, z1 = ents.back().cell / g.nx;
+    e->band_z0 = z0;
+    e->band_rows = z1 - z0 + 1;
+    e->band_off.assign((size_t)e->band_rows * g.nx + 1, 0);
+    for (const Ent& en : ents) {
+        e->band_off[en.cell - z0 * g.nx + 1]++;
+        e->ent_r.push_back(en.r);
+        e->ent_w.push_back(en.w);
+    }
+    for (size_t i = 1; i < e->band_off.size(); ++i) e->band_off[i] += e->band_off[i - 1];
+}
+
+int check_fits(const wg::Grid& g, int lds_limit, std::string& why) {
+    if (wg::slots_needed(g) < 0) {
+        why = "grid has more inner cells than the resident kernels hold";
+        return WG_E_TOO_LARGE;
+    }
+    if (g.nrec > 512) {
+        why = "more than 512 receivers";
+        return WG_E_TOO_LARGE;
+    }
+    if (g.ni >= 511 || g.nzi >= 511) {
+        why = "grid wider than 510 inner cells";
+        return WG_E_TOO_LARGE;
+    }
+    wg::Geometry dummy{};
+    dummy.band_rows = 2;
+    dummy.n_ent = 4 * g.nrec;
+    const size_t f = wg::forward_lds_bytes(g), a = wg::adjoint_lds_bytes(g, dummy);
+    if (f > (size_t)lds_limit || a > (size_t)lds_limit) {
+        why = "grid needs " + std::to_string(std::max(f, a)) + " B of LDS per workgroup, the device has " +
+              std::to_string(lds_limit);
+        return WG_E_TOO_LARGE;
+    }
+    return WG_OK;
+}
+
+wg::FwdParams fwd_params(wg_engine* e, const float* vel, float* traces) {
+    wg::FwdParams p{};
+    p.g = e->g;
+    p.geo = e->geo();
+    p.vel = vel;
+    p.wavelet = e->d_wavelet;
+    p.wav_stride = 0;
+    p.traces = traces;
+    return p;
+}
+
+wg::AdjParams adj_params(wg_engine* e, const float* vel, const float* resid, float* grad) {
+    wg::AdjParams p{};
+    p.g = e->g;
+    p.geo = e->geo();
+    p.vel = vel;
+    p.wavelet = e->d_wavelet;
+    p.wav_stride = 0;
+    p.resid = resid;
+    p.grad = grad;
+    return p;
+}
+
+size_t snap_bytes_per_unit(const wg_engine* e) { return (size_t)e->g.nt * e->g.inner * sizeof(float); }
+
+int models_per_chunk(wg_engine* e, int B) {
+    const size_t per_model = snap_bytes_per_unit(e) * e->cfg.shots;
+    size_t m = per_model ? e->snap_budget / per_model : (size_t)B;
+    if (m < 1) m = 1;
+    return (int)std::min<size_t>(m, (size_t)B);
+}
+
+#define LAUNCH(call)                                                                            \
+    do {                                                                                        \
+        hipError_t e_ = (call);                                                                 \
+        if (e_ != hipSuccess) return fail(WG_E_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// Forward (+ snapshots) of models [b0, b0 + nb) into traces, then optionally
+// the adjoint with residual `resid` ([B*S, nt, R] absolute) into gunit.
+// Records a (start, stop) event pair around a launch when timing is enabled.
+struct Timed {
+    wg_engine* e;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>>* list;
+    hipStream_t st;
+    hipEvent_t a = nullptr, b = nullptr;
+    Timed(wg_engine* en, std::vector<std::pair<hipEvent_t, hipEvent_t>>* l, hipStream_t s) : e(en), list(l), st(s) {
+        if (!e->timing) return;
+        if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+            e->timing = false;
+            return;
+        }
+        (void)hipEventRecord(a, st);
+    }
+    ~Timed() {
+        if (!a || !b) return;
+        (void)hipEventRecord(b, st);
+        list->emplace_back(a, b);
+    }
+};
+
+int run_chunk_forward(wg_engine* e, int b0, int nb, const float* vel, float* traces, bool store, hipStream_t st) {
+    wg::FwdParams p = fwd_params(e, vel, traces);
+    p.unit0 = b0 * e->cfg.shots;
+    p.snaps = store ? (float*)e->snaps.p : nullptr;
+    Timed tm(e, &e->ev_fwd, st);
+    LAUNCH(wg::launch_forward(p, nb * e->cfg.shots, store, false, st));
+    return WG_OK;
+}
+
+int run_chunk_adjoint(wg_engine* e, int b0, int nb, const float* vel, const float* resid, float* gunit, float* dwav,
+                      hipStream_t st) {
+    wg::AdjParams p = adj_params(e, vel, resid, gunit);
+    p.unit0 = b0 * e->cfg.shots;
+    p.snaps = (const float*)e->snaps.p;
+    p.dwav = dwav;
+    Timed tm(e, &e->ev_adj, st);
+    LAUNCH(wg::launch_adjoint(p, nb * e->cfg.shots, st));
+    return WG_OK;
+}
+
+// Loss and gradient of B models on stream st.  scale/energy may be preset by
+// the caller (host path) or computed here (device path).
+int loss_grad_core(wg_engine* e, int B, const float* vel, const float* obs, float* loss, float* grad,
+                   float* traces_out, double* energy, float* scale, bool compute_energy, hipStream_t st) {
+    const wg::Grid& g = e->g;
+    const int S = e->cfg.shots;
+    const long long per = (long long)S * g.nt * g.nrec;
+    const long long chunks = (per + 255) / 256;
+    int rc;
+    if ((rc = ensure(e, e->resid, (size_t)B * per * sizeof(float)))) return rc;
+    if ((rc = ensure(e, e->partials, (size_t)B * chunks * sizeof(float)))) return rc;
+    if ((rc = ensure(e, e->gunit, (size_t)B * S * g.nx * g.nz * sizeof(float)))) return rc;
+    float* traces = traces_out;
+    if (!traces) {
+        if ((rc = ensure(e, e->traces, (size_t)B * per * sizeof(float)))) return rc;
+        traces = (float*)e->traces.p;
+    }
+    const int mc = models_per_chunk(e, B);
+    if ((rc = ensure(e, e->snaps, snap_bytes_per_unit(e) * S * mc))) return rc;
+    if (compute_energy) LAUNCH(wg::launch_energy(obs, per, B, energy, scale, st));
+    for (int b0 = 0; b0 < B; b0 += mc) {
+        const int nb = std::min(mc, B - b0);
+        if ((rc = run_chunk_forward(e, b0, nb, vel, traces, true, st))) return rc;
+        LAUNCH(wg::launch_residual(traces + b0 * per, obs + b0 * per, scale + b0, per, nb,
+                                   (float*)e->resid.p + b0 * per, (float*)e->partials.p + b0 * chunks, st));
+        if ((rc = run_chunk_adjoint(e, b0, nb, vel, (const float*)e->resid.p, (float*)e->gunit.p, nullptr, st)))
+            return rc;
+    }
+    LAUNCH(wg::launch_loss_sum((const float*)e->partials.p, chunks, B, energy, loss, st));
+    LAUNCH(wg::launch_fold(g, (const float*)e->gunit.p, vel, energy, B, grad, st));
+    return WG_OK;
+}
+
+struct Guard {
+    wg_engine* e;
+    std::lock_guard<std::mutex> lk;
+    explicit Guard(wg_engine* en) : e(en), lk(en->mu) {}
+};
+
+}  // namespace
+
+extern "C" {
+
+void wg_config_default(wg_config* c) {
+    if (!c) return;
+    c->shots = 5;
+    c->nt = 2001;
+    c->dt = 0.0005f;
+    c->peak_hz = 15.0f;
+    c->pml = 20;
+    c->vmin = 1000.0f;
+    c->vmax = 5000.0f;
+    c->nvx = 70;
+    c->nvz = 70;
+    c->h = 10.0f;
+}
+
+const char* wg_last_error(void) { return g_err.c_str(); }
+
+// geology-core lib.rs:12-21 (float32, expf).
+void wg_ricker_wavelet(int32_t samples, float dt, float peak_hz, float* out) {
+    const float delay = 1.5f / peak_hz;
+    for (int32_t i = 0; i < samples; ++i) {
+        const float t = (float)i * dt - delay;
+        const float u = 3.14159265358979323846f * peak_hz * t;
+        const float a = u * u;
+        out[i] = (1.0f - 2.0f * a) * expf(-a);
+    }
+}
+
+// geology-core lib.rs:25-68 (double inside, float32 out).
+void wg_cpml_profiles(int32_t n, float h, float dt, int32_t width, float max_vel, float freq, float* ao, float* bo,
+                      float* dbo) {
+    std::vector<double> a(n, 0.0), b(n, 0.0), db(n, 0.0);
+    const double start0 = (double)(kFdPad + width);
+    const double start1 = (double)(n - 1 - kFdPad - width);
+    const double wf = (double)width, hf = (double)h, dtf = (double)dt;
+    const double sigma0 = -(3.0 * (double)max_vel * std::log(0.001)) / (2.0 * wf * hf);
+    for (int x = 0; x < n; ++x) {
+        const double xf = (double)x;
+        const double f0 = (start0 - xf) / wf, f1 = (xf - start1) / wf;
+        const double frac = std::min(std::max(std::max(f0, f1), 0.0), 1.0);
+        const double sigma = sigma0 * frac * frac;
+        const double alpha = 3.14159265358979323846 * (double)freq * (1.0 - frac);
+        const double axv = std::exp(-(sigma + alpha) * dtf);
+        b[x] = (sigma / (sigma + alpha)) * (axv - 1.0);
+        a[x] = frac == 0.0 ? 0.0 : axv;
+    }
+    for (int x = 2; x < n - 2; ++x)
+        db[x] = ((8.0 / 12.0) * (b[x + 1] - b[x - 1]) - (1.0 / 12.0) * (b[x + 2] - b[x - 2])) / hf;
+    for (int i = 0; i < n; ++i) {
+        ao[i] = (float)a[i];
+        bo[i] = (float)b[i];
+        dbo[i] = (float)db[i];
+    }
+}
+
+int32_t wg_engine_create(const wg_config* cfg, wg_engine** out) {
+    if (!cfg || !out) return fail(WG_E_INVALID, "null argument");
+    *out = nullptr;
+    const wg_config c = *cfg;
+    if (c.shots < 1 || c.nt < 1 || c.pml < 0 || c.nvx < 1 || c.nvz < 1)
+        return fail(WG_E_INVALID, "shots, nt, nvx, nvz must be >= 1 and pml >= 0");
+    if (!(c.h > 0.0f) || !(c.dt > 0.0f) || !(c.peak_hz > 0.0f))
+        return fail(WG_E_INVALID, "h, dt and peak_hz must be positive");
+    if (!(c.vmin <= c.vmax)) return fail(WG_E_INVALID, "vmin must not exceed vmax");
+
+    wg::Grid g{};
+    g.pad = kFdPad + c.pml;
+    g.nx = c.nvx + 2 * g.pad;
+    g.nz = c.nvz + 2 * g.pad;
+    g.ni = g.nx - 4;
+    g.nzi = g.nz - 4;
+    g.inner = g.ni * g.nzi;
+    g.region = c.pml + 2 * kFdPad;
+    g.region_adj = c.pml + 3 * kFdPad;
+    g.swf = g.region + 2;
+    g.swa = g.region_adj + 2;
+    g.nvx = c.nvx;
+    g.nvz = c.nvz;
+    g.nt = c.nt;
+    g.nrec = c.nvx;
+    g.shots = c.shots;
+    g.rh = 1.0f / c.h;
+    g.rh2 = g.rh * g.rh;
+    g.dt2 = c.dt * c.dt;
+    g.vmin = c.vmin;
+    g.vmax = c.vmax;
+
+    int dev = 0;
+    HIPCK(hipGetDevice(&dev));
+    int lds_limit = 0;
+    HIPCK(hipDeviceGetAttribute(&lds_limit, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    std::string why;
+    if (int rc = check_fits(g, lds_limit, why)) return fail(rc, why);
+
+    wg_engine* e = new wg_engine();
+    e->cfg = c;
+    e->g = g;
+    e->device = dev;
+    // Default sources: x_k = xmax k / (S-1), or xmax / 2 for one shot, at z = h
+    // (wave_wgpu.rs:203-214).
+    const float xmax = (float)(c.nvx - 1) * c.h;
+    e->src_xz.resize(2 * c.shots);
+    for (int k = 0; k < c.shots; ++k) {
+        e->src_xz[2 * k] = c.shots == 1 ? xmax * 0.5f : (xmax * (float)k) / (float)(c.shots - 1);
+        e->src_xz[2 * k + 1] = c.h;
+    }
+    build_sources(e);
+    build_receivers(e);
+    e->prof.resize(3 * (g.nx + g.nz));
+    wg_cpml_profiles(g.nx, c.h, c.dt, c.pml, c.vmax, c.peak_hz, e->prof.data(), e->prof.data() + g.nx,
+                     e->prof.data() + 2 * g.nx);
+    float* pz = e->prof.data() + 3 * g.nx;
+    wg_cpml_profiles(g.nz, c.h, c.dt, c.pml, c.vmax, c.peak_hz, pz, pz + g.nz, pz + 2 * g.nz);
+    e->wavelet.resize(c.nt);
+    wg_ricker_wavelet(c.nt, c.dt, c.peak_hz, e->wavelet.data());
+
+    int rc = WG_OK;
+    if (!rc) rc = upload(e->prof, &e->d_prof);
+    if (!rc) rc = upload(e->src_cell, &e->d_src_cell);
+    if (!rc) rc = upload(e->src_w, &e->d_src_w);
+    if (!rc) rc = upload(e->rec_cell, &e->d_rec_cell);
+    if (!rc) rc = upload(e->rec_w, &e->d_rec_w);
+    if (!rc) rc = upload(e->band_off, &e->d_band_off);
+    if (!rc) rc = upload(e->ent_r, &e->d_ent_r);
+    if (!rc) rc = upload(e->ent_w, &e->d_ent_w);
+    if (!rc) rc = upload(e->wavelet, &e->d_wavelet);
+    if (!rc) {
+        hipError_t he = hipStreamCreate(&e->stream);
+        if (he != hipSuccess) rc = fail(WG_E_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(he));
+    }
+    if (!rc) {
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) != hipSuccess) fr = (size_t)16 << 30;
+        e->snap_budget = std::min<size_t>(fr / 2, (size_t)96 << 30);
+    }
+    if (rc) {
+        wg_engine_destroy(e);
+        return rc;
+    }
+    *out = e;
+    return WG_OK;
+}
+
+void wg_engine_destroy(wg_engine* e) {
+    if (!e) return;
+    if (e->last_valid) (void)hipStreamSynchronize(e->last);
+    for (auto* l : {&e->ev_fwd, &e->ev_adj})
+        for (auto& pr : *l) {
+            (void)hipEventDestroy(pr.first);
+            (void)hipEventDestroy(pr.second);
+        }
+    for (void* p : {(void*)e->d_prof, (void*)e->d_src_cell, (void*)e->d_src_w, (void*)e->d_rec_cell,
+                    (void*)e->d_rec_w, (void*)e->d_band_off, (void*)e->d_ent_r, (void*)e->d_ent_w,
+                    (void*)e->d_wavelet})
+        if (p) (void)hipFree(p);
+    for (DevBuf* b : {&e->snaps, &e->traces, &e->resid, &e->gunit, &e->partials, &e->scale, &e->energy, &e->vel,
+                      &e->obs, &e->loss, &e->grad})
+        if (b->p) (void)hipFree(b->p);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+}
+
+int32_t wg_engine_shots(const wg_engine* e) { return e ? e->cfg.shots : -1; }
+int32_t wg_engine_nt(const wg_engine* e) { return e ? e->cfg.nt : -1; }
+int32_t wg_engine_nrec(const wg_engine* e) { return e ? e->g.nrec : -1; }
+
+int32_t wg_engine_dims(const wg_engine* e, int32_t* nx, int32_t* nz, int32_t* pad) {
+    if (!e) return fail(WG_E_INVALID, "null engine");
+    if (nx) *nx = e->g.nx;
+    if (nz) *nz = e->g.nz;
+    if (pad) *pad = e->g.pad;
+    return WG_OK;
+}
+
+int32_t wg_get_sources(const wg_engine* e, float* xz, int32_t n) {
+    if (!e || !xz) return fail(WG_E_INVALID, "null argument");
+    if (n != e->cfg.shots) return fail(WG_E_INVALID, "sources must have shape [shots, 2]");
+    std::memcpy(xz, e->src_xz.data(), sizeof(float) * 2 * n);
+    return WG_OK;
+}
+
+int32_t wg_set_sources(wg_engine* e, const float* xz, int32_t n) {
+    if (!e || !xz) return fail(WG_E_INVALID, "null argument");
+    if (n != e->cfg.shots) return fail(WG_E_INVALID, "sources must have shape [shots, 2]");
+    Guard G(e);
+    HIPCK(hipSetDevice(e->device));
+    if (e->last_valid) HIPCK(hipStreamSynchronize(e->last));
+    e->src_xz.assign(xz, xz + 2 * n);
+    build_sources(e);
+    HIPCK(hipMemcpy(e->d_src_cell, e->src_cell.data(), sizeof(int) * e->src_cell.size(), hipMemcpyHostToDevice));
+    HIPCK(hipMemcpy(e->d_src_w, e->src_w.data(), sizeof(float) * e->src_w.size(), hipMemcpyHostToDevice));
+    return WG_OK;
+}
+
+int32_t wg_get_wavelet(const wg_engine* e, float* w, int32_t n) {
+    if (!e || !w) return fail(WG_E_INVALID, "null argument");
+    if (n != e->cfg.nt) return fail(WG_E_INVALID, "wavelet length must equal nt");
+    std::memcpy(w, e->wavelet.data(), sizeof(float) * n);
+    return WG_OK;
+}
+
+int32_t wg_set_wavelet(wg_engine* e, const float* w, int32_t n) {
+    if (!e || !w) return fail(WG_E_INVALID, "null argument");
+    if (n != e->cfg.nt) return fail(WG_E_INVALID, "wavelet length must equal nt");
+    Guard G(e);
+    HIPCK(hipSetDevice(e->device));
+    if (e->last_valid) HIPCK(hipStreamSynchronize(e->last));
+    e->wavelet.assign(w, w + n);
+    HIPCK(hipMemcpy(e->d_wavelet, w, sizeof(float) * n, hipMemcpyHostToDevice));
+    return WG_OK;
+}
+
+int32_t wg_simulate(wg_engine* e, const float* vel_host, float* traces_host) {
+    if (!e || !vel_host || !traces_host) return fail(WG_E_INVALID, "null argument");
+    Guard G(e);
+    HIPCK(hipSetDevice(e->device));
+    int rc;
+    if ((rc = fence(e, e->stream))) return rc;
+    const size_t nvis = (size_t)e->g.nvx * e->g.nvz;
+    const size_t ntr = (size_t)e->cfg.shots * e->g.nt * e->g.nrec;
+    if ((rc = ensure(e, e->vel, nvis * sizeof(float)))) return rc;
+    if ((rc = ensure(e, e->traces, ntr * sizeof(float)))) return rc;
+    HIPCK(hipMemcpyAsync(e->vel.p, vel_host, nvis * sizeof(float), hipMemcpyHostToDevice, e->stream));
+    if ((rc = run_chunk_forward(e, 0, 1, (const float*)e->vel.p, (float*)e->traces.p, false, e->stream))) return rc;
+    HIPCK(hipMemcpyAsync(traces_host, e->traces.p, ntr * sizeof(float), hipMemcpyDeviceToHost, e->stream));
+    HIPCK(hipStreamSynchronize(e->stream));
+    return WG_OK;
+}
+
+int32_t wg_evaluate_adjoint(wg_engine* e, const float* vel_host, const float* obs_host, float* loss_host,
+                            float* grad_host) {
+    if (!e || !vel_host || !obs_host || !loss_host || !grad_host) return fail(WG_E_INVALID, "null argument");
+    Guard G(e);
+    HIPCK(hipSetDevice(e->device));
+    const size_t nvis = (size_t)e->g.nvx * e->g.nvz;
+    const size_t ntr = (size_t)e->cfg.shots * e->g.nt * e->g.nrec;
+    // wave_wgpu.rs:852-854: energy in f64 on the host, in order.
+    double energy = 0.0;
+    for (size_t i = 0; i < ntr; ++i) energy += (double)obs_host[i] * (double)obs_host[i];
+    if (!(energy > 0.0)) return fail(WG_E_ZERO_ENERGY, "Observed traces have zero energy");
+    const float scale = (float)(1.0 / energy);
+    int rc;
+    if ((rc = fence(e, e->stream))) return rc;
+    if ((rc = ensure(e, e->vel, nvis * sizeof(float)))) return rc;
+    if ((rc = ensure(e, e->obs, ntr * sizeof(float)))) return rc;
+    if ((rc = ensure(e, e->scale, 256))) return rc;
+    if ((rc = ensure(e, e->energy, 256))) return rc;
+    if ((rc = ensure(e, e->loss, 256))) return rc;
+    if ((rc = ensure(e, e->grad, nvis * sizeof(float)))) return rc;
+    HIPCK(hipMemcpyAsync(e->vel.p, vel_host, nvis * sizeof(float), hipMemcpyHostToDevice, e->stream));
+    HIPCK(hipMemcpyAsync(e->obs.p, obs_host, ntr * sizeof(float), hipMemcpyHostToDevice, e->stream));
+    HIPCK(hipMemcpyAsync(e->scale.p, &scale, sizeof(float), hipMemcpyHostToDevice, e->stream));
+    HIPCK(hipMemcpyAsync(e->energy.p, &energy, sizeof(double), hipMemcpyHostToDevice, e->stream));
+    if ((rc = loss_grad_core(e, 1, (const float*)e->vel.p, (const float*)e->obs.p, (float*)e->loss.p,
+                             (float*)e->grad.p, nullptr, (double*)e->energy.p, (float*)e->scale.p, false, e->stream)))
+        return rc;
+    HIPCK(hipMemcpyAsync(loss_host, e->loss.p, sizeof(float), hipMemcpyDeviceToHost, e->stream));
+    HIPCK(hipMemcpyAsync(grad_host, e->grad.p, nvis * sizeof(float), hipMemcpyDeviceToHost, e->stream));
+    HIPCK(hipStreamSynchronize(e->stream));
+    return WG_OK;
+}
+
+int32_t wg_simulate_dev(wg_engine* e, int32_t B, const float* vel_dev, float* traces_dev, void* stream) {
+    if (!e || !vel_dev || !traces_dev || B < 0) return fail(WG_E_INVALID, "invalid argument");
+    Guard G(e);
+    HIPCK(hipSetDevice(e->device));
+    hipStream_t st = (hipStream_t)stream;
+    int rc;
+    if ((rc = fence(e, st))) return rc;
+    if (B == 0) return WG_OK;
+    return run_chunk_forward(e, 0, B, vel_dev, traces_dev, false, st);
+}
+
+int32_t wg_loss_grad_dev(wg_engine* e, int32_t B, const float* vel_dev, const float* obs_dev, float* loss_dev,
+                         float* grad_dev, float* traces_dev, double* energy_dev, void* stream) {
+    if (!e || !vel_dev || !obs_dev || !loss_dev || !grad_dev || B < 0) return fail(WG_E_INVALID, "invalid argument");
+    Guard G(e);
+    HIPCK(hipSetDevice(e->device));
+    hipStream_t st = (hipStream_t)stream;
+    int rc;
+    if ((rc = fence(e, st))) return rc;
+    if (B == 0) return WG_OK;
+    if ((rc = ensure(e, e->scale, (size_t)B * sizeof(float)))) return rc;
+    double* energy = energy_dev;
+    if (!energy) {
+        if ((rc = ensure(e, e->energy, (size_t)B * sizeof(double)))) return rc;
+        energy = (double*)e->energy.p;
+    }
+    return loss_grad_core(e, B, vel_dev, obs_dev, loss_dev, grad_dev, traces_dev, energy, (float*)e->scale.p, true,
+                          st);
+}
+
+int32_t wg_adjoint_apply_dev(wg_engine* e, int32_t B, const float* vel_dev, const float* resid_dev, float* grad_dev,
+                             float* dwav_dev, float* traces_dev, void* stream) {
+    if (!e || !vel_dev || !resid_dev || B < 0) return fail(WG_E_INVALID, "invalid argument");
+    Guard G(e);
+    HIPCK(hipSetDevice(e->device));
+    hipStream_t st = (hipStream_t)stream;
+    int rc;
+    if ((rc = fence(e, st))) return rc;
+    if (B == 0) return WG_OK;
+    const wg::Grid& g = e->g;
+    const int S = e->cfg.shots;
+    const size_t per = (size_t)S * g.nt * g.nrec;
+    float* traces = traces_dev;
+    if (!traces) {
+        if ((rc = ensure(e, e->traces, B * per * sizeof(float)))) return rc;
+        traces = (float*)e->traces.p;
+    }
+    if ((rc = ensure(e, e->gunit, (size_t)B * S * g.nx * g.nz * sizeof(float)))) return rc;
+    const int mc = models_per_chunk(e, B);
+    if ((rc = ensure(e, e->snaps, snap_bytes_per_unit(e) * S * mc))) return rc;
+    for (int b0 = 0; b0 < B; b0 += mc) {
+        const int nb = std::min(mc, B - b0);
+        if ((rc = run_chunk_forward(e, b0, nb, vel_dev, traces, true, st))) return rc;
+        if ((rc = run_chunk_adjoint(e, b0, nb, vel_dev, resid_dev, (float*)e->gunit.p, dwav_dev, st))) return rc;
+    }
+    if (grad_dev) LAUNCH(wg::launch_fold(g, (const float*)e->gunit.p, vel_dev, nullptr, B, grad_dev, st));
+    return WG_OK;
+}
+
+int32_t wg_born_dev(wg_engine* e, int32_t B, const float* vel_dev, const float* dvel_dev, float* dtraces_dev,
+                    void* stream) {
+    if (!e || !vel_dev || !dvel_dev || !dtraces_dev || B < 0) return fail(WG_E_INVALID, "invalid argument");
+    Guard G(e);
+    HIPCK(hipSetDevice(e->device));
+    hipStream_t st = (hipStream_t)stream;
+    int rc;
+    if ((rc = fence(e, st))) return rc;
+    if (B == 0) return WG_OK;
+    const wg::Grid& g = e->g;
+    const int S = e->cfg.shots;
+    const size_t per = (size_t)S * g.nt * g.nrec;
+    if ((rc = ensure(e, e->traces, B * per * sizeof(float)))) return rc;
+    const int mc = models_per_chunk(e, B);
+    if ((rc = ensure(e, e->snaps, snap_bytes_per_unit(e) * S * mc))) return rc;
+    for (int b0 = 0; b0 < B; b0 += mc) {
+        const int nb = std::min(mc, B - b0);
+        if ((rc = run_chunk_forward(e, b0, nb, vel_dev, (float*)e->traces.p, true, st))) return rc;
+        wg::FwdParams p = fwd_params(e, vel_dev, dtraces_dev);
+        p.unit0 = b0 * S;
+        p.dvel = dvel_dev;
+        p.born_snaps = (const float*)e->snaps.p;
+        LAUNCH(wg::launch_forward(p, nb * S, false, true, st));
+    }
+    return WG_OK;
+}
+
+int32_t wg_simulate_wavelets_dev(wg_engine* e, int32_t B, const float* vel_dev, const float* wav_dev,
+                                 float* traces_dev, void* stream) {
+    if (!e || !vel_dev || !wav_dev || !traces_dev || B < 0) return fail(WG_E_INVALID, "invalid argument");
+    Guard G(e);
+    HIPCK(hipSetDevice(e->device));
+    hipStream_t st = (hipStream_t)stream;
+    int rc;
+    if ((rc = fence(e, st))) return rc;
+    if (B == 0) return WG_OK;
+    wg::FwdParams p = fwd_params(e, vel_dev, traces_dev);
+    p.wavelet = wav_dev;
+    p.wav_stride = e->g.nt;
+    LAUNCH(wg::launch_forward(p, B * e->cfg.shots, false, false, st));
+    return WG_OK;
+}
+
+int32_t wg_timing_enable(wg_engine* e, int32_t on) {
+    if (!e) return fail(WG_E_INVALID, "null engine");
+    Guard G(e);
+    e->timing = on != 0;
+    return WG_OK;
+}
+
+int32_t wg_timing_read(wg_engine* e, double* fwd_ms, int64_t* fwd_n, double* adj_ms, int64_t* adj_n) {
+    if (!e) return fail(WG_E_INVALID, "null engine");
+    Guard G(e);
+    HIPCK(hipSetDevice(e->device));
+    double sums[2] = {0.0, 0.0};
+    int64_t counts[2] = {0, 0};
+    int which = 0;
+    for (auto* l : {&e->ev_fwd, &e->ev_adj}) {
+        for (auto& pr : *l) {
+            HIPCK(hipEventSynchronize(pr.second));
+            float ms = 0.0f;
+            HIPCK(hipEventElapsedTime(&ms, pr.first, pr.second));
+            sums[which] += ms;
+            counts[which] += 1;
+            (void)hipEventDestroy(pr.first);
+            (void)hipEventDestroy(pr.second);
+        }
+        l->clear();
+        ++which;
+    }
+    if (fwd_ms) *fwd_ms = sums[0];
+    if (fwd_n) *fwd_n = counts[0];
+    if (adj_ms) *adj_ms = sums[1];
+    if (adj_n) *adj_n = counts[1];
+    return WG_OK;
+}
+
+}  // extern "C"

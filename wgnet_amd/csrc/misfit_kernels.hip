@@ -1,0 +1,143 @@
+// Misfit and gradient-fold kernels of the wave engine (gfx950).
+//   residual / loss  wave_wgpu.rs:13-54 (RESIDUAL_WGSL), 852-854, 960-965
+//   shot sum, clamp mask and fold  wave_wgpu.rs:974-1005
+// These are small HBM-bound passes over the gathers [B, shots, nt, nrec] and
+// the per-unit padded gradients; the float32 tree of the reference's 256-wide
+// residual workgroup is reproduced exactly so the loss matches bit for bit.
+#include "wave_kernels.h"
+
+#pragma clang fp contract(off)
+
+namespace wg {
+
+// ---------------------------------------------------------------------------
+// Misfit pieces.
+// ---------------------------------------------------------------------------
+// Per-model observed energy in f64 (wave_wgpu.rs:852; summation order is a
+// tree, not the reference's sequential loop) and scale = f32(1 / energy).
+__global__ __launch_bounds__(256) void k_energy(const float* obs, long long per, double* energy,
+                                                float* scale) {
+    __shared__ double part[256];
+    const float* o = obs + (size_t)blockIdx.x * per;
+    double acc = 0.0;
+    for (long long i = threadIdx.x; i < per; i += 256) acc += (double)o[i] * (double)o[i];
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) part[threadIdx.x] += part[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double e = part[0];
+        energy[blockIdx.x] = e;
+        scale[blockIdx.x] = e > 0.0 ? (float)(1.0 / e) : 0.0f;
+    }
+}
+
+// RESIDUAL_WGSL (wave_wgpu.rs:28-53): one 256-wide workgroup per chunk of
+// 256 elements of one model's [shots, nt, nrec] gathers; residual (p-d)*scale
+// and the float32 tree sum of 0.5 d^2 scale.
+__global__ __launch_bounds__(256) void k_residual(const float* pred, const float* obs,
+                                                  const float* scale, long long per,
+                                                  long long chunks, float* resid,
+                                                  float* partials) {
+    __shared__ float wg_sums[256];
+    const long long blk = blockIdx.x;
+    const long long b = blk / chunks;
+    const long long c = blk - b * chunks;
+    const long long i = c * 256 + threadIdx.x;
+    const float sc = scale[b];
+    float e = 0.0f;
+    if (i < per) {
+        const size_t o = (size_t)b * per + i;
+        const float d = pred[o] - obs[o];
+        resid[o] = d * sc;
+        e = 0.5f * d * d * sc;
+    }
+    wg_sums[threadIdx.x] = e;
+    __syncthreads();
+    for (int st = 128; st > 0; st /= 2) {
+        if ((int)threadIdx.x < st) wg_sums[threadIdx.x] += wg_sums[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[blk] = wg_sums[0];
+}
+
+// Host-side f64 sum of the partials (wave_wgpu.rs:960-965), one thread per
+// model, chunks in order.  NaN for zero observed energy.
+__global__ void k_loss_sum(const float* partials, long long chunks, int B, const double* energy,
+                           float* loss) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    double acc = 0.0;
+    for (long long c = 0; c < chunks; ++c) acc += (double)partials[(size_t)b * chunks + c];
+    loss[b] = energy[b] > 0.0 ? (float)acc : __builtin_nanf("");
+}
+
+// wave_wgpu.rs:974-981 then fold_node_gradient (:993-1005): one thread per
+// visible node; the padded cells folding onto it are visited in row-major
+// order and summed in f64, as the reference's sequential fold does.
+__global__ void k_fold(Grid g, const float* grad_unit, const float* vel, const double* energy,
+                       int B, float* out) {
+    const int nvis = g.nvx * g.nvz;
+    const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= (long long)B * nvis) return;
+    const int b = (int)(gid / nvis);
+    const int vi = (int)(gid - (long long)b * nvis);
+    const int vz = vi / g.nvx, vx = vi - (vi / g.nvx) * g.nvx;
+    const float v = vel[(size_t)b * nvis + vi];
+    const float mask = (v > g.vmin && v < g.vmax) ? 1.0f : 0.0f;
+    const int zlo = vz == 0 ? 0 : vz + g.pad;
+    const int zhi = vz == g.nvz - 1 ? g.nz - 1 : vz + g.pad;
+    const int xlo = vx == 0 ? 0 : vx + g.pad;
+    const int xhi = vx == g.nvx - 1 ? g.nx - 1 : vx + g.pad;
+    const int cells = g.nx * g.nz;
+    const float* gb = grad_unit + (size_t)b * g.shots * cells;
+    double acc = 0.0;
+    for (int z = zlo; z <= zhi; ++z)
+        for (int x = xlo; x <= xhi; ++x) {
+            const int c = z * g.nx + x;
+            float sum_s = 0.0f;
+            for (int s = 0; s < g.shots; ++s) sum_s += gb[(size_t)s * cells + c];
+            acc += (double)(sum_s * mask);
+        }
+    out[gid] = energy && !(energy[b] > 0.0) ? __builtin_nanf("") : (float)acc;
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+hipError_t launch_energy(const float* obs, long long per, int B, double* energy, float* scale,
+                         hipStream_t st) {
+    if (B <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_energy, dim3(B), dim3(256), 0, st, obs, per, energy, scale);
+    return hipGetLastError();
+}
+
+hipError_t launch_residual(const float* pred, const float* obs, const float* scale, long long per,
+                           int B, float* resid, float* partials, hipStream_t st) {
+    if (B <= 0) return hipSuccess;
+    const long long chunks = (per + 255) / 256;
+    hipLaunchKernelGGL(k_residual, dim3((unsigned)(chunks * B)), dim3(256), 0, st, pred, obs, scale,
+                       per, chunks, resid, partials);
+    return hipGetLastError();
+}
+
+hipError_t launch_loss_sum(const float* partials, long long chunks, int B, const double* energy,
+                           float* loss, hipStream_t st) {
+    if (B <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_loss_sum, dim3((B + 63) / 64), dim3(64), 0, st, partials, chunks, B, energy,
+                       loss);
+    return hipGetLastError();
+}
+
+hipError_t launch_fold(const Grid& g, const float* grad_unit, const float* vel, const double* energy,
+                       int B, float* grad_vis, hipStream_t st) {
+    if (B <= 0) return hipSuccess;
+    const long long n = (long long)B * g.nvx * g.nvz;
+    hipLaunchKernelGGL(k_fold, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, grad_unit, vel,
+                       energy, B, grad_vis);
+    return hipGetLastError();
+}
+
+}  // namespace wg

@@ -1,0 +1,94 @@
+// Internal interface between the host runtime (engine.hip) and the gfx950
+// kernels (wave_kernels.hip).  Not part of the public C-ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace wg {
+
+// Padded-grid description shared by every kernel.  nx = nvx + 2 pad,
+// pad = 2 + pml; the inner (updated) cells are x, z in [2, n-2).
+struct Grid {
+    int nx, nz;           // padded grid
+    int ni, nzi;          // inner grid (nx-4, nz-4)
+    int inner;            // ni * nzi
+    int pad;              // FD_PAD + pml
+    int region;           // forward C-PML region, pml + 4 (wave_wgpu.rs:291)
+    int region_adj;       // adjoint C-PML region, pml + 6 (wave_wgpu.rs:295)
+    int swf, swa;         // LDS strip widths: region + 2, region_adj + 2
+    int nvx, nvz;         // visible grid
+    int nt, nrec, shots;
+    float rh, rh2, dt2;   // 1/h, rh*rh, dt*dt (float32, lib.rs:95-97)
+    float vmin, vmax;
+};
+
+// Per-engine device tables.
+struct Geometry {
+    const float* prof;       // ax bx dbx (nx each) az bz dbz (nz each)
+    const int* src_cell;     // [shots*4] corner cells (padded index)
+    const float* src_w;      // [shots*4] bilinear weights
+    const int* rec_cell;     // [nrec*4]
+    const float* rec_w;      // [nrec*4]
+    // Receiver CSR restricted to the rows [band_z0, band_z0 + band_rows):
+    // band_off[(z - band_z0) * nx + x] .. band_off[... + 1] index ent_r/ent_w.
+    const int* band_off;
+    const int* ent_r;
+    const float* ent_w;
+    int band_z0, band_rows, n_ent;
+};
+
+struct FwdParams {
+    Grid g;
+    Geometry geo;
+    const float* vel;        // [B, nvz, nvx] m/s
+    const float* wavelet;    // [units or 1, nt]
+    long long wav_stride;    // 0 = one wavelet for every unit
+    float* traces;           // [units, nt, nrec]
+    float* snaps;            // [launch units, nt, inner] or null (indexed by
+                             // the unit's position in this launch)
+    const float* dvel;       // Born: [B, nvz, nvx] perturbation
+    const float* born_snaps; // Born: base snapshots, same indexing as snaps
+    int unit0;               // first unit of this launch (model b = unit / shots)
+};
+
+struct AdjParams {
+    Grid g;
+    Geometry geo;
+    const float* vel;        // [B, nvz, nvx]
+    const float* wavelet;    // [nt] (source term of the gradient)
+    long long wav_stride;
+    const float* resid;      // [units, nt, nrec] scaled residual
+    const float* snaps;      // [launch units, nt, inner] (position in launch)
+    float* grad;             // [units, cells] padded-grid gradient per unit
+    float* dwav;             // [units, nt] or null
+    int unit0;
+};
+
+// Returns hipSuccess or the launch error.  lds_bytes_* report the dynamic
+// LDS a launch needs (0 when the grid does not fit one workgroup).
+size_t forward_lds_bytes(const Grid& g);
+size_t adjoint_lds_bytes(const Grid& g, const Geometry& geo);
+// Register slots per thread for the resident kernels (a compiled K >=
+// ceil(inner / 512)), or -1 when the grid has too many inner cells.
+int slots_needed(const Grid& g);
+
+hipError_t launch_forward(const FwdParams& p, int units, bool store, bool born,
+                          hipStream_t st);
+hipError_t launch_adjoint(const AdjParams& p, int units, hipStream_t st);
+
+// Misfit: energy (f64), residual (p-d)*scale and the reference's 256-chunk
+// float32 tree partials summed in f64.
+hipError_t launch_energy(const float* obs, long long per_model, int B,
+                         double* energy, float* scale, hipStream_t st);
+hipError_t launch_residual(const float* pred, const float* obs,
+                           const float* scale, long long per_model, int B,
+                           float* resid, float* partials, hipStream_t st);
+hipError_t launch_loss_sum(const float* partials, long long chunks, int B,
+                           const double* energy, float* loss, hipStream_t st);
+// grad_vis[b] = fold(mask * sum_s grad_unit[b*S+s]); NaN where energy == 0.
+hipError_t launch_fold(const Grid& g, const float* grad_unit, const float* vel,
+                       const double* energy, int B, float* grad_vis,
+                       hipStream_t st);
+
+}  // namespace wg

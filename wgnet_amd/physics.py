@@ -1,0 +1,173 @@
+"""Wave-equation misfit as a PyTorch loss, batched on the GPU.
+
+Counterpart of python/wgnet/physics.py (and the engine factory/constants of
+python/wgnet/data.py:20-33, 190-193).  Same API (`lowpass_kernel`,
+`WavePhysics.set_band/bandlimit/simulate/loss_and_grad`), same units (km/s
+fields, gradients w.r.t. km/s) and layouts ([B, shots, nt, 70] gathers), but
+every model of a batch goes to the GPU in one call instead of a Python loop
+(physics.py:62-75), and `misfit()` exposes the loss as a torch.autograd.Function
+whose gradient is the HIP adjoint.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+
+import numpy as np
+import torch
+
+from . import _lib
+from .wave_gauss_rs import PyWaveEngine
+
+# python/wgnet/data.py:20-29
+N = 70
+H = 10.0
+SHOTS = 5
+NT = 2001
+DT = 0.0005
+PEAK_HZ = 15.0
+DECIMATE = 8
+NT_DEC = (NT - 1) // DECIMATE + 1
+
+
+def make_engine(shots: int = SHOTS, nt: int = NT, dt: float = DT, peak_hz: float = PEAK_HZ) -> PyWaveEngine:
+    """data.py:32-33 (dt and peak_hz are parameters here; defaults unchanged)."""
+    return PyWaveEngine(shots, nt, dt, peak_hz, 20, 1000.0, 5000.0)
+
+
+def source_positions_m(shots: int = SHOTS) -> np.ndarray:
+    """data.py:190-193: default source x positions in metres."""
+    xmax = (N - 1) * H
+    return np.array([xmax / 2.0], np.float32) if shots == 1 else np.linspace(0.0, xmax, shots, dtype=np.float32)
+
+
+def lowpass_kernel(fmax_hz: float, dt: float = DT) -> np.ndarray:
+    """Gaussian low-pass with its -3 dB point at fmax_hz, truncated at 4 sigma,
+    unit DC gain (physics.py:18-26)."""
+    sigma_s = math.sqrt(math.log(2.0) / 2.0) / (math.pi * fmax_hz)
+    half = int(math.ceil(4.0 * sigma_s / dt))
+    n = np.arange(-half, half + 1)
+    h = np.exp(-0.5 * (n * dt / sigma_s) ** 2)
+    return (h / h.sum()).astype(np.float32)
+
+
+def _dev_ptr(t: torch.Tensor) -> ctypes.c_void_p:
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream() -> ctypes.c_void_p:
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _device_f32(x, device) -> torch.Tensor:
+    if isinstance(x, torch.Tensor):
+        return x.to(device=device, dtype=torch.float32).contiguous()
+    return torch.from_numpy(np.ascontiguousarray(x, dtype=np.float32)).to(device)
+
+
+class WaveMisfit(torch.autograd.Function):
+    """losses[b] = 0.5 ||p_b - d_b||^2 / ||d_b||^2 for fields in km/s.
+
+    forward runs the HIP forward (storing snapshots), the misfit and the HIP
+    adjoint in one batched call; backward scales the adjoint gradient by the
+    upstream gradient of each model's loss."""
+
+    @staticmethod
+    def forward(ctx, fields_km: torch.Tensor, observed: torch.Tensor, physics: "WavePhysics"):
+        losses, grads_km = physics.loss_and_grad_torch(fields_km.detach(), observed)
+        ctx.save_for_backward(grads_km)
+        return losses
+
+    @staticmethod
+    def backward(ctx, g):
+        (grads_km,) = ctx.saved_tensors
+        return g.view(-1, 1, 1) * grads_km, None, None
+
+
+class WavePhysics:
+    """Owns one engine; frequency continuation by wavelet low-pass
+    (physics.py:29-76)."""
+
+    def __init__(self, shots: int = SHOTS, nt: int = NT, dt: float = DT, peak_hz: float = PEAK_HZ,
+                 device: str | torch.device = "cuda") -> None:
+        self.engine = make_engine(shots, nt, dt, peak_hz)
+        self.shots, self.nt, self.dt = shots, nt, dt
+        self.device = torch.device(device)
+        self.full_wavelet = self.engine.wavelet().copy()
+        self.kernel: np.ndarray | None = None
+        self.fmax: float | None = None
+
+    # -- band --------------------------------------------------------------
+    def set_band(self, fmax_hz: float | None) -> None:
+        """None restores the full-band Ricker (physics.py:37-46)."""
+        if fmax_hz is None:
+            self.kernel, self.fmax = None, None
+            self.engine.set_wavelet(self.full_wavelet)
+            return
+        self.kernel = lowpass_kernel(fmax_hz, self.dt)
+        self.fmax = float(fmax_hz)
+        w = np.convolve(self.full_wavelet, self.kernel, mode="full")[: self.nt].astype(np.float32)
+        self.engine.set_wavelet(np.ascontiguousarray(w))
+
+    def bandlimit(self, traces):
+        """Causal low-pass along time of [..., nt, nrec] data (physics.py:48-57);
+        numpy in -> numpy out, tensor in -> tensor out on its device."""
+        if self.kernel is None:
+            return traces
+        is_np = not isinstance(traces, torch.Tensor)
+        t = torch.from_numpy(np.ascontiguousarray(traces, dtype=np.float32)) if is_np else traces.float()
+        lead = t.shape[:-2]
+        x = t.reshape(-1, 1, t.shape[-2], t.shape[-1]).transpose(2, 3).reshape(-1, 1, t.shape[-2])
+        k = torch.from_numpy(self.kernel).to(t.device).view(1, 1, -1)
+        y = torch.nn.functional.conv1d(torch.nn.functional.pad(x, (len(self.kernel) - 1, 0)), k)
+        out = y.view(*lead, t.shape[-1], t.shape[-2]).transpose(-1, -2).contiguous()
+        return out.numpy() if is_np else out
+
+    # -- solves --------------------------------------------------------------
+    def simulate_torch(self, fields_km: torch.Tensor) -> torch.Tensor:
+        """[B, 70, 70] km/s (device) -> [B, shots, nt, 70] (device)."""
+        vel = (_device_f32(fields_km, self.device) * 1000.0).contiguous()
+        B = vel.shape[0]
+        out = torch.empty((B, self.shots, self.nt, self.engine.nrec), dtype=torch.float32, device=self.device)
+        _lib.check(self.engine._lib.wg_simulate_dev(self.engine.handle, B, _dev_ptr(vel), _dev_ptr(out), _stream()))
+        return out
+
+    def simulate(self, fields_km):
+        """[B, 70, 70] km/s -> [B, shots, nt, 70] with the current passband
+        (physics.py:59-64).  numpy in -> numpy out."""
+        if isinstance(fields_km, torch.Tensor):
+            return self.simulate_torch(fields_km)
+        return self.simulate_torch(torch.from_numpy(np.asarray(fields_km, np.float32))).cpu().numpy()
+
+    def loss_and_grad_torch(self, fields_km: torch.Tensor, observed: torch.Tensor,
+                            traces_out: torch.Tensor | None = None) -> tuple[torch.Tensor, torch.Tensor]:
+        """Per-model normalised misfit and its gradient w.r.t. km/s, on the
+        device, for a whole batch in one call."""
+        vel = (_device_f32(fields_km, self.device) * 1000.0).contiguous()
+        obs = _device_f32(observed, self.device)
+        B = vel.shape[0]
+        if obs.numel() != B * self.shots * self.nt * self.engine.nrec:
+            raise ValueError("observed length does not match shots * nt * nrec")
+        losses = torch.empty(B, dtype=torch.float32, device=self.device)
+        grads = torch.empty_like(vel)
+        energy = torch.empty(B, dtype=torch.float64, device=self.device)
+        tr = _dev_ptr(traces_out) if traces_out is not None else None
+        _lib.check(self.engine._lib.wg_loss_grad_dev(self.engine.handle, B, _dev_ptr(vel), _dev_ptr(obs),
+                                                     _dev_ptr(losses), _dev_ptr(grads), tr, _dev_ptr(energy),
+                                                     _stream()))
+        if bool((energy <= 0).any()):
+            raise ValueError("Observed traces have zero energy")
+        return losses, grads * 1000.0  # d/d(km/s) = 1000 d/d(m/s)
+
+    def loss_and_grad(self, fields_km, observed):
+        """physics.py:66-76: numpy in -> (losses [B] f32, grads [B, 70, 70] f32)."""
+        if isinstance(fields_km, torch.Tensor):
+            return self.loss_and_grad_torch(fields_km, observed)
+        losses, grads = self.loss_and_grad_torch(torch.from_numpy(np.asarray(fields_km, np.float32)),
+                                                 torch.from_numpy(np.asarray(observed, np.float32)))
+        return losses.cpu().numpy(), grads.cpu().numpy()
+
+    def misfit(self, fields_km: torch.Tensor, observed: torch.Tensor) -> torch.Tensor:
+        """Differentiable per-model losses [B] (torch.autograd.Function)."""
+        return WaveMisfit.apply(fields_km, _device_f32(observed, self.device), self)
