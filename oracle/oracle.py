@@ -76,6 +76,7 @@ def lib() -> ctypes.CDLL:
             "or_adjoint_f32": (None, [P, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p]),
             "or_adjoint_f64": (None, [P, _f32p, _f64p, _f64p, _f64p, _f64p, _f64p]),
             "or_born_f64": (None, [P, _f32p, _f64p, _f64p, _f64p]),
+            "or_born_f32": (None, [P, _f32p, _f32p, _f32p, _f32p]),
             "or_misfit_f32": (ctypes.c_float, [ctypes.c_int64, _f32p, _f32p, ctypes.c_float, _f32p]),
             "or_energy": (ctypes.c_double, [ctypes.c_int64, _f32p]),
             "or_evaluate_adjoint_f32": (ctypes.c_int32, [P, ctypes.c_int32, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p]),
@@ -222,6 +223,14 @@ def born_f64(g: Geometry, vel_vis, dvel_vis, wavelet) -> np.ndarray:
     lib().or_born_f64(ctypes.byref(g.problem()), _p(np.ascontiguousarray(vel_vis, np.float32).reshape(-1), np.float32),
                       _p(np.ascontiguousarray(dvel_vis, np.float64).reshape(-1), np.float64),
                       _p(np.ascontiguousarray(wavelet, np.float64), np.float64), _p(out, np.float64))
+    return out
+
+
+def born_f32(g: Geometry, vel_vis, dvel_vis, wavelet) -> np.ndarray:
+    out = np.zeros((g.nt, g.nrec), np.float32)
+    lib().or_born_f32(ctypes.byref(g.problem()), _p(np.ascontiguousarray(vel_vis, np.float32).reshape(-1), np.float32),
+                      _p(np.ascontiguousarray(dvel_vis, np.float32).reshape(-1), np.float32),
+                      _p(np.ascontiguousarray(wavelet, np.float32), np.float32), _p(out, np.float32))
     return out
 
 

@@ -203,6 +203,33 @@ void or_born_f64(const or_problem *p, const float *vel_vis,
     free(snaps);
 }
 
+/* The same tangent-linear model in float32 arithmetic (the GPU Born kernel's
+ * operation order), for the roundoff level of the FP32 dot-product test. */
+void or_born_f32(const or_problem *p, const float *vel_vis, const float *dvel_vis,
+                 const float *wavelet, float *dtraces) {
+    or_dims d;
+    or_dims_of(p, &d);
+    float *snaps = (float *)calloc((size_t)p->nt * d.cells, sizeof(float));
+    float *traces = (float *)malloc(sizeof(float) * (size_t)p->nt * (p->nrec > 0 ? p->nrec : 1));
+    or_forward_f32(p, vel_vis, wavelet, traces, snaps);
+    float *velp = (float *)malloc(sizeof(float) * 2 * (size_t)d.cells);
+    or_pad_velocity(p, vel_vis, velp, velp + d.cells);
+    float *dvp = (float *)malloc(sizeof(float) * (size_t)d.cells);
+    for (int z = 0; z < d.nz; ++z) {
+        const int vz = clampi(z - d.pad, 0, p->nvz - 1);
+        for (int x = 0; x < d.nx; ++x) {
+            const int vx = clampi(x - d.pad, 0, p->nvx - 1);
+            const int i = z * d.nx + x;
+            dvp[i] = dvel_vis[vz * p->nvx + vx] * velp[d.cells + i];
+        }
+    }
+    forward_core_f32(p, vel_vis, wavelet, dtraces, NULL, dvp, snaps);
+    free(dvp);
+    free(velp);
+    free(traces);
+    free(snaps);
+}
+
 double or_energy(int64_t count, const float *obs) {
     double e = 0.0;
     for (int64_t i = 0; i < count; ++i) e += (double)obs[i] * (double)obs[i];

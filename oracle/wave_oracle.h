@@ -111,6 +111,9 @@ void or_born_f64(const or_problem *p, const float *vel_vis,
                  const double *dvel_vis, const double *wavelet,
                  double *dtraces);
 
+void or_born_f32(const or_problem *p, const float *vel_vis,
+                 const float *dvel_vis, const float *wavelet, float *dtraces);
+
 /* a8: residual in place semantics.  count elements; returns the loss and
  * writes resid = (pred - obs) * scale.  scale = f32(1 / sum obs^2 [f64]). */
 float or_misfit_f32(int64_t count, const float *pred, const float *obs,
