@@ -97,6 +97,29 @@ def test_reference_two_layer_kat(W, orc):
     assert relerr(one, ref) <= 1e-5
 
 
+def test_other_grid_runtime_kernels(W, orc):
+    """A grid other than 70 x 70 / pml 20 runs the runtime-geometry kernels:
+    gathers and gradient must equal the oracle's on it too."""
+    nvx, nvz, pml, nt = 64, 48, 12, 300
+    E = W.PyWaveEngine(2, nt, 0.001, 15.0, pml, 1000.0, 5000.0, nvx=nvx, nvz=nvz)
+    E.set_sources(np.array([[31.0, 10.0], [407.5, 24.0]], np.float32))
+    rng = np.random.default_rng(11)
+    true = np.full((nvz, nvx), 2000.0, np.float32)
+    true[12:] = 2800.0
+    true[30:] = 3600.0
+    cur = true + rng.uniform(-80, 80, true.shape).astype(np.float32)
+    obs = E.simulate(true.reshape(-1))
+    for s, (x, z) in enumerate(E.sources()):
+        g = orc.Geometry(nvx=nvx, nvz=nvz, pml=pml, nt=nt, dt=0.001, src_x=float(x), src_z=float(z))
+        assert relerr(obs[s], orc.forward_f32(g, true, E.wavelet())) <= 1e-5
+    loss, grad = E.evaluate_adjoint(cur.reshape(-1), obs.reshape(-1))
+    g = orc.Geometry(nvx=nvx, nvz=nvz, pml=pml, nt=nt, dt=0.001)
+    rl, rg = orc.evaluate_adjoint_f32(g, E.sources(), cur, E.wavelet(), obs)
+    print(f"64x48 pml12: loss {loss:.9e} vs {rl:.9e}, grad rel-L2 {relerr(grad, rg):.3e}")
+    assert abs(loss - rl) <= 1e-6 * rl
+    assert relerr(grad, rg) <= 1e-6
+
+
 def test_default_sources_and_setters(W):
     E = W.PyWaveEngine(5, 50)
     np.testing.assert_allclose(E.sources(), [[0, 10], [172.5, 10], [345, 10], [517.5, 10], [690, 10]])

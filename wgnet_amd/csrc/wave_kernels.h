@@ -75,6 +75,14 @@ int slots_needed(const Grid& g);
 
 hipError_t launch_forward(const FwdParams& p, int units, bool store, bool born,
                           hipStream_t st);
+
+// Kernels specialised for the reference grid (70 x 70, pml 20): return false
+// when the grid is another one (then the runtime-geometry kernels run).
+bool launch_forward_fx(const FwdParams& p, int units, bool store, bool born, hipStream_t st,
+                       hipError_t* err);
+bool launch_adjoint_fx(const AdjParams& p, int units, hipStream_t st, hipError_t* err);
+size_t forward_lds_bytes_fx(const Grid& g);
+size_t adjoint_lds_bytes_fx(const Grid& g, const Geometry& geo);
 hipError_t launch_adjoint(const AdjParams& p, int units, hipStream_t st);
 
 // Misfit: energy (f64), residual (p-d)*scale and the reference's 256-chunk

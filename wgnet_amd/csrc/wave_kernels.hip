@@ -814,6 +814,8 @@ static hipError_t adj_t(const AdjParams& p, int units, hipStream_t st) {
 
 hipError_t launch_forward(const FwdParams& p, int units, bool store, bool born, hipStream_t st) {
     if (units <= 0) return hipSuccess;
+    hipError_t err;
+    if (launch_forward_fx(p, units, store, born, st, &err)) return err;
     switch (pick_tuple(p.g, p.g.region - 2)) {
         case 0: return fwd_t<1, 1, 1, 1>(p, units, store, born, st);
         case 1: return fwd_t<2, 2, 2, 2>(p, units, store, born, st);
@@ -825,6 +827,8 @@ hipError_t launch_forward(const FwdParams& p, int units, bool store, bool born, 
 
 hipError_t launch_adjoint(const AdjParams& p, int units, hipStream_t st) {
     if (units <= 0) return hipSuccess;
+    hipError_t err;
+    if (launch_adjoint_fx(p, units, st, &err)) return err;
     switch (pick_tuple(p.g, p.g.region_adj - 2)) {
         case 0: return adj_t<1, 1, 1, 1>(p, units, st);
         case 1: return adj_t<2, 2, 2, 2>(p, units, st);
