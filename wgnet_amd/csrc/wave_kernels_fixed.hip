@@ -97,6 +97,17 @@ struct Place {
         last_ok = active && (row0 + (R::K - 1) * R::RP < R::H);
     }
     __device__ bool valid(int k) const { return k < R::K - 1 ? active : last_ok; }
+    // The slot whose padded row lies in [z_lo, z_hi), or -1 (rows of one
+    // thread are RP > 1 apart, so a band of at most RP rows holds one).
+    __device__ int slot_in_rows(int z_lo, int z_hi) const {
+        int r = -1;
+#pragma unroll
+        for (int k = 0; k < R::K; ++k) {
+            const int z = zi0 + k * R::RP + 2;
+            if (valid(k) && z >= z_lo && z < z_hi) r = k;
+        }
+        return r;
+    }
 };
 
 // Which rectangle of band width RB holds inner cell (xi, zi), and the owner
@@ -466,12 +477,14 @@ __device__ __forceinline__ void a_phase1(ASt<R::K, R::BX, R::BZ>& S, const Place
 
 template <class G, class R>
 __device__ __forceinline__ void a_phase2(ASt<R::K, R::BX, R::BZ>& S, const Place<R>& pl, const ACtxF& C,
-                                         const ColX& cx, const SrcOwnF& so, const float* snap_t, float f_t) {
+                                         const ColX& cx, const SrcOwnF& so, int rslot, const float* snap_t,
+                                         float f_t) {
     if (!pl.active) return;
     constexpr int SW2 = 2 * G::SWA;
     const int xi = opq(pl.xi), zi0 = opq(pl.zi0);
     const float* PR = opq_s(C.PR);
     const int sk = opq(so.slot);
+    const int rk = opq(rslot);
     const int x = xi + 2;
     const int z0 = zi0 + 2;
     const int i0 = z0 * G::NX + x;
@@ -503,7 +516,7 @@ __device__ __forceinline__ void a_phase2(ASt<R::K, R::BX, R::BZ>& S, const Place
         }
         float next = w + 2.0f * S.lam[k] - S.lamp[k];
         float gk = S.lam[k] * Sn;  // store_every = 1 (wgsl:234-236)
-        if (z >= C.bz0 && z < C.bz1) {  // receiver residuals, CSR order (wgsl:239-242)
+        if (k == rk) {  // receiver residuals, CSR order (wgsl:239-242)
             const int o = (z - C.bz0) * G::NX + x;
             for (int e = C.BOFF[o]; e < C.BOFF[o + 1]; ++e) next += C.EW[e] * C.RES[C.ER[e]];
         }
@@ -647,6 +660,13 @@ __global__ __launch_bounds__(NT) void k_adjoint_fx(AdjParams P) {
         for (int d = 0; d < 5; ++d) cx.opb[d] = PR[Prof<G>::OPBX + x - 2 + d];
     }
 
+    int rs[6];  // slot holding the receiver rows, per rectangle
+    rs[0] = pI.slot_in_rows(C.bz0, C.bz1);
+    rs[1] = pX.slot_in_rows(C.bz0, C.bz1);
+    rs[2] = pZT.slot_in_rows(C.bz0, C.bz1);
+    rs[3] = pZB.slot_in_rows(C.bz0, C.bz1);
+    rs[4] = pCT.slot_in_rows(C.bz0, C.bz1);
+    rs[5] = pCB.slot_in_rows(C.bz0, C.bz1);
     for (int t = g.nt - 1; t >= 0; --t) {
         for (int rr = tid; rr < g.nrec; rr += NT) RES[rr] = resid[(size_t)t * g.nrec + rr];
         a_phase1<G>(sI, pI, C, cx, so[0]);
@@ -664,12 +684,12 @@ __global__ __launch_bounds__(NT) void k_adjoint_fx(AdjParams P) {
         }
         const float f_t = wav[t];
         const float* snap_t = snaps + (size_t)t * G::INNER;
-        a_phase2<G>(sI, pI, C, cx, so[0], snap_t, f_t);
-        a_phase2<G>(sX, pX, C, cx, so[1], snap_t, f_t);
-        a_phase2<G>(sZT, pZT, C, cx, so[2], snap_t, f_t);
-        a_phase2<G>(sZB, pZB, C, cx, so[3], snap_t, f_t);
-        a_phase2<G>(sCT, pCT, C, cx, so[4], snap_t, f_t);
-        a_phase2<G>(sCB, pCB, C, cx, so[5], snap_t, f_t);
+        a_phase2<G>(sI, pI, C, cx, so[0], rs[0], snap_t, f_t);
+        a_phase2<G>(sX, pX, C, cx, so[1], rs[1], snap_t, f_t);
+        a_phase2<G>(sZT, pZT, C, cx, so[2], rs[2], snap_t, f_t);
+        a_phase2<G>(sZB, pZB, C, cx, so[3], rs[3], snap_t, f_t);
+        a_phase2<G>(sCT, pCT, C, cx, so[4], rs[4], snap_t, f_t);
+        a_phase2<G>(sCB, pCB, C, cx, so[5], rs[5], snap_t, f_t);
         __syncthreads();
     }
     float* gout = P.grad + (size_t)unit * G::NX * G::NZ;
