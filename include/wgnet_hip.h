@@ -131,6 +131,17 @@ int32_t wg_simulate_wavelets_dev(wg_engine *e, int32_t B, const float *vel_dev,
                                  const float *wav_dev, float *traces_dev,
                                  void *stream);
 
+/* Upper bound on device memory for wavefield snapshots and checkpoints (no
+ * reference counterpart: the reference keeps every snapshot,
+ * wave_wgpu.rs:180-189).  Batches are processed in model chunks that fit;
+ * grids too large for one workgroup switch to a checkpoint every 100 steps
+ * with segment recompute when a model's snapshots do not fit.  Default:
+ * half the free device memory at engine creation, at most 96 GiB. */
+int32_t wg_set_snapshot_budget(wg_engine *e, uint64_t bytes);
+/* 1 when the engine runs the HBM-streamed kernels (grid too large for the
+ * resident ones), 0 otherwise. */
+int32_t wg_engine_streamed(const wg_engine *e);
+
 /* Kernel timing (no reference counterpart; the reference records no
  * timestamps, wave_wgpu.rs:829).  When enabled, HIP events are recorded on the
  * launch stream around every forward and adjoint kernel launch; wg_timing_read

@@ -94,6 +94,10 @@ struct wg_engine {
     float* d_wavelet = nullptr;
     // workspace, grown on demand
     DevBuf snaps, traces, resid, gunit, partials, scale, energy, vel, obs, loss, grad;
+    // HBM-streamed path (grids too large for one workgroup): step state,
+    // checkpoints every seg_len steps, one segment of snapshots.
+    bool streamed = false;
+    DevBuf fstate, astate, ckpt;
     hipStream_t stream = nullptr;  // host-pointer calls
     hipStream_t last = nullptr;    // stream of the last enqueued call
     bool last_valid = false;
@@ -288,7 +292,11 @@ struct Timed {
     }
 };
 
+int stream_simulate(wg_engine* e, int b0, int nb, const float* vel, const float* wav, long long wav_stride,
+                    float* traces, hipStream_t st);
+
 int run_chunk_forward(wg_engine* e, int b0, int nb, const float* vel, float* traces, bool store, hipStream_t st) {
+    if (e->streamed && !store) return stream_simulate(e, b0, nb, vel, e->d_wavelet, 0, traces, st);
     wg::FwdParams p = fwd_params(e, vel, traces);
     p.unit0 = b0 * e->cfg.shots;
     p.snaps = store ? (float*)e->snaps.p : nullptr;
@@ -305,6 +313,143 @@ int run_chunk_adjoint(wg_engine* e, int b0, int nb, const float* vel, const floa
     p.dwav = dwav;
     Timed tm(e, &e->ev_adj, st);
     LAUNCH(wg::launch_adjoint(p, nb * e->cfg.shots, st));
+    return WG_OK;
+}
+
+
+// ---------------------------------------------------------------------------
+// HBM-streamed path: one launch per time step, state in HBM.
+// ---------------------------------------------------------------------------
+struct StreamPlan {
+    int models;   // models per chunk
+    int seg_len;  // steps per snapshot segment
+    int nseg;     // segments (1: all snapshots kept, no recompute)
+};
+
+constexpr int kCheckpointEvery = 100;  // steps between forward checkpoints when snapshots do not fit
+constexpr int kMaxStepUnits = 65535;   // units of one step launch (grid z dimension)
+
+StreamPlan plan_stream(wg_engine* e, int B, bool need_snaps) {
+    const wg::Grid& g = e->g;
+    const size_t S = e->cfg.shots, cells = (size_t)g.nx * g.nz, inner = g.inner;
+    const size_t state = 18 * cells * sizeof(float);
+    StreamPlan P{1, g.nt, 1};
+    const size_t full = state + (need_snaps ? (size_t)g.nt * inner * sizeof(float) : 0);
+    if (!need_snaps || full * S <= e->snap_budget) {
+        P.models = (int)std::max<size_t>(1, std::min<size_t>(B, e->snap_budget / std::max<size_t>(full * S, 1)));
+        P.models = std::min(P.models, std::max(1, kMaxStepUnits / (int)S));
+        return P;
+    }
+    P.seg_len = std::min(g.nt, kCheckpointEvery);
+    P.nseg = (g.nt + P.seg_len - 1) / P.seg_len;
+    const size_t per = state + (size_t)P.seg_len * inner * sizeof(float) + (size_t)P.nseg * 8 * cells * sizeof(float);
+    P.models = (int)std::max<size_t>(1, std::min<size_t>(B, e->snap_budget / (per * S)));
+    P.models = std::min(P.models, std::max(1, kMaxStepUnits / (int)S));
+    return P;
+}
+
+wg::StepParams step_params(wg_engine* e, const float* vel, int units, int unit0) {
+    wg::StepParams p{};
+    p.g = e->g;
+    p.geo = e->geo();
+    p.vel = vel;
+    p.wavelet = e->d_wavelet;
+    p.wav_stride = 0;
+    p.units = units;
+    p.unit0 = unit0;
+    return p;
+}
+
+int stream_alloc(wg_engine* e, const StreamPlan& P, int units) {
+    const size_t cells = (size_t)e->g.nx * e->g.nz;
+    int rc;
+    if ((rc = ensure(e, e->fstate, 8 * units * cells * sizeof(float)))) return rc;
+    if ((rc = ensure(e, e->astate, 10 * units * cells * sizeof(float)))) return rc;
+    if ((rc = ensure(e, e->snaps, (size_t)units * P.seg_len * e->g.inner * sizeof(float)))) return rc;
+    if (P.nseg > 1 && (rc = ensure(e, e->ckpt, (size_t)P.nseg * 8 * units * cells * sizeof(float)))) return rc;
+    return WG_OK;
+}
+
+// Forward of units [unit0, unit0 + units) over all nt steps from rest:
+// records traces (if non-null), stores all snapshots when P.nseg == 1 and
+// `store`, and checkpoints the state every P.seg_len steps when P.nseg > 1.
+int stream_forward(wg_engine* e, const StreamPlan& P, int units, int unit0, const float* vel, const float* wav,
+                   long long wav_stride, float* traces, bool store, hipStream_t st) {
+    const size_t cells = (size_t)e->g.nx * e->g.nz;
+    const size_t sbytes = 8 * units * cells * sizeof(float);
+    HIPCK(hipMemsetAsync(e->fstate.p, 0, sbytes, st));
+    wg::StepParams p = step_params(e, vel, units, unit0);
+    p.wavelet = wav;
+    p.wav_stride = wav_stride;
+    p.traces = traces;
+    p.snaps = (float*)e->snaps.p;
+    p.snap_t0 = 0;
+    p.seg_len = P.seg_len;
+    p.state = (float*)e->fstate.p;
+    const bool keep = store && P.nseg == 1;
+    for (int t = 0; t < e->g.nt; ++t) {
+        if (store && P.nseg > 1 && t % P.seg_len == 0)
+            HIPCK(hipMemcpyAsync((char*)e->ckpt.p + (size_t)(t / P.seg_len) * sbytes, e->fstate.p, sbytes,
+                                 hipMemcpyDeviceToDevice, st));
+        p.t = t;
+        LAUNCH(wg::launch_fwd_step(p, units, keep, false, st));
+    }
+    return WG_OK;
+}
+
+// Adjoint of units [unit0, unit0 + units) with residual `resid` (all units),
+// gradient accumulated into gunit (zeroed here for these units); segments are
+// recomputed from the checkpoints when P.nseg > 1.
+int stream_adjoint(wg_engine* e, const StreamPlan& P, int units, int unit0, const float* vel, const float* resid,
+                   float* gunit, float* dwav, hipStream_t st) {
+    const wg::Grid& g = e->g;
+    const size_t cells = (size_t)g.nx * g.nz;
+    const size_t sbytes = 8 * units * cells * sizeof(float);
+    HIPCK(hipMemsetAsync(e->astate.p, 0, 10 * units * cells * sizeof(float), st));
+    HIPCK(hipMemsetAsync(gunit + (size_t)unit0 * cells, 0, units * cells * sizeof(float), st));
+    wg::StepParams pf = step_params(e, vel, units, unit0);
+    pf.snaps = (float*)e->snaps.p;
+    pf.seg_len = P.seg_len;
+    pf.state = (float*)e->fstate.p;
+    wg::StepParams pa = step_params(e, vel, units, unit0);
+    pa.snaps = (float*)e->snaps.p;
+    pa.seg_len = P.seg_len;
+    pa.state = (float*)e->astate.p;
+    pa.resid = resid;
+    pa.grad = gunit;
+    pa.dwav = dwav;
+    for (int seg = P.nseg - 1; seg >= 0; --seg) {
+        const int t0 = seg * P.seg_len, t1 = std::min(g.nt, t0 + P.seg_len);
+        if (P.nseg > 1) {
+            HIPCK(hipMemcpyAsync(e->fstate.p, (char*)e->ckpt.p + (size_t)seg * sbytes, sbytes,
+                                 hipMemcpyDeviceToDevice, st));
+            pf.snap_t0 = t0;
+            for (int t = t0; t < t1; ++t) {
+                pf.t = t;
+                LAUNCH(wg::launch_fwd_step(pf, units, true, false, st));
+            }
+        }
+        pa.snap_t0 = P.nseg > 1 ? t0 : 0;
+        for (int t = t1 - 1; t >= t0; --t) {
+            pa.t = t;
+            LAUNCH(wg::launch_adj_step(pa, units, st));
+        }
+    }
+    return WG_OK;
+}
+
+// Non-storing streamed forward of models [b0, b0 + nb), chunked by memory.
+int stream_simulate(wg_engine* e, int b0, int nb, const float* vel, const float* wav, long long wav_stride,
+                    float* traces, hipStream_t st) {
+    const int S = e->cfg.shots;
+    const StreamPlan P = plan_stream(e, nb, false);
+    int rc;
+    if ((rc = stream_alloc(e, P, P.models * S))) return rc;
+    for (int c0 = b0; c0 < b0 + nb; c0 += P.models) {
+        const int n = std::min(P.models, b0 + nb - c0);
+        Timed tm(e, &e->ev_fwd, st);
+        if ((rc = stream_forward(e, P, n * S, c0 * S, vel, wav, wav_stride, traces, false, st))) return rc;
+    }
     return WG_OK;
 }
 
@@ -325,9 +470,29 @@ int loss_grad_core(wg_engine* e, int B, const float* vel, const float* obs, floa
         if ((rc = ensure(e, e->traces, (size_t)B * per * sizeof(float)))) return rc;
         traces = (float*)e->traces.p;
     }
+    if (compute_energy) LAUNCH(wg::launch_energy(obs, per, B, energy, scale, st));
+    if (e->streamed) {
+        const StreamPlan P = plan_stream(e, B, true);
+        if ((rc = stream_alloc(e, P, P.models * S))) return rc;
+        for (int b0 = 0; b0 < B; b0 += P.models) {
+            const int nb = std::min(P.models, B - b0);
+            {
+                Timed tm(e, &e->ev_fwd, st);
+                if ((rc = stream_forward(e, P, nb * S, b0 * S, vel, e->d_wavelet, 0, traces, true, st))) return rc;
+            }
+            LAUNCH(wg::launch_residual(traces + b0 * per, obs + b0 * per, scale + b0, per, nb,
+                                       (float*)e->resid.p + b0 * per, (float*)e->partials.p + b0 * chunks, st));
+            Timed tm(e, &e->ev_adj, st);
+            if ((rc = stream_adjoint(e, P, nb * S, b0 * S, vel, (const float*)e->resid.p, (float*)e->gunit.p,
+                                     nullptr, st)))
+                return rc;
+        }
+        LAUNCH(wg::launch_loss_sum((const float*)e->partials.p, chunks, B, energy, loss, st));
+        LAUNCH(wg::launch_fold(g, (const float*)e->gunit.p, vel, energy, B, grad, st));
+        return WG_OK;
+    }
     const int mc = models_per_chunk(e, B);
     if ((rc = ensure(e, e->snaps, snap_bytes_per_unit(e) * S * mc))) return rc;
-    if (compute_energy) LAUNCH(wg::launch_energy(obs, per, B, energy, scale, st));
     for (int b0 = 0; b0 < B; b0 += mc) {
         const int nb = std::min(mc, B - b0);
         if ((rc = run_chunk_forward(e, b0, nb, vel, traces, true, st))) return rc;
@@ -442,9 +607,12 @@ int32_t wg_engine_create(const wg_config* cfg, wg_engine** out) {
     int lds_limit = 0;
     HIPCK(hipDeviceGetAttribute(&lds_limit, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     std::string why;
-    if (int rc = check_fits(g, lds_limit, why)) return fail(rc, why);
+    const bool resident = check_fits(g, lds_limit, why) == WG_OK;
+    if (!resident && (size_t)g.nx * g.nz > ((size_t)1 << 28))
+        return fail(WG_E_TOO_LARGE, "grid of " + std::to_string((size_t)g.nx * g.nz) + " cells is too large");
 
     wg_engine* e = new wg_engine();
+    e->streamed = !resident;
     e->cfg = c;
     e->g = g;
     e->device = dev;
@@ -506,7 +674,7 @@ void wg_engine_destroy(wg_engine* e) {
                     (void*)e->d_wavelet})
         if (p) (void)hipFree(p);
     for (DevBuf* b : {&e->snaps, &e->traces, &e->resid, &e->gunit, &e->partials, &e->scale, &e->energy, &e->vel,
-                      &e->obs, &e->loss, &e->grad})
+                      &e->obs, &e->loss, &e->grad, &e->fstate, &e->astate, &e->ckpt})
         if (b->p) (void)hipFree(b->p);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
@@ -660,6 +828,18 @@ int32_t wg_adjoint_apply_dev(wg_engine* e, int32_t B, const float* vel_dev, cons
         traces = (float*)e->traces.p;
     }
     if ((rc = ensure(e, e->gunit, (size_t)B * S * g.nx * g.nz * sizeof(float)))) return rc;
+    if (e->streamed) {
+        const StreamPlan P = plan_stream(e, B, true);
+        if ((rc = stream_alloc(e, P, P.models * S))) return rc;
+        for (int b0 = 0; b0 < B; b0 += P.models) {
+            const int nb = std::min(P.models, B - b0);
+            if ((rc = stream_forward(e, P, nb * S, b0 * S, vel_dev, e->d_wavelet, 0, traces, true, st))) return rc;
+            if ((rc = stream_adjoint(e, P, nb * S, b0 * S, vel_dev, resid_dev, (float*)e->gunit.p, dwav_dev, st)))
+                return rc;
+        }
+        if (grad_dev) LAUNCH(wg::launch_fold(g, (const float*)e->gunit.p, vel_dev, nullptr, B, grad_dev, st));
+        return WG_OK;
+    }
     const int mc = models_per_chunk(e, B);
     if ((rc = ensure(e, e->snaps, snap_bytes_per_unit(e) * S * mc))) return rc;
     for (int b0 = 0; b0 < B; b0 += mc) {
@@ -684,6 +864,29 @@ int32_t wg_born_dev(wg_engine* e, int32_t B, const float* vel_dev, const float* 
     const int S = e->cfg.shots;
     const size_t per = (size_t)S * g.nt * g.nrec;
     if ((rc = ensure(e, e->traces, B * per * sizeof(float)))) return rc;
+    if (e->streamed) {
+        const StreamPlan P = plan_stream(e, B, true);
+        if (P.nseg > 1) return fail(WG_E_TOO_LARGE, "the Born operator needs every snapshot of a model in memory");
+        if ((rc = stream_alloc(e, P, P.models * S))) return rc;
+        const size_t cells = (size_t)g.nx * g.nz;
+        for (int b0 = 0; b0 < B; b0 += P.models) {
+            const int nb = std::min(P.models, B - b0);
+            if ((rc = stream_forward(e, P, nb * S, b0 * S, vel_dev, e->d_wavelet, 0, (float*)e->traces.p, true, st)))
+                return rc;
+            HIPCK(hipMemsetAsync(e->astate.p, 0, 8 * nb * S * cells * sizeof(float), st));
+            wg::StepParams p = step_params(e, vel_dev, nb * S, b0 * S);
+            p.traces = dtraces_dev;
+            p.seg_len = P.seg_len;
+            p.state = (float*)e->astate.p;
+            p.dvel = dvel_dev;
+            p.born_snaps = (const float*)e->snaps.p;
+            for (int t = 0; t < g.nt; ++t) {
+                p.t = t;
+                LAUNCH(wg::launch_fwd_step(p, nb * S, false, true, st));
+            }
+        }
+        return WG_OK;
+    }
     const int mc = models_per_chunk(e, B);
     if ((rc = ensure(e, e->snaps, snap_bytes_per_unit(e) * S * mc))) return rc;
     for (int b0 = 0; b0 < B; b0 += mc) {
@@ -707,12 +910,22 @@ int32_t wg_simulate_wavelets_dev(wg_engine* e, int32_t B, const float* vel_dev, 
     int rc;
     if ((rc = fence(e, st))) return rc;
     if (B == 0) return WG_OK;
+    if (e->streamed) return stream_simulate(e, 0, B, vel_dev, wav_dev, e->g.nt, traces_dev, st);
     wg::FwdParams p = fwd_params(e, vel_dev, traces_dev);
     p.wavelet = wav_dev;
     p.wav_stride = e->g.nt;
     LAUNCH(wg::launch_forward(p, B * e->cfg.shots, false, false, st));
     return WG_OK;
 }
+
+int32_t wg_set_snapshot_budget(wg_engine* e, uint64_t bytes) {
+    if (!e || bytes == 0) return fail(WG_E_INVALID, "budget must be positive");
+    Guard G(e);
+    e->snap_budget = (size_t)bytes;
+    return WG_OK;
+}
+
+int32_t wg_engine_streamed(const wg_engine* e) { return e ? (e->streamed ? 1 : 0) : -1; }
 
 int32_t wg_timing_enable(wg_engine* e, int32_t on) {
     if (!e) return fail(WG_E_INVALID, "null engine");

@@ -65,6 +65,31 @@ struct AdjParams {
     int unit0;
 };
 
+// One time step of the HBM-streamed kernels (stream_kernels.hip) for the units
+// [unit0, unit0 + units) of a chunk.  state holds the chunk's arrays, each
+// [units, nx * nz]: forward u0 u1 psix0 psix1 psiz0 psiz1 zetax zetaz; adjoint
+// lambda0 lambda1 Px0 Px1 Zx0 Zx1 Pz0 Pz1 Zz0 Zz1.
+struct StepParams {
+    Grid g;
+    Geometry geo;
+    const float* vel;        // [B, nvz, nvx]
+    const float* wavelet;    // [units or 1, nt]
+    long long wav_stride;
+    float* traces;           // forward: [all units, nt, nrec] or null (no recording)
+    float* snaps;            // [units, seg_len, inner]: written (forward STORE) or read (adjoint)
+    int snap_t0, seg_len;    // snapshot of step t is entry t - snap_t0
+    float* state;
+    int units, unit0, t;
+    const float* dvel;       // Born
+    const float* born_snaps; // Born: base snapshots, same layout as snaps
+    const float* resid;      // adjoint: [all units, nt, nrec]
+    float* grad;             // adjoint: [all units, nx * nz], accumulated
+    float* dwav;             // adjoint: [all units, nt] or null
+};
+
+hipError_t launch_fwd_step(const StepParams& p, int units, bool store, bool born, hipStream_t st);
+hipError_t launch_adj_step(const StepParams& p, int units, hipStream_t st);
+
 // Returns hipSuccess or the launch error.  lds_bytes_* report the dynamic
 // LDS a launch needs (0 when the grid does not fit one workgroup).
 size_t forward_lds_bytes(const Grid& g);

@@ -119,6 +119,15 @@ class PyWaveEngine:
     def nvz(self) -> int:
         return self._nvz
 
+    @property
+    def streamed(self) -> bool:
+        """True when the grid runs on the HBM-streamed kernels."""
+        return self._lib.wg_engine_streamed(self._h) == 1
+
+    def set_snapshot_budget(self, nbytes: int) -> None:
+        """Device-memory bound for snapshots and checkpoints."""
+        _lib.check(self._lib.wg_set_snapshot_budget(self._h, int(nbytes)))
+
     def dims(self) -> tuple[int, int, int]:
         nx, nz, pad = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
         _lib.check(self._lib.wg_engine_dims(self._h, ctypes.byref(nx), ctypes.byref(nz), ctypes.byref(pad)))
