@@ -144,9 +144,11 @@ int32_t wg_engine_streamed(const wg_engine *e);
 
 /* Kernel timing (no reference counterpart; the reference records no
  * timestamps, wave_wgpu.rs:829).  When enabled, HIP events are recorded on the
- * launch stream around every forward and adjoint kernel launch; wg_timing_read
- * waits for them and returns the summed durations (ms) and launch counts since
- * the last read, then clears them. */
+ * launch stream around forward and adjoint kernel launches (around each launch
+ * of the resident kernels; around each run of per-step launches of the
+ * HBM-streamed kernels, a checkpoint recompute counting as forward);
+ * wg_timing_read waits for them and returns the summed durations (ms) and
+ * kernel launch counts since the last read, then clears them. */
 int32_t wg_timing_enable(wg_engine *e, int32_t on);
 int32_t wg_timing_read(wg_engine *e, double *fwd_ms, int64_t *fwd_launches,
                        double *adj_ms, int64_t *adj_launches);

@@ -74,6 +74,11 @@ void corners(float x, float z, float h, int pad, int nx, int nz, int cell[4], fl
 
 }  // namespace
 
+struct TimedSpan {
+    hipEvent_t a, b;
+    int64_t n;  // kernel launches between a and b
+};
+
 struct wg_engine {
     wg_config cfg{};
     wg::Grid g{};
@@ -103,9 +108,9 @@ struct wg_engine {
     bool last_valid = false;
     size_t snap_budget = 0;
     std::mutex mu;
-    // kernel timing: (start, stop) event pairs of forward / adjoint launches
+    // kernel timing: (start, stop, launches) records of forward / adjoint launches
     bool timing = false;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_fwd, ev_adj;
+    std::vector<TimedSpan> ev_fwd, ev_adj;
 
     wg::Geometry geo() const {
         wg::Geometry G;
@@ -271,13 +276,16 @@ int models_per_chunk(wg_engine* e, int B) {
 
 // Forward (+ snapshots) of models [b0, b0 + nb) into traces, then optionally
 // the adjoint with residual `resid` ([B*S, nt, R] absolute) into gunit.
-// Records a (start, stop) event pair around a launch when timing is enabled.
+// Records a (start, stop) event pair around `n` back-to-back kernel launches
+// when timing is enabled.
 struct Timed {
     wg_engine* e;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>>* list;
+    std::vector<TimedSpan>* list;
     hipStream_t st;
+    int64_t n;
     hipEvent_t a = nullptr, b = nullptr;
-    Timed(wg_engine* en, std::vector<std::pair<hipEvent_t, hipEvent_t>>* l, hipStream_t s) : e(en), list(l), st(s) {
+    Timed(wg_engine* en, std::vector<TimedSpan>* l, hipStream_t s, int64_t launches = 1)
+        : e(en), list(l), st(s), n(launches) {
         if (!e->timing) return;
         if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
             e->timing = false;
@@ -288,7 +296,7 @@ struct Timed {
     ~Timed() {
         if (!a || !b) return;
         (void)hipEventRecord(b, st);
-        list->emplace_back(a, b);
+        list->push_back(TimedSpan{a, b, n});
     }
 };
 
@@ -324,6 +332,7 @@ struct StreamPlan {
     int models;   // models per chunk
     int seg_len;  // steps per snapshot segment
     int nseg;     // segments (1: all snapshots kept, no recompute)
+    bool snaps;   // snapshots and adjoint state are needed
 };
 
 constexpr int kCheckpointEvery = 100;  // steps between forward checkpoints when snapshots do not fit
@@ -333,7 +342,7 @@ StreamPlan plan_stream(wg_engine* e, int B, bool need_snaps) {
     const wg::Grid& g = e->g;
     const size_t S = e->cfg.shots, cells = (size_t)g.nx * g.nz, inner = g.inner;
     const size_t state = 18 * cells * sizeof(float);
-    StreamPlan P{1, g.nt, 1};
+    StreamPlan P{1, g.nt, 1, need_snaps};
     const size_t full = state + (need_snaps ? (size_t)g.nt * inner * sizeof(float) : 0);
     if (!need_snaps || full * S <= e->snap_budget) {
         P.models = (int)std::max<size_t>(1, std::min<size_t>(B, e->snap_budget / std::max<size_t>(full * S, 1)));
@@ -364,6 +373,7 @@ int stream_alloc(wg_engine* e, const StreamPlan& P, int units) {
     const size_t cells = (size_t)e->g.nx * e->g.nz;
     int rc;
     if ((rc = ensure(e, e->fstate, 8 * units * cells * sizeof(float)))) return rc;
+    if (!P.snaps) return WG_OK;
     if ((rc = ensure(e, e->astate, 10 * units * cells * sizeof(float)))) return rc;
     if ((rc = ensure(e, e->snaps, (size_t)units * P.seg_len * e->g.inner * sizeof(float)))) return rc;
     if (P.nseg > 1 && (rc = ensure(e, e->ckpt, (size_t)P.nseg * 8 * units * cells * sizeof(float)))) return rc;
@@ -387,6 +397,7 @@ int stream_forward(wg_engine* e, const StreamPlan& P, int units, int unit0, cons
     p.seg_len = P.seg_len;
     p.state = (float*)e->fstate.p;
     const bool keep = store && P.nseg == 1;
+    Timed tm(e, &e->ev_fwd, st, e->g.nt);
     for (int t = 0; t < e->g.nt; ++t) {
         if (store && P.nseg > 1 && t % P.seg_len == 0)
             HIPCK(hipMemcpyAsync((char*)e->ckpt.p + (size_t)(t / P.seg_len) * sbytes, e->fstate.p, sbytes,
@@ -424,12 +435,14 @@ int stream_adjoint(wg_engine* e, const StreamPlan& P, int units, int unit0, cons
             HIPCK(hipMemcpyAsync(e->fstate.p, (char*)e->ckpt.p + (size_t)seg * sbytes, sbytes,
                                  hipMemcpyDeviceToDevice, st));
             pf.snap_t0 = t0;
+            Timed tm(e, &e->ev_fwd, st, t1 - t0);
             for (int t = t0; t < t1; ++t) {
                 pf.t = t;
                 LAUNCH(wg::launch_fwd_step(pf, units, true, false, st));
             }
         }
         pa.snap_t0 = P.nseg > 1 ? t0 : 0;
+        Timed tm(e, &e->ev_adj, st, t1 - t0);
         for (int t = t1 - 1; t >= t0; --t) {
             pa.t = t;
             LAUNCH(wg::launch_adj_step(pa, units, st));
@@ -447,7 +460,6 @@ int stream_simulate(wg_engine* e, int b0, int nb, const float* vel, const float*
     if ((rc = stream_alloc(e, P, P.models * S))) return rc;
     for (int c0 = b0; c0 < b0 + nb; c0 += P.models) {
         const int n = std::min(P.models, b0 + nb - c0);
-        Timed tm(e, &e->ev_fwd, st);
         if ((rc = stream_forward(e, P, n * S, c0 * S, vel, wav, wav_stride, traces, false, st))) return rc;
     }
     return WG_OK;
@@ -476,13 +488,9 @@ int loss_grad_core(wg_engine* e, int B, const float* vel, const float* obs, floa
         if ((rc = stream_alloc(e, P, P.models * S))) return rc;
         for (int b0 = 0; b0 < B; b0 += P.models) {
             const int nb = std::min(P.models, B - b0);
-            {
-                Timed tm(e, &e->ev_fwd, st);
-                if ((rc = stream_forward(e, P, nb * S, b0 * S, vel, e->d_wavelet, 0, traces, true, st))) return rc;
-            }
+            if ((rc = stream_forward(e, P, nb * S, b0 * S, vel, e->d_wavelet, 0, traces, true, st))) return rc;
             LAUNCH(wg::launch_residual(traces + b0 * per, obs + b0 * per, scale + b0, per, nb,
                                        (float*)e->resid.p + b0 * per, (float*)e->partials.p + b0 * chunks, st));
-            Timed tm(e, &e->ev_adj, st);
             if ((rc = stream_adjoint(e, P, nb * S, b0 * S, vel, (const float*)e->resid.p, (float*)e->gunit.p,
                                      nullptr, st)))
                 return rc;
@@ -666,8 +674,8 @@ void wg_engine_destroy(wg_engine* e) {
     if (e->last_valid) (void)hipStreamSynchronize(e->last);
     for (auto* l : {&e->ev_fwd, &e->ev_adj})
         for (auto& pr : *l) {
-            (void)hipEventDestroy(pr.first);
-            (void)hipEventDestroy(pr.second);
+            (void)hipEventDestroy(pr.a);
+            (void)hipEventDestroy(pr.b);
         }
     for (void* p : {(void*)e->d_prof, (void*)e->d_src_cell, (void*)e->d_src_w, (void*)e->d_rec_cell,
                     (void*)e->d_rec_w, (void*)e->d_band_off, (void*)e->d_ent_r, (void*)e->d_ent_w,
@@ -943,13 +951,13 @@ int32_t wg_timing_read(wg_engine* e, double* fwd_ms, int64_t* fwd_n, double* adj
     int which = 0;
     for (auto* l : {&e->ev_fwd, &e->ev_adj}) {
         for (auto& pr : *l) {
-            HIPCK(hipEventSynchronize(pr.second));
+            HIPCK(hipEventSynchronize(pr.b));
             float ms = 0.0f;
-            HIPCK(hipEventElapsedTime(&ms, pr.first, pr.second));
+            HIPCK(hipEventElapsedTime(&ms, pr.a, pr.b));
             sums[which] += ms;
-            counts[which] += 1;
-            (void)hipEventDestroy(pr.first);
-            (void)hipEventDestroy(pr.second);
+            counts[which] += pr.n;
+            (void)hipEventDestroy(pr.a);
+            (void)hipEventDestroy(pr.b);
         }
         l->clear();
         ++which;

@@ -6,16 +6,21 @@
 // (double-buffered, neighbours read it) and zeta (own cell only, in place);
 // adjoint lambda (double-buffered likewise), P and Z (double-buffered by the
 // step parity, as splatAdjoint.wgsl:70-73) and the gradient.  A 256-thread
-// workgroup computes a 32 x 8 tile of inner cells; the neighbour values it
+// workgroup computes a 64 x 16 tile of inner cells, four cells per thread
+// (rows lz, lz + 4, lz + 8, lz + 12 of column lx); the neighbour values it
 // needs (u or V = v^2 dt^2 lambda, a*psi, T, U1) are staged in LDS with a
-// 2-cell halo.  Per-cell arithmetic is stencil.h's, in the reference's order.
+// 2-cell halo.  Tiles with no C-PML cell along an axis skip that axis's strips
+// and memory variables (they are zero there).  Per-cell arithmetic is
+// stencil.h's, in the reference's order.
 #include "stencil.h"
 #include "wave_kernels.h"
 
 namespace wg {
 namespace st {
 
-constexpr int TX = 32, TZ = 8, NT = TX * TZ;
+constexpr int TX = 64, TZ = 16, NT = 256;
+constexpr int RZ = NT / TX;   // rows per pass
+constexpr int CPT = TZ / RZ;  // cells per thread
 constexpr int HX = TX + 4, HZ = TZ + 4;
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
@@ -78,81 +83,89 @@ __global__ __launch_bounds__(NT) void k_fwd_step(StepParams P) {
         const int x = xi0 + (j % HX), z = zi0 + (j / HX);  // padded coords: halo starts at inner - 2
         sU[j] = (x < g.nx && z < g.nz) ? Ucur[z * g.nx + x] : 0.0f;
     }
-    for (int j = tid; j < TZ * HX; j += NT) {
-        const int x = xi0 + (j % HX), z = zi0 + 2 + (j / HX);
-        sAX[j] = (x < g.nx && z < g.nz) ? pax[x] * PXc[z * g.nx + x] : 0.0f;
-    }
-    for (int j = tid; j < HZ * TX; j += NT) {
-        const int x = xi0 + 2 + (j % TX), z = zi0 + (j / TX);
-        sAZ[j] = (x < g.nx && z < g.nz) ? paz[z] * PZc[z * g.nx + x] : 0.0f;
-    }
+    // does the tile hold C-PML cells along x / z (padded coords [xi0+2, xi0+TX+2))?
+    const bool tpx = xi0 + 2 < g.region || xi0 + TX + 2 > g.nx - g.region;
+    const bool tpz = zi0 + 2 < g.region || zi0 + TZ + 2 > g.nz - g.region;
+    if (tpx)
+        for (int j = tid; j < TZ * HX; j += NT) {
+            const int x = xi0 + (j % HX), z = zi0 + 2 + (j / HX);
+            sAX[j] = (x < g.nx && z < g.nz) ? pax[x] * PXc[z * g.nx + x] : 0.0f;
+        }
+    if (tpz)
+        for (int j = tid; j < HZ * TX; j += NT) {
+            const int x = xi0 + 2 + (j % TX), z = zi0 + (j / TX);
+            sAZ[j] = (x < g.nx && z < g.nz) ? paz[z] * PZc[z * g.nx + x] : 0.0f;
+        }
     __syncthreads();
-    const int lx = tid % TX, lz = tid / TX;
-    const int x = xi0 + lx + 2, z = zi0 + lz + 2;
-    if (x >= g.nx - 2 || z >= g.nz - 2) return;
-    const int i = z * g.nx + x;
-    const int li = (lz + 2) * HX + (lx + 2);
-    const float u = sU[li];
-    const bool px = x < g.region || x >= g.nx - g.region;
-    const bool pz = z < g.region || z >= g.nz - g.region;
-    Pml mx{0, 0, 0, 0}, mz{0, 0, 0, 0};
-    float psx = 0.0f, zex = 0.0f, psz = 0.0f, zez = 0.0f;
-    if (px) {
-        const float bb = pbx[x];
-        mx = Pml{pax[x], 1.0f + bb, bb, pdbx[x]};
-        psx = PXc[i];
-        zex = ZX[i];
-    }
-    if (pz) {
-        const float bb = pbz[z];
-        mz = Pml{paz[z], 1.0f + bb, bb, pdbz[z]};
-        psz = PZc[i];
-        zez = ZZ[i];
-    }
-    const float* AXc = sAX + lz * HX + (lx + 2);
-    const float* AZc = sAZ + (lz + 2) * TX + lx;
-    float w;
-    if (px && pz)
-        w = fwd_w<true, true>(sU, li, HX, u, g.rh, g.rh2, AXc, mx, psx, zex, AZc, TX, mz, psz, zez);
-    else if (px)
-        w = fwd_w<true, false>(sU, li, HX, u, g.rh, g.rh2, AXc, mx, psx, zex, AZc, TX, mz, psz, zez);
-    else if (pz)
-        w = fwd_w<false, true>(sU, li, HX, u, g.rh, g.rh2, AXc, mx, psx, zex, AZc, TX, mz, psz, zez);
-    else
-        w = fwd_w<false, false>(sU, li, HX, u, g.rh, g.rh2, AXc, mx, psx, zex, AZc, TX, mz, psz, zez);
-    const float vr = vis_raw(vvis, g, x, z);
-    const float v = clampv(vr, g.vmin, g.vmax);
-    const size_t q = (size_t)(z - 2) * g.ni + (x - 2);
-    if (STORE) P.snaps[((size_t)ul * P.seg_len + (t - P.snap_t0)) * g.inner + q] = 2.0f * v * g.dt2 * w;
-    float next = v * v * g.dt2 * w + 2.0f * u - Unxt[i];
-    if (BORN) {
-        const float m = (vr > g.vmin && vr < g.vmax) ? 1.0f : 0.0f;
-        next += (vis_raw(P.dvel + (size_t)b * g.nvx * g.nvz, g, x, z) * m) *
-                P.born_snaps[((size_t)ul * P.seg_len + (t - P.snap_t0)) * g.inner + q];
-    }
-    // point source after the update (lib.rs:209-215); Born: its derivative
+    const int lx = tid % TX;
     const float f_t = P.wavelet[(size_t)unit * P.wav_stride + t];
+    for (int k = 0; k < CPT; ++k) {
+        const int lz = tid / TX + RZ * k;
+        const int x = xi0 + lx + 2, z = zi0 + lz + 2;
+        if (x >= g.nx - 2 || z >= g.nz - 2) continue;
+        const int i = z * g.nx + x;
+        const int li = (lz + 2) * HX + (lx + 2);
+        const float u = sU[li];
+        const bool px = x < g.region || x >= g.nx - g.region;
+        const bool pz = z < g.region || z >= g.nz - g.region;
+        Pml mx{0, 0, 0, 0}, mz{0, 0, 0, 0};
+        float psx = 0.0f, zex = 0.0f, psz = 0.0f, zez = 0.0f;
+        if (px) {
+            const float bb = pbx[x];
+            mx = Pml{pax[x], 1.0f + bb, bb, pdbx[x]};
+            psx = PXc[i];
+            zex = ZX[i];
+        }
+        if (pz) {
+            const float bb = pbz[z];
+            mz = Pml{paz[z], 1.0f + bb, bb, pdbz[z]};
+            psz = PZc[i];
+            zez = ZZ[i];
+        }
+        const float* AXc = sAX + lz * HX + (lx + 2);
+        const float* AZc = sAZ + (lz + 2) * TX + lx;
+        float w;
+        if (px && pz)
+            w = fwd_w<true, true>(sU, li, HX, u, g.rh, g.rh2, AXc, mx, psx, zex, AZc, TX, mz, psz, zez);
+        else if (px)
+            w = fwd_w<true, false>(sU, li, HX, u, g.rh, g.rh2, AXc, mx, psx, zex, AZc, TX, mz, psz, zez);
+        else if (pz)
+            w = fwd_w<false, true>(sU, li, HX, u, g.rh, g.rh2, AXc, mx, psx, zex, AZc, TX, mz, psz, zez);
+        else
+            w = fwd_w<false, false>(sU, li, HX, u, g.rh, g.rh2, AXc, mx, psx, zex, AZc, TX, mz, psz, zez);
+        const float vr = vis_raw(vvis, g, x, z);
+        const float v = clampv(vr, g.vmin, g.vmax);
+        const size_t q = (size_t)(z - 2) * g.ni + (x - 2);
+        if (STORE) P.snaps[((size_t)ul * P.seg_len + (t - P.snap_t0)) * g.inner + q] = 2.0f * v * g.dt2 * w;
+        float next = v * v * g.dt2 * w + 2.0f * u - Unxt[i];
+        if (BORN) {
+            const float m = (vr > g.vmin && vr < g.vmax) ? 1.0f : 0.0f;
+            next += (vis_raw(P.dvel + (size_t)b * g.nvx * g.nvz, g, x, z) * m) *
+                    P.born_snaps[((size_t)ul * P.seg_len + (t - P.snap_t0)) * g.inner + q];
+        }
+        // point source after the update (lib.rs:209-215); Born: its derivative
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        const float wc = P.geo.src_w[s * 4 + c];
-        if (P.geo.src_cell[s * 4 + c] == i && wc != 0.0f) {
-            if (BORN) {
-                const float m = (vr > g.vmin && vr < g.vmax) ? 1.0f : 0.0f;
-                const float dv = vis_raw(P.dvel + (size_t)b * g.nvx * g.nvz, g, x, z) * m;
-                next += -(2.0f * v * dv * g.dt2) * wc * f_t;
-            } else {
-                next += -(v * v * g.dt2) * wc * f_t;
+        for (int c = 0; c < 4; ++c) {
+            const float wc = P.geo.src_w[s * 4 + c];
+            if (P.geo.src_cell[s * 4 + c] == i && wc != 0.0f) {
+                if (BORN) {
+                    const float m = (vr > g.vmin && vr < g.vmax) ? 1.0f : 0.0f;
+                    const float dv = vis_raw(P.dvel + (size_t)b * g.nvx * g.nvz, g, x, z) * m;
+                    next += -(2.0f * v * dv * g.dt2) * wc * f_t;
+                } else {
+                    next += -(v * v * g.dt2) * wc * f_t;
+                }
             }
         }
-    }
-    Unxt[i] = next;
-    if (px) {
-        PXn[i] = psx;
-        ZX[i] = zex;
-    }
-    if (pz) {
-        PZn[i] = psz;
-        ZZ[i] = zez;
+        Unxt[i] = next;
+        if (px) {
+            PXn[i] = psx;
+            ZX[i] = zex;
+        }
+        if (pz) {
+            PZn[i] = psz;
+            ZZ[i] = zez;
+        }
     }
 }
 
@@ -216,77 +229,85 @@ __global__ __launch_bounds__(NT) void k_adj_step(StepParams P) {
     }
     __syncthreads();
     // T = (1+b)V + bZ, U1 = db T + bP of the x-neighbours (x halo) and z-neighbours
-    for (int j = tid; j < TZ * HX; j += NT) {
-        const int hx = j % HX, hz = j / HX;
-        const int x = xi0 + hx, z = zi0 + 2 + hz;
-        float T = 0.0f, U1 = 0.0f;
-        if (x < g.nx && z < g.nz) {
-            const int i = z * g.nx + x;
-            const float bb = pbx[x];
-            T = (1.0f + bb) * sV[(hz + 2) * HX + hx] + bb * ZXr[i];
-            U1 = pdbx[x] * T + bb * PXr[i];
+    // does the tile hold cells of the adjoint C-PML region along x / z?
+    const bool tax = xi0 + 2 < g.region_adj || xi0 + TX + 2 > g.nx - g.region_adj;
+    const bool taz = zi0 + 2 < g.region_adj || zi0 + TZ + 2 > g.nz - g.region_adj;
+    if (tax)
+        for (int j = tid; j < TZ * HX; j += NT) {
+            const int hx = j % HX, hz = j / HX;
+            const int x = xi0 + hx, z = zi0 + 2 + hz;
+            float T = 0.0f, U1 = 0.0f;
+            if (x < g.nx && z < g.nz) {
+                const int i = z * g.nx + x;
+                const float bb = pbx[x];
+                T = (1.0f + bb) * sV[(hz + 2) * HX + hx] + bb * ZXr[i];
+                U1 = pdbx[x] * T + bb * PXr[i];
+            }
+            sTX[j] = T;
+            sUX[j] = U1;
         }
-        sTX[j] = T;
-        sUX[j] = U1;
-    }
-    for (int j = tid; j < HZ * TX; j += NT) {
-        const int hx = j % TX, hz = j / TX;
-        const int x = xi0 + 2 + hx, z = zi0 + hz;
-        float T = 0.0f, U1 = 0.0f;
-        if (x < g.nx && z < g.nz) {
-            const int i = z * g.nx + x;
-            const float bb = pbz[z];
-            T = (1.0f + bb) * sV[hz * HX + hx + 2] + bb * ZZr[i];
-            U1 = pdbz[z] * T + bb * PZr[i];
+    if (taz)
+        for (int j = tid; j < HZ * TX; j += NT) {
+            const int hx = j % TX, hz = j / TX;
+            const int x = xi0 + 2 + hx, z = zi0 + hz;
+            float T = 0.0f, U1 = 0.0f;
+            if (x < g.nx && z < g.nz) {
+                const int i = z * g.nx + x;
+                const float bb = pbz[z];
+                T = (1.0f + bb) * sV[hz * HX + hx + 2] + bb * ZZr[i];
+                U1 = pdbz[z] * T + bb * PZr[i];
+            }
+            sTZ[j] = T;
+            sUZ[j] = U1;
         }
-        sTZ[j] = T;
-        sUZ[j] = U1;
-    }
     __syncthreads();
-    const int lx = tid % TX, lz = tid / TX;
-    const int x = xi0 + lx + 2, z = zi0 + lz + 2;
-    if (x >= g.nx - 2 || z >= g.nz - 2) return;
-    const int i = z * g.nx + x;
-    const int li = (lz + 2) * HX + (lx + 2);
-    const float v0 = sV[li];
-    const float rh = g.rh;
-    float w = 0.0f;
-    if (x < g.region_adj || x >= g.nx - g.region_adj) {
-        const float opb[5] = {1.0f + pbx[x - 2], 1.0f + pbx[x - 1], 1.0f + pbx[x], 1.0f + pbx[x + 1], 1.0f + pbx[x + 2]};
-        float Pv = PXr[i], Zv = ZXr[i];
-        w += adj_axis_pml(sTX + lz * HX + (lx + 2), sUX + lz * HX + (lx + 2), 1, opb, pax[x], v0, rh, Pv, Zv);
-        PXw[i] = Pv;
-        ZXw[i] = Zv;
-    } else {
-        w += adj_axis_int(sV, li, 1, v0, rh);
-    }
-    if (z < g.region_adj || z >= g.nz - g.region_adj) {
-        const float opb[5] = {1.0f + pbz[z - 2], 1.0f + pbz[z - 1], 1.0f + pbz[z], 1.0f + pbz[z + 1], 1.0f + pbz[z + 2]};
-        float Pv = PZr[i], Zv = ZZr[i];
-        w += adj_axis_pml(sTZ + (lz + 2) * TX + lx, sUZ + (lz + 2) * TX + lx, TX, opb, paz[z], v0, rh, Pv, Zv);
-        PZw[i] = Pv;
-        ZZw[i] = Zv;
-    } else {
-        w += adj_axis_int(sV, li, HX, v0, rh);
-    }
-    const float lam = Lcur[i];
-    float next = w + 2.0f * lam - Lnxt[i];
-    const size_t qq = (size_t)(z - 2) * g.ni + (x - 2);
-    float gk = lam * P.snaps[((size_t)ul * P.seg_len + (t - P.snap_t0)) * g.inner + qq];
-    if (z >= P.geo.band_z0 && z < P.geo.band_z0 + P.geo.band_rows) {
-        const int o = (z - P.geo.band_z0) * g.nx + x;
-        const float* res = P.resid + ((size_t)unit * g.nt + t) * g.nrec;
-        for (int e = P.geo.band_off[o]; e < P.geo.band_off[o + 1]; ++e) next += P.geo.ent_w[e] * res[P.geo.ent_r[e]];
-    }
+    const int lx = tid % TX;
     const float f_t = P.wavelet[(size_t)unit * P.wav_stride + t];
-#pragma unroll
-    for (int c = 0; c < 4; ++c)
-        if (P.geo.src_cell[s * 4 + c] == i) {
-            const float v = vel_at(vvis, g, x, z);
-            gk += lam * (-2.0f * v * g.dt2 * P.geo.src_w[s * 4 + c] * f_t);
+    const float* res = P.resid + ((size_t)unit * g.nt + t) * g.nrec;
+    for (int k = 0; k < CPT; ++k) {
+        const int lz = tid / TX + RZ * k;
+        const int x = xi0 + lx + 2, z = zi0 + lz + 2;
+        if (x >= g.nx - 2 || z >= g.nz - 2) continue;
+        const int i = z * g.nx + x;
+        const int li = (lz + 2) * HX + (lx + 2);
+        const float v0 = sV[li];
+        const float rh = g.rh;
+        float w = 0.0f;
+        if (x < g.region_adj || x >= g.nx - g.region_adj) {
+            const float opb[5] = {1.0f + pbx[x - 2], 1.0f + pbx[x - 1], 1.0f + pbx[x], 1.0f + pbx[x + 1], 1.0f + pbx[x + 2]};
+            float Pv = PXr[i], Zv = ZXr[i];
+            w += adj_axis_pml(sTX + lz * HX + (lx + 2), sUX + lz * HX + (lx + 2), 1, opb, pax[x], v0, rh, Pv, Zv);
+            PXw[i] = Pv;
+            ZXw[i] = Zv;
+        } else {
+            w += adj_axis_int(sV, li, 1, v0, rh);
         }
-    grad[i] += gk;
-    Lnxt[i] = next;
+        if (z < g.region_adj || z >= g.nz - g.region_adj) {
+            const float opb[5] = {1.0f + pbz[z - 2], 1.0f + pbz[z - 1], 1.0f + pbz[z], 1.0f + pbz[z + 1], 1.0f + pbz[z + 2]};
+            float Pv = PZr[i], Zv = ZZr[i];
+            w += adj_axis_pml(sTZ + (lz + 2) * TX + lx, sUZ + (lz + 2) * TX + lx, TX, opb, paz[z], v0, rh, Pv, Zv);
+            PZw[i] = Pv;
+            ZZw[i] = Zv;
+        } else {
+            w += adj_axis_int(sV, li, HX, v0, rh);
+        }
+        const float lam = Lcur[i];
+        float next = w + 2.0f * lam - Lnxt[i];
+        const size_t qq = (size_t)(z - 2) * g.ni + (x - 2);
+        float gk = lam * P.snaps[((size_t)ul * P.seg_len + (t - P.snap_t0)) * g.inner + qq];
+        if (z >= P.geo.band_z0 && z < P.geo.band_z0 + P.geo.band_rows) {
+            const int o = (z - P.geo.band_z0) * g.nx + x;
+            for (int e = P.geo.band_off[o]; e < P.geo.band_off[o + 1]; ++e) next += P.geo.ent_w[e] * res[P.geo.ent_r[e]];
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+            if (P.geo.src_cell[s * 4 + c] == i) {
+                const float v = vel_at(vvis, g, x, z);
+                gk += lam * (-2.0f * v * g.dt2 * P.geo.src_w[s * 4 + c] * f_t);
+            }
+        grad[i] += gk;
+        Lnxt[i] = next;
+    }
 }
 
 }  // namespace st

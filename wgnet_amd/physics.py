@@ -31,9 +31,11 @@ DECIMATE = 8
 NT_DEC = (NT - 1) // DECIMATE + 1
 
 
-def make_engine(shots: int = SHOTS, nt: int = NT, dt: float = DT, peak_hz: float = PEAK_HZ) -> PyWaveEngine:
-    """data.py:32-33 (dt and peak_hz are parameters here; defaults unchanged)."""
-    return PyWaveEngine(shots, nt, dt, peak_hz, 20, 1000.0, 5000.0)
+def make_engine(shots: int = SHOTS, nt: int = NT, dt: float = DT, peak_hz: float = PEAK_HZ, *, nvx: int = N,
+                nvz: int = N, h: float = H) -> PyWaveEngine:
+    """data.py:32-33 (dt, peak_hz and the grid are parameters here; defaults
+    unchanged)."""
+    return PyWaveEngine(shots, nt, dt, peak_hz, 20, 1000.0, 5000.0, nvx=nvx, nvz=nvz, h=h)
 
 
 def source_positions_m(shots: int = SHOTS) -> np.ndarray:
@@ -90,8 +92,8 @@ class WavePhysics:
     (physics.py:29-76)."""
 
     def __init__(self, shots: int = SHOTS, nt: int = NT, dt: float = DT, peak_hz: float = PEAK_HZ,
-                 device: str | torch.device = "cuda") -> None:
-        self.engine = make_engine(shots, nt, dt, peak_hz)
+                 device: str | torch.device = "cuda", *, nvx: int = N, nvz: int = N, h: float = H) -> None:
+        self.engine = make_engine(shots, nt, dt, peak_hz, nvx=nvx, nvz=nvz, h=h)
         self.shots, self.nt, self.dt = shots, nt, dt
         self.device = torch.device(device)
         self.full_wavelet = self.engine.wavelet().copy()
@@ -126,7 +128,7 @@ class WavePhysics:
 
     # -- solves --------------------------------------------------------------
     def simulate_torch(self, fields_km: torch.Tensor) -> torch.Tensor:
-        """[B, 70, 70] km/s (device) -> [B, shots, nt, 70] (device)."""
+        """[B, nvz, nvx] km/s (device) -> [B, shots, nt, nrec] (device)."""
         vel = (_device_f32(fields_km, self.device) * 1000.0).contiguous()
         B = vel.shape[0]
         out = torch.empty((B, self.shots, self.nt, self.engine.nrec), dtype=torch.float32, device=self.device)

@@ -8,6 +8,8 @@ Families:
            wavelength U[20, 70] cells
   faulted  flat layers cut by one dipping fault with a throw of U{2..10} cells
   anomaly  isotropic Gaussian, sigma U[3, 8] cells, amplitude +-U[0, 300] m/s
+  large    1000 x 300: 8 curved layers plus 20 Gaussian anomalies (sigma
+           U[5, 30] cells, amplitude +-U[0, 400] m/s), clipped to [1500, 4500]
 """
 
 from __future__ import annotations
@@ -85,4 +87,20 @@ def batch(n: int, seed: int, nvz: int = 70, nvx: int = 70):
         m = FAMILIES[b % 3](rng, nvz, nvx)
         start[b] = m
         true[b] = np.clip(m + gaussian_anomaly(rng, nvz, nvx), 1500.0, 4500.0)
+    return start, true
+
+
+def large_batch(n: int, seed: int, nvz: int = 300, nvx: int = 1000, layers: int = 8, anomalies: int = 20):
+    """(start, true) [n, nvz, nvx] m/s for the streamed large-grid workload:
+    start = curved layers, true = start + anomalies, both clipped to [1500, 4500]."""
+    start = np.empty((n, nvz, nvx), np.float32)
+    true = np.empty_like(start)
+    for b in range(n):
+        rng = np.random.default_rng(seed + b)
+        m = curved(rng, nvz, nvx, layers)
+        a = np.zeros((nvz, nvx), np.float32)
+        for _ in range(anomalies):
+            a += gaussian_anomaly(rng, nvz, nvx, sigma=(5.0, 30.0), amp=400.0)
+        start[b] = np.clip(m, 1500.0, 4500.0)
+        true[b] = np.clip(m + a, 1500.0, 4500.0)
     return start, true
