@@ -34,7 +34,7 @@ import numpy as np
 METRIC = "fwd+adjoint shot-solves/s at 70x70x1000 FP32 vs CPU ref; refine solves-to-misfit"
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
-CFG_SRC = np.array([[0, 10], [170, 10], [340, 10], [520, 10], [690, 10]], np.float32)
+SURFACE_SRC = np.array([[0, 10], [170, 10], [340, 10], [520, 10], [690, 10]], np.float32)
 
 
 def cpu_baseline(start, true, nt, dt, threads, units):
@@ -43,10 +43,10 @@ def cpu_baseline(start, true, nt, dt, threads, units):
 
     g = O.Geometry(nt=nt, dt=dt)
     w = O.ricker_f32(nt, dt, 15.0)
-    S = CFG_SRC.shape[0]
+    S = SURFACE_SRC.shape[0]
     idx = np.arange(units)
     b, s = idx // S, idx % S
-    src = CFG_SRC[s]
+    src = SURFACE_SRC[s]
     vel = np.ascontiguousarray(start[b].reshape(units, -1))
     obs = np.ascontiguousarray(O.batch_forward_f32(g, src, np.ascontiguousarray(true[b].reshape(units, -1)), w, threads))
     O.batch_fwd_adj_f32(g, src[:threads], vel[:threads], w, obs[:threads], threads)  # warm-up
@@ -108,7 +108,7 @@ def main() -> int:
 
     B, S, nt, dt = args.models, 5, args.nt, 0.001
     ph = WavePhysics(shots=S, nt=nt, dt=dt)
-    ph.engine.set_sources(CFG_SRC)
+    ph.engine.set_sources(SURFACE_SRC)
     start, true = synthetic.batch(B, seed=1000)
     start_km = torch.from_numpy(start / 1000.0).cuda()
     obs = ph.simulate_torch(torch.from_numpy(true / 1000.0).cuda())

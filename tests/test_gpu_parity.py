@@ -19,7 +19,7 @@ import torch
 
 pytestmark = pytest.mark.gpu
 
-CFG0_SRC = np.array([[0, 10], [170, 10], [340, 10], [520, 10], [690, 10]], np.float32)
+SURFACE_SRC = np.array([[0, 10], [170, 10], [340, 10], [520, 10], [690, 10]], np.float32)
 
 
 def relerr(a, b):
@@ -66,17 +66,17 @@ def oracle_gathers(orc, vel, src, nt, dt, wavelet, f64=False):
     return np.stack(out)
 
 
-def test_forward_cfg0_matches_cpu_solver(W, orc):
-    """Config 0: 3 flat layers, 5 shots at x-cells {0,17,34,52,69}, 1 ms x 1000."""
+def test_forward_70x70x1000_5shots_matches_cpu_solver(W, orc):
+    """3 flat layers, 5 shots at x-cells {0,17,34,52,69}, 1 ms x 1000."""
     E = W.PyWaveEngine(5, 1000, 0.001, 15.0, 20, 1000.0, 5000.0)
-    E.set_sources(CFG0_SRC)
+    E.set_sources(SURFACE_SRC)
     vel = three_layers(0)
     gpu = E.simulate(vel.reshape(-1))
     assert gpu.shape == (5, 1000, 70) and gpu[:, ::4].shape == (5, 250, 70)
-    ref32 = oracle_gathers(orc, vel, CFG0_SRC, 1000, 0.001, E.wavelet())
-    ref64 = oracle_gathers(orc, vel, CFG0_SRC, 1000, 0.001, E.wavelet(), f64=True)
+    ref32 = oracle_gathers(orc, vel, SURFACE_SRC, 1000, 0.001, E.wavelet())
+    ref64 = oracle_gathers(orc, vel, SURFACE_SRC, 1000, 0.001, E.wavelet(), f64=True)
     e32, e64 = relerr(gpu, ref32), relerr(gpu, ref64)
-    print(f"cfg0 gathers: rel-L2 vs FP32 CPU {e32:.3e} (bit-equal {np.mean(gpu == ref32):.4f}), vs FP64 CPU {e64:.3e}")
+    print(f"70x70x1000 gathers: rel-L2 vs FP32 CPU {e32:.3e} (bit-equal {np.mean(gpu == ref32):.4f}), vs FP64 CPU {e64:.3e}")
     assert e32 <= 1e-5
     assert e64 <= 1e-4
 
