@@ -107,6 +107,17 @@ int32_t wg_loss_grad_dev(wg_engine *e, int32_t B, const float *vel_dev,
                          float *grad_dev, float *traces_dev,
                          double *energy_dev, void *stream);
 
+/* wg_loss_grad_dev with per-(model, shot) wavelets wav_dev [B, shots, nt]
+ * in place of the engine wavelet, so that models of one batch can sit in
+ * different passbands (the reference changes the engine wavelet per band,
+ * physics.py:37-46; here each model of a batched call carries its own).
+ * The source term of the gradient uses the same wavelets. */
+int32_t wg_loss_grad_wavelets_dev(wg_engine *e, int32_t B, const float *vel_dev,
+                                  const float *wav_dev, const float *obs_dev,
+                                  float *loss_dev, float *grad_dev,
+                                  float *traces_dev, double *energy_dev,
+                                  void *stream);
+
 /* Transpose of the linearised engine at vel_dev, for the dot-product test
  * and gradient checks (no reference counterpart: the reference exposes the
  * adjoint only through evaluate_adjoint, splatAdjoint.wgsl:58-131).

@@ -60,6 +60,7 @@ SIGNATURES = {
     "wg_evaluate_adjoint": (_i32, [_vp, _vp, _vp, _vp, _vp]),
     "wg_simulate_dev": (_i32, [_vp, _i32, _vp, _vp, _vp]),
     "wg_loss_grad_dev": (_i32, [_vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "wg_loss_grad_wavelets_dev": (_i32, [_vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "wg_adjoint_apply_dev": (_i32, [_vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp]),
     "wg_born_dev": (_i32, [_vp, _i32, _vp, _vp, _vp, _vp]),
     "wg_simulate_wavelets_dev": (_i32, [_vp, _i32, _vp, _vp, _vp, _vp]),

@@ -107,6 +107,9 @@ struct wg_engine {
     hipStream_t last = nullptr;    // stream of the last enqueued call
     bool last_valid = false;
     size_t snap_budget = 0;
+    // per-(model, shot) wavelets [B, shots, nt] of the call in progress, or
+    // null for the engine wavelet (set by WavScope under the engine mutex)
+    const float* wav_dev = nullptr;
     std::mutex mu;
     // kernel timing: (start, stop, launches) records of forward / adjoint launches
     bool timing = false;
@@ -236,13 +239,22 @@ int check_fits(const wg::Grid& g, int lds_limit, std::string& why) {
     return WG_OK;
 }
 
+const float* wav_ptr(const wg_engine* e) { return e->wav_dev ? e->wav_dev : e->d_wavelet; }
+long long wav_stride(const wg_engine* e) { return e->wav_dev ? e->g.nt : 0; }
+
+struct WavScope {
+    wg_engine* e;
+    WavScope(wg_engine* en, const float* w) : e(en) { e->wav_dev = w; }
+    ~WavScope() { e->wav_dev = nullptr; }
+};
+
 wg::FwdParams fwd_params(wg_engine* e, const float* vel, float* traces) {
     wg::FwdParams p{};
     p.g = e->g;
     p.geo = e->geo();
     p.vel = vel;
-    p.wavelet = e->d_wavelet;
-    p.wav_stride = 0;
+    p.wavelet = wav_ptr(e);
+    p.wav_stride = wav_stride(e);
     p.traces = traces;
     return p;
 }
@@ -252,8 +264,8 @@ wg::AdjParams adj_params(wg_engine* e, const float* vel, const float* resid, flo
     p.g = e->g;
     p.geo = e->geo();
     p.vel = vel;
-    p.wavelet = e->d_wavelet;
-    p.wav_stride = 0;
+    p.wavelet = wav_ptr(e);
+    p.wav_stride = wav_stride(e);
     p.resid = resid;
     p.grad = grad;
     return p;
@@ -304,7 +316,7 @@ int stream_simulate(wg_engine* e, int b0, int nb, const float* vel, const float*
                     float* traces, hipStream_t st);
 
 int run_chunk_forward(wg_engine* e, int b0, int nb, const float* vel, float* traces, bool store, hipStream_t st) {
-    if (e->streamed && !store) return stream_simulate(e, b0, nb, vel, e->d_wavelet, 0, traces, st);
+    if (e->streamed && !store) return stream_simulate(e, b0, nb, vel, wav_ptr(e), wav_stride(e), traces, st);
     wg::FwdParams p = fwd_params(e, vel, traces);
     p.unit0 = b0 * e->cfg.shots;
     p.snaps = store ? (float*)e->snaps.p : nullptr;
@@ -362,8 +374,8 @@ wg::StepParams step_params(wg_engine* e, const float* vel, int units, int unit0)
     p.g = e->g;
     p.geo = e->geo();
     p.vel = vel;
-    p.wavelet = e->d_wavelet;
-    p.wav_stride = 0;
+    p.wavelet = wav_ptr(e);
+    p.wav_stride = wav_stride(e);
     p.units = units;
     p.unit0 = unit0;
     return p;
@@ -488,7 +500,7 @@ int loss_grad_core(wg_engine* e, int B, const float* vel, const float* obs, floa
         if ((rc = stream_alloc(e, P, P.models * S))) return rc;
         for (int b0 = 0; b0 < B; b0 += P.models) {
             const int nb = std::min(P.models, B - b0);
-            if ((rc = stream_forward(e, P, nb * S, b0 * S, vel, e->d_wavelet, 0, traces, true, st))) return rc;
+            if ((rc = stream_forward(e, P, nb * S, b0 * S, vel, wav_ptr(e), wav_stride(e), traces, true, st))) return rc;
             LAUNCH(wg::launch_residual(traces + b0 * per, obs + b0 * per, scale + b0, per, nb,
                                        (float*)e->resid.p + b0 * per, (float*)e->partials.p + b0 * chunks, st));
             if ((rc = stream_adjoint(e, P, nb * S, b0 * S, vel, (const float*)e->resid.p, (float*)e->gunit.p,
@@ -799,10 +811,12 @@ int32_t wg_simulate_dev(wg_engine* e, int32_t B, const float* vel_dev, float* tr
     return run_chunk_forward(e, 0, B, vel_dev, traces_dev, false, st);
 }
 
-int32_t wg_loss_grad_dev(wg_engine* e, int32_t B, const float* vel_dev, const float* obs_dev, float* loss_dev,
-                         float* grad_dev, float* traces_dev, double* energy_dev, void* stream) {
+static int32_t loss_grad_entry(wg_engine* e, int32_t B, const float* vel_dev, const float* wav_dev,
+                               const float* obs_dev, float* loss_dev, float* grad_dev, float* traces_dev,
+                               double* energy_dev, void* stream) {
     if (!e || !vel_dev || !obs_dev || !loss_dev || !grad_dev || B < 0) return fail(WG_E_INVALID, "invalid argument");
     Guard G(e);
+    WavScope W(e, wav_dev);
     HIPCK(hipSetDevice(e->device));
     hipStream_t st = (hipStream_t)stream;
     int rc;
@@ -816,6 +830,18 @@ int32_t wg_loss_grad_dev(wg_engine* e, int32_t B, const float* vel_dev, const fl
     }
     return loss_grad_core(e, B, vel_dev, obs_dev, loss_dev, grad_dev, traces_dev, energy, (float*)e->scale.p, true,
                           st);
+}
+
+int32_t wg_loss_grad_dev(wg_engine* e, int32_t B, const float* vel_dev, const float* obs_dev, float* loss_dev,
+                         float* grad_dev, float* traces_dev, double* energy_dev, void* stream) {
+    return loss_grad_entry(e, B, vel_dev, nullptr, obs_dev, loss_dev, grad_dev, traces_dev, energy_dev, stream);
+}
+
+int32_t wg_loss_grad_wavelets_dev(wg_engine* e, int32_t B, const float* vel_dev, const float* wav_dev,
+                                  const float* obs_dev, float* loss_dev, float* grad_dev, float* traces_dev,
+                                  double* energy_dev, void* stream) {
+    if (!wav_dev) return fail(WG_E_INVALID, "invalid argument");
+    return loss_grad_entry(e, B, vel_dev, wav_dev, obs_dev, loss_dev, grad_dev, traces_dev, energy_dev, stream);
 }
 
 int32_t wg_adjoint_apply_dev(wg_engine* e, int32_t B, const float* vel_dev, const float* resid_dev, float* grad_dev,
@@ -841,7 +867,7 @@ int32_t wg_adjoint_apply_dev(wg_engine* e, int32_t B, const float* vel_dev, cons
         if ((rc = stream_alloc(e, P, P.models * S))) return rc;
         for (int b0 = 0; b0 < B; b0 += P.models) {
             const int nb = std::min(P.models, B - b0);
-            if ((rc = stream_forward(e, P, nb * S, b0 * S, vel_dev, e->d_wavelet, 0, traces, true, st))) return rc;
+            if ((rc = stream_forward(e, P, nb * S, b0 * S, vel_dev, wav_ptr(e), wav_stride(e), traces, true, st))) return rc;
             if ((rc = stream_adjoint(e, P, nb * S, b0 * S, vel_dev, resid_dev, (float*)e->gunit.p, dwav_dev, st)))
                 return rc;
         }
@@ -879,7 +905,7 @@ int32_t wg_born_dev(wg_engine* e, int32_t B, const float* vel_dev, const float* 
         const size_t cells = (size_t)g.nx * g.nz;
         for (int b0 = 0; b0 < B; b0 += P.models) {
             const int nb = std::min(P.models, B - b0);
-            if ((rc = stream_forward(e, P, nb * S, b0 * S, vel_dev, e->d_wavelet, 0, (float*)e->traces.p, true, st)))
+            if ((rc = stream_forward(e, P, nb * S, b0 * S, vel_dev, wav_ptr(e), wav_stride(e), (float*)e->traces.p, true, st)))
                 return rc;
             HIPCK(hipMemsetAsync(e->astate.p, 0, 8 * nb * S * cells * sizeof(float), st));
             wg::StepParams p = step_params(e, vel_dev, nb * S, b0 * S);

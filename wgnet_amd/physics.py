@@ -112,27 +112,55 @@ class WavePhysics:
         w = np.convolve(self.full_wavelet, self.kernel, mode="full")[: self.nt].astype(np.float32)
         self.engine.set_wavelet(np.ascontiguousarray(w))
 
+    def band_wavelet(self, fmax_hz: float | None) -> np.ndarray:
+        """The source wavelet of passband fmax_hz (None: full band), as
+        set_band would install it."""
+        if fmax_hz is None:
+            return self.full_wavelet.copy()
+        k = lowpass_kernel(fmax_hz, self.dt)
+        return np.convolve(self.full_wavelet, k, mode="full")[: self.nt].astype(np.float32)
+
     def bandlimit(self, traces):
         """Causal low-pass along time of [..., nt, nrec] data (physics.py:48-57);
         numpy in -> numpy out, tensor in -> tensor out on its device."""
-        if self.kernel is None:
+        return self.bandlimit_to(traces, self.fmax)
+
+    def bandlimit_to(self, traces, fmax_hz: float | None):
+        """`bandlimit` with an explicit passband (None: unchanged)."""
+        if fmax_hz is None:
             return traces
+        kernel = lowpass_kernel(fmax_hz, self.dt)
         is_np = not isinstance(traces, torch.Tensor)
         t = torch.from_numpy(np.ascontiguousarray(traces, dtype=np.float32)) if is_np else traces.float()
         lead = t.shape[:-2]
         x = t.reshape(-1, 1, t.shape[-2], t.shape[-1]).transpose(2, 3).reshape(-1, 1, t.shape[-2])
-        k = torch.from_numpy(self.kernel).to(t.device).view(1, 1, -1)
-        y = torch.nn.functional.conv1d(torch.nn.functional.pad(x, (len(self.kernel) - 1, 0)), k)
+        k = torch.from_numpy(kernel).to(t.device).view(1, 1, -1)
+        y = torch.nn.functional.conv1d(torch.nn.functional.pad(x, (len(kernel) - 1, 0)), k)
         out = y.view(*lead, t.shape[-1], t.shape[-2]).transpose(-1, -2).contiguous()
         return out.numpy() if is_np else out
 
     # -- solves --------------------------------------------------------------
-    def simulate_torch(self, fields_km: torch.Tensor) -> torch.Tensor:
-        """[B, nvz, nvx] km/s (device) -> [B, shots, nt, nrec] (device)."""
+    def _unit_wavelets(self, wavelets, B: int) -> torch.Tensor:
+        """[B, nt] or [B, shots, nt] -> contiguous [B, shots, nt] on the device."""
+        w = _device_f32(wavelets, self.device)
+        if w.dim() == 2:
+            w = w[:, None, :].expand(B, self.shots, self.nt)
+        if tuple(w.shape) != (B, self.shots, self.nt):
+            raise ValueError("wavelets must have shape [B, nt] or [B, shots, nt]")
+        return w.contiguous()
+
+    def simulate_torch(self, fields_km: torch.Tensor, wavelets=None) -> torch.Tensor:
+        """[B, nvz, nvx] km/s (device) -> [B, shots, nt, nrec] (device); with
+        `wavelets` ([B, nt] or [B, shots, nt]) each model uses its own."""
         vel = (_device_f32(fields_km, self.device) * 1000.0).contiguous()
         B = vel.shape[0]
         out = torch.empty((B, self.shots, self.nt, self.engine.nrec), dtype=torch.float32, device=self.device)
-        _lib.check(self.engine._lib.wg_simulate_dev(self.engine.handle, B, _dev_ptr(vel), _dev_ptr(out), _stream()))
+        lib, h = self.engine._lib, self.engine.handle
+        if wavelets is None:
+            _lib.check(lib.wg_simulate_dev(h, B, _dev_ptr(vel), _dev_ptr(out), _stream()))
+        else:
+            w = self._unit_wavelets(wavelets, B)
+            _lib.check(lib.wg_simulate_wavelets_dev(h, B, _dev_ptr(vel), _dev_ptr(w), _dev_ptr(out), _stream()))
         return out
 
     def simulate(self, fields_km):
@@ -143,9 +171,11 @@ class WavePhysics:
         return self.simulate_torch(torch.from_numpy(np.asarray(fields_km, np.float32))).cpu().numpy()
 
     def loss_and_grad_torch(self, fields_km: torch.Tensor, observed: torch.Tensor,
-                            traces_out: torch.Tensor | None = None) -> tuple[torch.Tensor, torch.Tensor]:
+                            traces_out: torch.Tensor | None = None,
+                            wavelets=None) -> tuple[torch.Tensor, torch.Tensor]:
         """Per-model normalised misfit and its gradient w.r.t. km/s, on the
-        device, for a whole batch in one call."""
+        device, for a whole batch in one call; with `wavelets` ([B, nt] or
+        [B, shots, nt]) each model uses its own source wavelet."""
         vel = (_device_f32(fields_km, self.device) * 1000.0).contiguous()
         obs = _device_f32(observed, self.device)
         B = vel.shape[0]
@@ -155,7 +185,13 @@ class WavePhysics:
         grads = torch.empty_like(vel)
         energy = torch.empty(B, dtype=torch.float64, device=self.device)
         tr = _dev_ptr(traces_out) if traces_out is not None else None
-        _lib.check(self.engine._lib.wg_loss_grad_dev(self.engine.handle, B, _dev_ptr(vel), _dev_ptr(obs),
+        lib, h = self.engine._lib, self.engine.handle
+        if wavelets is None:
+            _lib.check(lib.wg_loss_grad_dev(h, B, _dev_ptr(vel), _dev_ptr(obs), _dev_ptr(losses), _dev_ptr(grads), tr,
+                                            _dev_ptr(energy), _stream()))
+        else:
+            w = self._unit_wavelets(wavelets, B)
+            _lib.check(lib.wg_loss_grad_wavelets_dev(h, B, _dev_ptr(vel), _dev_ptr(w), _dev_ptr(obs),
                                                      _dev_ptr(losses), _dev_ptr(grads), tr, _dev_ptr(energy),
                                                      _stream()))
         if bool((energy <= 0).any()):

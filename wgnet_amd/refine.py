@@ -8,9 +8,18 @@ at most 6 halvings, step norm <= 1), a frequency-continuation schedule over
 passbands, and the reference's solve accounting: 1 solve per misfit-only
 evaluation, 2 per evaluation with gradient (refine.py:99-108).
 
-`precondition`, when given, maps the velocity-space gradient [70, 70] (the
-hook point of refine.py:105-106) and the current field [70, 70] to the
-velocity-space vector that the chain rule carries to the search parameters.
+Each model's optimiser is a generator (`_program`) that yields evaluation
+requests and receives their results; `refine` drives one program, and
+`refine_batch` drives many in lockstep, answering every round's requests with
+one batched misfit call and one batched misfit-and-gradient call in which each
+model carries its own passband.  A model's sequence of evaluations and
+decisions is the same either way.
+
+`precondition`, when given, maps the velocity-space gradient (the hook point of
+refine.py:105-106) and the current field, both km/s-based [n, 70, 70] device
+tensors, to the velocity-space vector that the chain rule carries to the
+search parameters.  A model whose preconditioned line search fails to reduce
+the misfit continues with plain gradients (its L-BFGS memory is reset).
 """
 
 from __future__ import annotations
@@ -44,6 +53,7 @@ class RefineResult(NamedTuple):
     history: list
     gaussians: np.ndarray  # [K, 6] refined active slots
     coarse_km: np.ndarray  # [9, 9]
+    preconditioned: bool = False  # still preconditioned at the end
 
 
 def lbfgs_direction(grad: np.ndarray, s_hist: list, y_hist: list, rho: list) -> np.ndarray:
@@ -61,82 +71,72 @@ def lbfgs_direction(grad: np.ndarray, s_hist: list, y_hist: list, rho: list) -> 
     return -q
 
 
-def _misfit_f64(pred: np.ndarray, observed: np.ndarray) -> float:
-    d = pred.reshape(-1).astype(np.float64) - observed.reshape(-1).astype(np.float64)
-    o = observed.reshape(-1).astype(np.float64)
-    return 0.5 * float(d @ d) / float(o @ o)
+class _Scene:
+    """Static data of one model's scene: parameters are [coarse 81 | 6 per
+    active Gaussian] in scaled units (refine.py:62-90)."""
 
+    def __init__(self, dense_km: np.ndarray, gaussians: np.ndarray, amp_keep_km: float, device):
+        self.device = torch.device(device)
+        self.active = np.flatnonzero(np.abs(gaussians[:, 5]) >= amp_keep_km)
+        self.n_g = len(self.active)
+        self.n_c = COARSE * COARSE
+        self.slots = torch.as_tensor(self.active, device=self.device)
+        self.dense_t = torch.as_tensor(np.asarray(dense_km, np.float32), device=self.device)
+        self.scales = np.concatenate([np.full(self.n_c, COARSE_SCALE), np.tile(np.array(GAUSS_SCALES), self.n_g)])
+        self.x0 = np.concatenate([np.zeros(self.n_c), gaussians[self.active].astype(np.float64).reshape(-1)])
 
-def refine(
-    dense_km: np.ndarray,
-    gaussians: np.ndarray,
-    observed_full: np.ndarray,
-    physics: WavePhysics,
-    budget_solves: int = 160,
-    bands_hz: Sequence[float | None] = (8.0, 12.0, None),
-    amp_keep_km: float = ACTIVE_AMP_KM,
-    device: torch.device | str = "cpu",
-    precondition: Callable[[np.ndarray, np.ndarray], np.ndarray] | None = None,
-) -> RefineResult:
-    """Refine one scene.  dense_km [70, 70], gaussians [1225, 6] activated,
-    observed_full [shots, nt, 70] (refine.py:56-169)."""
-    t0 = time.perf_counter()
-    active = np.flatnonzero(np.abs(gaussians[:, 5]) >= amp_keep_km)
-    n_g = len(active)
-    slots = torch.as_tensor(active, device=device)
-    dense_t = torch.as_tensor(np.asarray(dense_km, np.float32), device=device)
-    n_c = COARSE * COARSE
-    scales = np.concatenate([np.full(n_c, COARSE_SCALE), np.tile(np.array(GAUSS_SCALES), n_g)])
-    x = np.concatenate([np.zeros(n_c), gaussians[active].astype(np.float64).reshape(-1)])
-
-    def project(vec: np.ndarray) -> np.ndarray:
+    def project(self, vec: np.ndarray) -> np.ndarray:
         out = vec.copy()
+        n_c, n_g = self.n_c, self.n_g
         out[:n_c] = np.clip(out[:n_c], -COARSE_MAX_KM, COARSE_MAX_KM)
         if n_g:
             out[n_c:] = clamp_gaussians(torch.from_numpy(out[n_c:].reshape(n_g, 6))).numpy().reshape(-1)
         return out
 
-    def assemble(vec: np.ndarray, need_grad: bool):
-        coarse = torch.tensor(vec[:n_c].reshape(1, 1, COARSE, COARSE), dtype=torch.float32, device=device,
+    def assemble(self, vec: np.ndarray, need_grad: bool):
+        n_c, n_g = self.n_c, self.n_g
+        coarse = torch.tensor(vec[:n_c].reshape(1, 1, COARSE, COARSE), dtype=torch.float32, device=self.device,
                               requires_grad=need_grad)
-        g = torch.tensor(vec[n_c:].reshape(n_g, 6), dtype=torch.float32, device=device, requires_grad=need_grad)
-        field = dense_t + F.interpolate(coarse, size=(GRID, GRID), mode="bilinear", align_corners=True)[0, 0]
+        g = torch.tensor(vec[n_c:].reshape(n_g, 6), dtype=torch.float32, device=self.device, requires_grad=need_grad)
+        field = self.dense_t + F.interpolate(coarse, size=(GRID, GRID), mode="bilinear", align_corners=True)[0, 0]
         if n_g:
-            field = field + render_residual(g[None], slots)[0]
+            field = field + render_residual(g[None], self.slots)[0]
         return field, coarse, g
 
+    def chain(self, field, coarse, g, gv: torch.Tensor, retain: bool) -> np.ndarray:
+        """Parameter-space gradient (scaled units) of <field, gv>."""
+        gc, gg = torch.autograd.grad((field * gv.to(self.device)).sum(), [coarse, g], allow_unused=True,
+                                     retain_graph=retain)
+        parts = [gc.detach().cpu().numpy().reshape(-1)]
+        parts.append(gg.detach().cpu().numpy().reshape(-1) if gg is not None else np.zeros(6 * self.n_g))
+        return np.concatenate(parts).astype(np.float64) * self.scales
+
+
+class _Request(NamedTuple):
+    need_grad: bool
+    vec: np.ndarray
+    band: float | None
+    precondition: bool
+
+
+def _program(scene: _Scene, budget_solves: int, bands_hz: Sequence[float | None], use_pc: bool):
+    """One model's projected L-BFGS with the band schedule (refine.py:92-169).
+    Yields _Request; receives (loss, plain_grad, pc_grad) for gradient
+    requests (pc_grad None when not preconditioning) and the loss otherwise.
+    Returns (x, initial_loss, final_loss, solves, history, preconditioned)."""
     solves = 0
     history: list = []
-
-    def evaluate(vec: np.ndarray, need_grad: bool, observed: np.ndarray):
-        nonlocal solves
-        field, coarse, g = assemble(vec, need_grad)
-        field_np = field.detach().cpu().numpy()
-        if not need_grad:
-            solves += 1
-            pred = physics.engine.simulate(np.ascontiguousarray(field_np.reshape(-1) * 1000.0, dtype=np.float32))
-            return _misfit_f64(pred, observed), field_np, None
-        solves += 2
-        losses, grads = physics.loss_and_grad(field_np[None], observed[None])
-        gv = grads[0]
-        if precondition is not None:
-            gv = precondition(gv, field_np)
-        gc, gg = torch.autograd.grad((field * torch.as_tensor(gv, device=device)).sum(), [coarse, g],
-                                     allow_unused=True)
-        parts = [gc.detach().cpu().numpy().reshape(-1)]
-        parts.append(gg.detach().cpu().numpy().reshape(-1) if gg is not None else np.zeros(6 * n_g))
-        return float(losses[0]), field_np, np.concatenate(parts).astype(np.float64) * scales
-
-    x = project(x)
-    physics.set_band(None)
-    initial_loss, initial_field, _ = evaluate(x, False, observed_full)
+    x = scene.project(scene.x0)
+    initial_loss = yield _Request(False, x, None, False)
+    solves += 1
     per_band = budget_solves // len(bands_hz)
+    scales = scene.scales
 
     for band in bands_hz:
-        physics.set_band(band)
-        observed = physics.bandlimit(observed_full)
         band_budget = solves + per_band
-        cur_loss, _, cur_grad = evaluate(x, True, observed)
+        cur_loss, g_plain, g_pc = yield _Request(True, x, band, use_pc)
+        solves += 2
+        cur_grad = g_pc if use_pc else g_plain
         s_hist: list = []
         y_hist: list = []
         rho: list = []
@@ -154,38 +154,172 @@ def refine(
             for _ in range(LINE_SEARCH_TRIALS):
                 if solves + 1 >= band_budget:
                     break
-                cand = project(x + step * scales * d)
-                cand_loss, _, _ = evaluate(cand, False, observed)
+                cand = scene.project(x + step * scales * d)
+                cand_loss = yield _Request(False, cand, band, False)
+                solves += 1
                 if cand_loss <= cur_loss + ARMIJO * step * slope:
                     if solves + 2 > band_budget:
                         x, cur_loss = cand, cand_loss
                         accepted = True
                         break
-                    cand_loss, _, cand_grad = evaluate(cand, True, observed)
+                    cand_loss, c_plain, c_pc = yield _Request(True, cand, band, use_pc)
+                    solves += 2
+                    cand_grad = c_pc if use_pc else c_plain
                     s, y = (cand - x) / scales, cand_grad - cur_grad
                     sy = float(s @ y)
                     if sy > 1e-12:
                         if len(s_hist) >= LBFGS_PAIRS:
                             s_hist.pop(0), y_hist.pop(0), rho.pop(0)
                         s_hist.append(s), y_hist.append(y), rho.append(1.0 / sy)
-                    x, cur_loss, cur_grad = cand, cand_loss, cand_grad
+                    x, cur_loss, cur_grad, g_plain = cand, cand_loss, cand_grad, c_plain
                     accepted = True
                     break
                 step *= 0.5
-            history.append({"band_hz": band, "solves": solves, "loss": cur_loss, "accepted": accepted})
+            history.append({"band_hz": band, "solves": solves, "loss": cur_loss, "accepted": accepted,
+                            "preconditioned": use_pc})
             if not accepted:
+                if use_pc:
+                    # plain L-BFGS from here on: the last plain gradient is at x
+                    use_pc = False
+                    cur_grad = g_plain
+                    s_hist, y_hist, rho = [], [], []
+                    continue
                 break
 
-    physics.set_band(None)
-    final_loss, final_field, _ = evaluate(x, False, observed_full)
+    final_loss = yield _Request(False, x, None, False)
+    solves += 1
+    return x, initial_loss, final_loss, solves, history, use_pc
+
+
+def _misfit_f64(pred: torch.Tensor, observed: torch.Tensor) -> torch.Tensor:
+    """Per-model 0.5 ||p - d||^2 / ||d||^2 accumulated in float64."""
+    d = (pred.double() - observed.double()).flatten(1)
+    o = observed.double().flatten(1)
+    return 0.5 * (d * d).sum(1) / (o * o).sum(1)
+
+
+def _drive(scenes, programs, physics: WavePhysics, observed_full: torch.Tensor, precondition):
+    """Runs the programs in lockstep; returns their return values."""
+    dev = physics.device
+    band_obs: dict = {}
+    band_wav: dict = {}
+
+    def observed(i, band):
+        key = (i, band)
+        if key not in band_obs:
+            band_obs[key] = physics.bandlimit_to(observed_full[i:i + 1], band)[0].contiguous()
+        return band_obs[key]
+
+    def wavelet(band):
+        if band not in band_wav:
+            band_wav[band] = torch.from_numpy(physics.band_wavelet(band)).to(dev)
+        return band_wav[band]
+
+    pending = {i: next(p) for i, p in enumerate(programs)}
+    results: dict = {}
+    while pending:
+        answers = {}
+        loss_ids = [i for i, r in pending.items() if not r.need_grad]
+        grad_ids = [i for i, r in pending.items() if r.need_grad]
+        if loss_ids:
+            with torch.no_grad():
+                fields = torch.stack([scenes[i].assemble(pending[i].vec, False)[0] for i in loss_ids]).to(dev)
+            wav = torch.stack([wavelet(pending[i].band) for i in loss_ids])
+            pred = physics.simulate_torch(fields, wavelets=wav)
+            obs = torch.stack([observed(i, pending[i].band) for i in loss_ids])
+            losses = _misfit_f64(pred, obs).cpu().numpy()
+            for k, i in enumerate(loss_ids):
+                answers[i] = float(losses[k])
+        if grad_ids:
+            built = [scenes[i].assemble(pending[i].vec, True) for i in grad_ids]
+            fields = torch.stack([b[0].detach() for b in built]).to(dev)
+            wav = torch.stack([wavelet(pending[i].band) for i in grad_ids])
+            obs = torch.stack([observed(i, pending[i].band) for i in grad_ids])
+            losses, grads = physics.loss_and_grad_torch(fields, obs, wavelets=wav)
+            pc_rows = [k for k, i in enumerate(grad_ids) if pending[i].precondition]
+            pcg = None
+            if pc_rows and precondition is not None:
+                sel = torch.tensor(pc_rows, device=dev)
+                pcg = precondition(grads[sel], fields[sel])
+            losses = losses.cpu().numpy()
+            for k, i in enumerate(grad_ids):
+                field, coarse, g = built[k]
+                want_pc = pending[i].precondition and pcg is not None
+                plain = scenes[i].chain(field, coarse, g, grads[k], retain=want_pc)
+                pc = scenes[i].chain(field, coarse, g, pcg[pc_rows.index(k)], retain=False) if want_pc else None
+                answers[i] = (float(losses[k]), plain, pc)
+        for i, ans in answers.items():
+            try:
+                pending[i] = programs[i].send(ans)
+            except StopIteration as stop:
+                results[i] = stop.value
+                del pending[i]
+    return [results[i] for i in range(len(programs))]
+
+
+def _result(scene: _Scene, out, seconds: float) -> RefineResult:
+    x, initial_loss, final_loss, solves, history, pc = out
+    with torch.no_grad():
+        initial_field = scene.assemble(scene.project(scene.x0), False)[0].cpu().numpy()
+        final_field = scene.assemble(x, False)[0].cpu().numpy()
+    n_c, n_g = scene.n_c, scene.n_g
     return RefineResult(
         initial_field_km=initial_field,
         final_field_km=final_field,
         initial_loss=initial_loss,
         final_loss=final_loss,
         solves=solves,
-        seconds=time.perf_counter() - t0,
+        seconds=seconds,
         history=history,
         gaussians=x[n_c:].reshape(n_g, 6).astype(np.float32),
         coarse_km=x[:n_c].reshape(COARSE, COARSE).astype(np.float32),
+        preconditioned=pc,
     )
+
+
+def refine_batch(
+    dense_km: np.ndarray,
+    gaussians: np.ndarray,
+    observed_full,
+    physics: WavePhysics,
+    budget_solves: int = 160,
+    bands_hz: Sequence[float | None] = (8.0, 12.0, None),
+    amp_keep_km: float = ACTIVE_AMP_KM,
+    device: torch.device | str = "cpu",
+    precondition: Callable[[torch.Tensor, torch.Tensor], torch.Tensor] | None = None,
+) -> list[RefineResult]:
+    """Refines B scenes together: dense_km [B, 70, 70], gaussians [B, 1225, 6]
+    activated, observed_full [B, shots, nt, 70] (numpy or tensor).  Scene
+    algebra runs on `device`; the solves on the physics device."""
+    t0 = time.perf_counter()
+    obs = observed_full if isinstance(observed_full, torch.Tensor) else torch.from_numpy(
+        np.ascontiguousarray(observed_full, np.float32))
+    obs = obs.to(physics.device, torch.float32)
+    scenes = [_Scene(dense_km[b], gaussians[b], amp_keep_km, device) for b in range(len(dense_km))]
+    programs = [_program(sc, budget_solves, tuple(bands_hz), precondition is not None) for sc in scenes]
+    outs = _drive(scenes, programs, physics, obs, precondition)
+    el = time.perf_counter() - t0
+    return [_result(sc, out, el) for sc, out in zip(scenes, outs)]
+
+
+def refine(
+    dense_km: np.ndarray,
+    gaussians: np.ndarray,
+    observed_full: np.ndarray,
+    physics: WavePhysics,
+    budget_solves: int = 160,
+    bands_hz: Sequence[float | None] = (8.0, 12.0, None),
+    amp_keep_km: float = ACTIVE_AMP_KM,
+    device: torch.device | str = "cpu",
+    precondition: Callable[[np.ndarray, np.ndarray], np.ndarray] | None = None,
+) -> RefineResult:
+    """Refine one scene.  dense_km [70, 70], gaussians [1225, 6] activated,
+    observed_full [shots, nt, 70] (refine.py:56-169).  `precondition` here maps
+    numpy [70, 70] (gradient, field) to numpy [70, 70]."""
+    hook = None
+    if precondition is not None:
+        def hook(g: torch.Tensor, f: torch.Tensor) -> torch.Tensor:
+            out = [precondition(g[k].cpu().numpy(), f[k].cpu().numpy()) for k in range(g.shape[0])]
+            return torch.from_numpy(np.stack(out).astype(np.float32)).to(g.device)
+    return refine_batch(np.asarray(dense_km)[None], np.asarray(gaussians)[None], np.asarray(observed_full)[None],
+                        physics, budget_solves, bands_hz, amp_keep_km, device, hook)[0]
