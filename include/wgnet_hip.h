@@ -172,6 +172,47 @@ void wg_cpml_profiles(int32_t n, float h, float dt, int32_t width,
                       float max_vel, float freq, float *a, float *b,
                       float *db);
 
+/* ------------------------------------------------------------------------
+ * Learned gradient preconditioner (no reference counterpart: the reference
+ * refines with plain projected L-BFGS, python/wgnet/refine.py:92-169; this
+ * network is applied at its gradient hook, refine.py:105-106).
+ *
+ * A 3x3 convolutional network 2 -> 32 -> 32 -> 32 -> 1 channels with GELU
+ * after the first three layers and zero padding, mapping
+ * x [N, 2, H, W] (gradient, normalised velocity) to y [N, H, W].  bfloat16
+ * weights and activations, float32 MFMA accumulation, float32 master weights
+ * and AdamW moments.  Parameters in PyTorch Conv2d order (per layer
+ * W[co][ci][ky][kx], then b[co]).  Status codes as above; messages from
+ * wg_pc_last_error().  One network must not be driven by two threads at once
+ * (calls are serialised by a mutex).
+ * ------------------------------------------------------------------------ */
+typedef struct wg_pc wg_pc;
+
+const char *wg_pc_last_error(void);
+/* max_batch images of height x width; deterministic U(+-1/sqrt(fan_in)) init */
+int32_t wg_pc_create(int32_t max_batch, int32_t height, int32_t width,
+                     uint64_t seed, wg_pc **out);
+void wg_pc_destroy(wg_pc *pc);
+int32_t wg_pc_num_params(void);
+int32_t wg_pc_get_params(wg_pc *pc, float *host, int32_t n);
+/* also resets the AdamW moments and step count */
+int32_t wg_pc_set_params(wg_pc *pc, const float *host, int32_t n);
+/* y_dev [N, H, W] = net(x_dev [N, 2, H, W]), enqueued on stream */
+int32_t wg_pc_forward_dev(wg_pc *pc, int32_t N, const float *x_dev,
+                          float *y_dev, void *stream);
+/* loss = mean((net(x) - target)^2) and its parameter gradient (host,
+ * PyTorch order), without updating; waits for the stream */
+int32_t wg_pc_loss_grad(wg_pc *pc, int32_t N, const float *x_dev,
+                        const float *target_dev, float *loss_host,
+                        float *grad_host, void *stream);
+/* one AdamW step (beta 0.9 / 0.999, eps 1e-8, decoupled weight decay) on the
+ * mean squared error; loss_dev [1] (may be NULL) receives the loss before
+ * the step */
+int32_t wg_pc_train_step_dev(wg_pc *pc, int32_t N, const float *x_dev,
+                             const float *target_dev, float lr,
+                             float weight_decay, float *loss_dev,
+                             void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -71,6 +71,15 @@ SIGNATURES = {
                               ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]),
     "wg_ricker_wavelet": (None, [_i32, ctypes.c_float, ctypes.c_float, _vp]),
     "wg_cpml_profiles": (None, [_i32, ctypes.c_float, ctypes.c_float, _i32, ctypes.c_float, ctypes.c_float, _vp, _vp, _vp]),
+    "wg_pc_last_error": (ctypes.c_char_p, []),
+    "wg_pc_create": (_i32, [_i32, _i32, _i32, ctypes.c_uint64, ctypes.POINTER(_vp)]),
+    "wg_pc_destroy": (None, [_vp]),
+    "wg_pc_num_params": (_i32, []),
+    "wg_pc_get_params": (_i32, [_vp, _vp, _i32]),
+    "wg_pc_set_params": (_i32, [_vp, _vp, _i32]),
+    "wg_pc_forward_dev": (_i32, [_vp, _i32, _vp, _vp, _vp]),
+    "wg_pc_loss_grad": (_i32, [_vp, _i32, _vp, _vp, _vp, _vp, _vp]),
+    "wg_pc_train_step_dev": (_i32, [_vp, _i32, _vp, _vp, ctypes.c_float, ctypes.c_float, _vp, _vp]),
 }
 
 _LIB: ctypes.CDLL | None = None
@@ -99,11 +108,13 @@ def load() -> ctypes.CDLL:
     return lib
 
 
-def check(rc: int) -> None:
-    """Maps a status code to the reference's Python exception types."""
+def check(rc: int, pc: bool = False) -> None:
+    """Maps a status code to the reference's Python exception types (`pc`:
+    the status came from a wg_pc_* call)."""
     if rc == WG_OK:
         return
-    msg = load().wg_last_error().decode("utf-8", "replace")
+    lib = load()
+    msg = (lib.wg_pc_last_error() if pc else lib.wg_last_error()).decode("utf-8", "replace")
     if rc in (WG_E_INVALID, WG_E_TOO_LARGE, WG_E_ZERO_ENERGY):
         raise ValueError(msg)
     raise WaveEngineError(msg)
