@@ -89,7 +89,15 @@ def main() -> int:
     ap.add_argument("--nt", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--workload", choices=("batch70", "large", "copy"), default="batch70")
+    ap.add_argument("--workload", choices=("batch70", "large", "copy", "refine"), default="batch70")
+    ap.add_argument("--train-pairs", type=int, default=20000)
+    ap.add_argument("--eval-models", type=int, default=256)
+    ap.add_argument("--budget", type=int, default=160)
+    ap.add_argument("--epochs", type=int, default=100)
+    ap.add_argument("--save-params", default="")
+    ap.add_argument("--pc-mode", choices=("first", "replace"), default="first")
+    ap.add_argument("--lr", type=float, default=3e-3)
+    ap.add_argument("--compress", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only; this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -98,6 +106,8 @@ def main() -> int:
         return bench_large(args)
     if args.workload == "copy":
         return bench_copy(args)
+    if args.workload == "refine":
+        return bench_refine(args)
 
     import ctypes
 
@@ -199,6 +209,107 @@ def main() -> int:
                       f"{threads} threads, {cel:.1f} s wall; float32 C restatement of the reference solver",
         }
         out["speedup_vs_cpu"] = round(value / rate, 1)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def refine_scenes(n: int, seed: int):
+    """Held-out refinement problems: dense = the starting model (km/s), true =
+    start + anomaly, and a lattice of 7 x 7 active Gaussian slots (every fifth
+    slot, sigma 1.5 px, amplitude +-0.05 km/s in a checkerboard)."""
+    from wgnet_amd import synthetic
+    from wgnet_amd.render import N_SLOTS, SLOT_SIDE
+
+    start, true = synthetic.batch(n, seed=seed)
+    g = np.zeros((n, N_SLOTS, 6), np.float32)
+    for iz in range(2, SLOT_SIDE, 5):
+        for ix in range(2, SLOT_SIDE, 5):
+            k = iz * SLOT_SIDE + ix
+            g[:, k, 2] = g[:, k, 3] = np.log(1.5)
+            g[:, k, 5] = 0.05 if ((iz + ix) // 5) % 2 == 0 else -0.05
+    return start / 1000.0, true / 1000.0, g
+
+
+def bench_refine(args) -> int:
+    """Learned-preconditioned refinement against plain projected L-BFGS:
+    solves needed to reach plain L-BFGS's final data misfit."""
+    import torch
+
+    from wgnet_amd.physics import WavePhysics
+    from wgnet_amd.precond import Preconditioner, make_pairs, train
+    from wgnet_amd.refine import refine_batch
+
+    S, nt, dt = 5, 1000, 0.001
+    bands = (6.0, 9.0, 12.0, 15.0)
+    ph = WavePhysics(shots=S, nt=nt, dt=dt)
+    ph.engine.set_sources(SURFACE_SRC)
+    t0 = time.perf_counter()
+    grads, fields, target = make_pairs(ph, args.train_pairs, seed=4, bands_hz=bands)
+    torch.cuda.synchronize()
+    t_pairs = time.perf_counter() - t0
+    baseline = float((target ** 2).mean())
+    print(f"refine: {grads.shape[0]} training pairs in {t_pairs:.1f} s (zero-prediction loss {baseline:.4f})",
+          file=sys.stderr, flush=True)
+    net = Preconditioner(max_batch=256, seed=1)
+    net.compress = args.compress
+    t0 = time.perf_counter()
+    hist = train(net, grads, fields, target, epochs=args.epochs, lr=args.lr)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    print(f"refine: trained {args.epochs} epochs in {t_train:.1f} s, loss {hist[0]:.4f} -> {hist[-1]:.4f}",
+          file=sys.stderr, flush=True)
+    if args.save_params:
+        net.save(args.save_params)
+    del grads, fields, target
+
+    start_km, true_km, gauss = refine_scenes(args.eval_models, seed=900000)
+    obs = ph.simulate_torch(torch.from_numpy(true_km).cuda())
+    B = args.budget
+    fracs = (0.25, 0.5, 0.75, 1.0)
+    plain_final = {}
+    t_plain = 0.0
+    for f in fracs:
+        t0 = time.perf_counter()
+        plain = refine_batch(start_km, gauss, obs, ph, budget_solves=int(B * f), bands_hz=bands)
+        t_plain = time.perf_counter() - t0
+        plain_final[f] = np.array([r.final_loss for r in plain])
+        print(f"refine: plain L-BFGS {int(B * f)} solves in {t_plain:.1f} s, median final misfit "
+              f"{np.median(plain_final[f]):.4e}", file=sys.stderr, flush=True)
+    m_plain = plain_final[1.0]
+    pc_final = {}
+    hook = net.hook()
+    for f in fracs:
+        res = refine_batch(start_km, gauss, obs, ph, budget_solves=int(B * f), bands_hz=bands,
+                           precondition=hook, pc_mode=args.pc_mode)
+        pc_final[f] = np.array([r.final_loss for r in res])
+        print(f"refine: preconditioned {int(B * f)} solves: median final misfit {np.median(pc_final[f]):.4e}, "
+              f"reaches plain in {np.mean(pc_final[f] <= m_plain):.3f} of models", file=sys.stderr, flush=True)
+    need = np.full(len(m_plain), np.inf)
+    for f in reversed(fracs):
+        need[pc_final[f] <= m_plain] = f
+    init = np.array([r.initial_loss for r in plain])
+    out = {
+        "metric": "refine solves-to-misfit: preconditioned solves to reach plain L-BFGS's final misfit / plain budget",
+        "value": float(np.median(need)),
+        "unit": "fraction of the plain budget (median over models; inf = not reached)",
+        "n_gpus": 1,
+        "higher_is_better": False,
+        "vs_baseline": None,
+        "dtype": "f32 solver, bf16 network",
+        "data": "synthetic (flat/curved/faulted 3-layer models + Gaussian anomaly)",
+        "config": {"workload": f"{args.eval_models} held-out models, 5 shots, 70x70, nt={nt}, bands {bands} Hz, "
+                               f"plain budget {B} solves; network trained on {args.train_pairs} solver pairs",
+                   "budget": B, "eval_models": args.eval_models, "train_pairs": args.train_pairs,
+                   "epochs": args.epochs, "pc_mode": args.pc_mode},
+        "reached_at": {str(f): float(np.mean(need <= f)) for f in fracs},
+        "median_final_misfit": {"initial": float(np.median(init)),
+                                **{f"plain_{f}": float(np.median(plain_final[f])) for f in fracs},
+                                **{f"pc_{f}": float(np.median(pc_final[f])) for f in fracs}},
+        "pc_better_at_equal_budget": {str(f): float(np.mean(pc_final[f] < plain_final[f])) for f in fracs},
+        "train": {"pairs_s": round(t_pairs, 2), "train_s": round(t_train, 2), "loss_first": hist[0],
+                  "loss_last": hist[-1], "zero_prediction_loss": baseline, "grad_scale": net.grad_scale},
+        "plain_refine_s": round(t_plain, 2),
+    }
     print(json.dumps(out), flush=True)
     return 0
 

@@ -88,3 +88,71 @@ def test_lbfgs_two_loop_equals_bfgs_matrix():
     # secant condition on the newest pair: H y = s
     np.testing.assert_allclose(lbfgs_direction(y_hist[-1], s_hist, y_hist, rho), -s_hist[-1], rtol=1e-10)
     assert np.array_equal(lbfgs_direction(g, [], [], []), -g)
+
+
+class _LinearPhysics:
+    """Stand-in for WavePhysics on the CPU: traces = band-scaled linear map of
+    the field, misfit 0.5 ||p - d||^2 / ||d||^2 with its exact gradient."""
+
+    def __init__(self, seed=0):
+        g = torch.Generator().manual_seed(seed)
+        self.A = torch.randn(40, 70 * 70, generator=g, dtype=torch.float64) / 70.0
+        self.device = torch.device("cpu")
+
+    def band_wavelet(self, band):
+        return np.full(4, 1.0 if band is None else band / 20.0, np.float32)
+
+    def bandlimit_to(self, traces, band):
+        return traces * (1.0 if band is None else band / 20.0)
+
+    def _pred(self, fields, wavelets):
+        return (fields.double().reshape(fields.shape[0], -1) @ self.A.T) * wavelets[:, :1].double()
+
+    def simulate_torch(self, fields, wavelets):
+        return self._pred(fields, wavelets).float()
+
+    def loss_and_grad_torch(self, fields, obs, wavelets):
+        p = self._pred(fields, wavelets)
+        d = p - obs.double().reshape(p.shape)
+        e = (obs.double().reshape(p.shape) ** 2).sum(1)
+        loss = 0.5 * (d * d).sum(1) / e
+        grad = ((d / e[:, None]) * wavelets[:, :1].double()) @ self.A
+        return loss.float(), grad.reshape(fields.shape).float()
+
+
+def _linear_problem(phys, seed):
+    gen = torch.Generator().manual_seed(seed)
+    base = np.full((70, 70), 2.5, np.float32)
+    tp = random_scene(gen, 6)
+    true = torch.from_numpy(base) + R.render_residual(tp)[0]
+    obs = phys.simulate_torch(true[None], torch.ones(1, 4)).numpy()[0]
+    init = tp.clone()
+    init[..., 5] *= 0.5
+    return base, init[0].numpy(), obs
+
+
+def test_refine_batch_driver_on_a_linear_problem():
+    """Lockstep driver == one model at a time; the budget holds; the
+    preconditioned arm falls back to plain gradients when its step fails."""
+    from wgnet_amd.refine import refine_batch
+
+    phys = _LinearPhysics()
+    probs = [_linear_problem(phys, s) for s in (1, 2, 3)]
+    dense = np.stack([p[0] for p in probs])
+    init = np.stack([p[1] for p in probs])
+    obs = np.stack([p[2] for p in probs])
+    bands = (8.0, None)
+    batch = refine_batch(dense, init, obs, phys, budget_solves=30, bands_hz=bands)
+    for k in range(3):
+        one = refine_batch(dense[k:k + 1], init[k:k + 1], obs[k:k + 1], phys, budget_solves=30, bands_hz=bands)[0]
+        assert batch[k].solves == one.solves <= 32
+        assert batch[k].final_loss == one.final_loss
+        assert batch[k].final_loss < batch[k].initial_loss
+    # a "preconditioner" that points uphill: every preconditioned line search
+    # fails, so each model switches to plain gradients and still descends
+    anti = refine_batch(dense, init, obs, phys, budget_solves=30, bands_hz=bands,
+                        precondition=lambda g, f: -g)
+    for k in range(3):
+        assert not anti[k].preconditioned
+        assert any(h["preconditioned"] and not h["accepted"] for h in anti[k].history)
+        assert anti[k].final_loss < anti[k].initial_loss

@@ -7,8 +7,8 @@ No reference counterpart: the reference refines with plain projected L-BFGS
 normalised velocity) and predicts the model update (true - current), the pairs
 coming from the batched wave solver itself.  In refine() it is applied at the
 gradient hook of refine.py:105-106, where it returns minus the predicted
-update rescaled to the gradient's norm; refine falls back to plain gradients
-for a model whose preconditioned line search fails.
+update rescaled to the gradient's norm (refine_batch's `pc_mode` decides how
+L-BFGS uses it; plain gradients remain the fallback).
 """
 
 from __future__ import annotations
@@ -33,12 +33,15 @@ def _stream() -> ctypes.c_void_p:
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
-def features(grads: torch.Tensor, fields_km: torch.Tensor) -> torch.Tensor:
+def features(grads: torch.Tensor, fields_km: torch.Tensor, grad_scale: float, compress: bool = False) -> torch.Tensor:
     """[N, H, W] gradient and field (km/s) -> network input [N, 2, H, W]:
-    the gradient over its per-model RMS, and (v - 3) / 1.5."""
-    g = grads.float()
-    rms = g.flatten(1).pow(2).mean(1).sqrt().clamp_min(1e-30)
-    return torch.stack([g / rms[:, None, None], (fields_km.float() - VEL_CENTRE_KM) / VEL_SCALE_KM], 1).contiguous()
+    grad_scale * gradient (one constant for all models, so the input keeps
+    the gradient's magnitude, which grows with the size of the model error;
+    asinh-compressed when `compress`), and (v - 3) / 1.5."""
+    g = grads.float() * grad_scale
+    if compress:
+        g = torch.asinh(g)
+    return torch.stack([g, (fields_km.float() - VEL_CENTRE_KM) / VEL_SCALE_KM], 1).contiguous()
 
 
 class Preconditioner:
@@ -51,6 +54,8 @@ class Preconditioner:
         self._h = h
         self.max_batch, self.height, self.width = int(max_batch), int(height), int(width)
         self.num_params = int(self._lib.wg_pc_num_params())
+        self.grad_scale = 1.0  # input scaling of the gradient channel (fit_scale)
+        self.compress = False  # asinh compression of the gradient channel
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -69,11 +74,22 @@ class Preconditioner:
             raise ValueError(f"expected {self.num_params} parameters")
         _lib.check(self._lib.wg_pc_set_params(self._h, ctypes.c_void_p(p.ctypes.data), self.num_params), pc=True)
 
+    def fit_scale(self, grads: torch.Tensor) -> float:
+        """Sets grad_scale to 1 / (median per-model RMS of `grads`)."""
+        rms = grads.float().flatten(1).pow(2).mean(1).sqrt()
+        self.grad_scale = float(1.0 / max(float(rms.median()), 1e-30))
+        return self.grad_scale
+
     def save(self, path: str) -> None:
-        np.save(path, self.get_params())
+        """Parameters (PyTorch order), then grad_scale and compress, one .npy."""
+        extra = np.array([self.grad_scale, 1.0 if self.compress else 0.0], np.float32)
+        np.save(path, np.concatenate([self.get_params(), extra]))
 
     def load(self, path: str) -> None:
-        self.set_params(np.load(path, allow_pickle=False))
+        v = np.load(path, allow_pickle=False)
+        self.set_params(v[: self.num_params])
+        self.grad_scale = float(v[self.num_params])
+        self.compress = bool(v[self.num_params + 1] != 0.0)
 
     def _check_x(self, x: torch.Tensor) -> torch.Tensor:
         if x.dim() != 4 or x.shape[1] != 2 or x.shape[2] != self.height or x.shape[3] != self.width:
@@ -111,7 +127,7 @@ class Preconditioner:
     # -- use in refine ------------------------------------------------------
     def direction(self, grads: torch.Tensor, fields_km: torch.Tensor) -> torch.Tensor:
         """Predicted model update (km/s) for velocity-space gradients."""
-        return self.forward(features(grads, fields_km)) * TARGET_SCALE_KM
+        return self.forward(features(grads, fields_km, self.grad_scale, self.compress)) * TARGET_SCALE_KM
 
     def hook(self):
         """refine_batch `precondition` callable: minus the predicted update,
@@ -131,10 +147,10 @@ def make_pairs(physics, n: int, seed: int, bands_hz=(6.0, 9.0, 12.0, 15.0), batc
     (the flat / curved / faulted families plus a Gaussian anomaly), the
     velocity-space gradient at the start model in a passband drawn per model,
     and the target (true - start) / 0.3 km/s.  Returns device tensors
-    x [n, 2, 70, 70] and target [n, 70, 70]."""
+    grads [n, 70, 70], fields_km [n, 70, 70] and target [n, 70, 70]."""
     from . import synthetic
 
-    xs, ts = [], []
+    gs, fs, ts = [], [], []
     rng = np.random.default_rng(seed)
     for s in range(0, n, batch):
         m = min(batch, n - s)
@@ -145,15 +161,19 @@ def make_pairs(physics, n: int, seed: int, bands_hz=(6.0, 9.0, 12.0, 15.0), batc
         wav = torch.stack([torch.from_numpy(physics.band_wavelet(b)) for b in bands]).cuda()
         obs = physics.simulate_torch(true_km, wavelets=wav)
         _, grads = physics.loss_and_grad_torch(start_km, obs, wavelets=wav)
-        xs.append(features(grads, start_km))
+        gs.append(grads)
+        fs.append(start_km.float())
         ts.append(((true_km - start_km) / TARGET_SCALE_KM).float())
-    return torch.cat(xs), torch.cat(ts)
+    return torch.cat(gs), torch.cat(fs), torch.cat(ts)
 
 
-def train(net: Preconditioner, x: torch.Tensor, target: torch.Tensor, epochs: int = 20, batch: int = 128,
-          lr: float = 2e-3, weight_decay: float = 1e-4, seed: int = 0) -> list[float]:
-    """Mini-batch AdamW with a cosine learning-rate decay; returns the mean
-    loss of each epoch."""
+def train(net: Preconditioner, grads: torch.Tensor, fields_km: torch.Tensor, target: torch.Tensor,
+          epochs: int = 20, batch: int = 128, lr: float = 2e-3, weight_decay: float = 1e-4,
+          seed: int = 0) -> list[float]:
+    """Fits grad_scale, then mini-batch AdamW with a cosine learning-rate
+    decay; returns the mean loss of each epoch."""
+    net.fit_scale(grads)
+    x = features(grads, fields_km, net.grad_scale, net.compress)
     n = x.shape[0]
     g = torch.Generator(device="cpu").manual_seed(seed)
     steps = epochs * ((n + batch - 1) // batch)

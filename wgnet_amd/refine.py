@@ -17,9 +17,15 @@ decisions is the same either way.
 
 `precondition`, when given, maps the velocity-space gradient (the hook point of
 refine.py:105-106) and the current field, both km/s-based [n, 70, 70] device
-tensors, to the velocity-space vector that the chain rule carries to the
-search parameters.  A model whose preconditioned line search fails to reduce
-the misfit continues with plain gradients (its L-BFGS memory is reset).
+tensors, to a velocity-space vector that the chain rule carries to the search
+parameters.  Two ways of using it (`pc_mode`):
+  "first"    L-BFGS runs on plain gradients; the first step of each band goes
+             along the preconditioned direction (rescaled to the gradient's
+             norm) when that is a descent direction of the true misfit, and
+             is judged by the Armijo test with the true slope.
+  "replace"  L-BFGS runs on the preconditioned gradients; a model whose
+             preconditioned line search fails to reduce the misfit continues
+             with plain gradients (its L-BFGS memory is reset).
 """
 
 from __future__ import annotations
@@ -71,17 +77,41 @@ def lbfgs_direction(grad: np.ndarray, s_hist: list, y_hist: list, rho: list) -> 
     return -q
 
 
-class _Scene:
-    """Static data of one model's scene: parameters are [coarse 81 | 6 per
-    active Gaussian] in scaled units (refine.py:62-90)."""
+class _Group:
+    """Models that share one set of active Gaussian slots: their scenes are
+    assembled, and chain-ruled, together."""
 
-    def __init__(self, dense_km: np.ndarray, gaussians: np.ndarray, amp_keep_km: float, device):
+    def __init__(self, dense_km: np.ndarray, active: np.ndarray, device):
         self.device = torch.device(device)
-        self.active = np.flatnonzero(np.abs(gaussians[:, 5]) >= amp_keep_km)
-        self.n_g = len(self.active)
+        self.active = active
+        self.n_g = len(active)
+        self.slots = torch.as_tensor(active, device=self.device)
+        self.dense = torch.as_tensor(np.asarray(dense_km, np.float32), device=self.device)  # [m, 70, 70]
+
+    def assemble(self, rows: list[int], vecs: list[np.ndarray], need_grad: bool):
+        """fields [n, 70, 70] of members `rows` at parameter vectors `vecs`,
+        and the leaf tensor of those vectors."""
+        n, n_c = len(rows), COARSE * COARSE
+        v = torch.tensor(np.stack(vecs), dtype=torch.float32, device=self.device, requires_grad=need_grad)
+        coarse = v[:, :n_c].reshape(n, 1, COARSE, COARSE)
+        field = self.dense[rows] + F.interpolate(coarse, size=(GRID, GRID), mode="bilinear", align_corners=True)[:, 0]
+        if self.n_g:
+            field = field + render_residual(v[:, n_c:].reshape(n, self.n_g, 6), self.slots)
+        return field, v
+
+
+class _Scene:
+    """One model's scene: parameters are [coarse 81 | 6 per active Gaussian]
+    in scaled units (refine.py:62-90); `group`/`row` locate its dense field."""
+
+    def __init__(self, group: _Group, row: int, gaussians: np.ndarray):
+        self.group, self.row = group, row
+        self.device = group.device
+        self.active = group.active
+        self.n_g = group.n_g
         self.n_c = COARSE * COARSE
-        self.slots = torch.as_tensor(self.active, device=self.device)
-        self.dense_t = torch.as_tensor(np.asarray(dense_km, np.float32), device=self.device)
+        self.slots = group.slots
+        self.dense_t = group.dense[row]
         self.scales = np.concatenate([np.full(self.n_c, COARSE_SCALE), np.tile(np.array(GAUSS_SCALES), self.n_g)])
         self.x0 = np.concatenate([np.zeros(self.n_c), gaussians[self.active].astype(np.float64).reshape(-1)])
 
@@ -93,23 +123,9 @@ class _Scene:
             out[n_c:] = clamp_gaussians(torch.from_numpy(out[n_c:].reshape(n_g, 6))).numpy().reshape(-1)
         return out
 
-    def assemble(self, vec: np.ndarray, need_grad: bool):
-        n_c, n_g = self.n_c, self.n_g
-        coarse = torch.tensor(vec[:n_c].reshape(1, 1, COARSE, COARSE), dtype=torch.float32, device=self.device,
-                              requires_grad=need_grad)
-        g = torch.tensor(vec[n_c:].reshape(n_g, 6), dtype=torch.float32, device=self.device, requires_grad=need_grad)
-        field = self.dense_t + F.interpolate(coarse, size=(GRID, GRID), mode="bilinear", align_corners=True)[0, 0]
-        if n_g:
-            field = field + render_residual(g[None], self.slots)[0]
-        return field, coarse, g
-
-    def chain(self, field, coarse, g, gv: torch.Tensor, retain: bool) -> np.ndarray:
-        """Parameter-space gradient (scaled units) of <field, gv>."""
-        gc, gg = torch.autograd.grad((field * gv.to(self.device)).sum(), [coarse, g], allow_unused=True,
-                                     retain_graph=retain)
-        parts = [gc.detach().cpu().numpy().reshape(-1)]
-        parts.append(gg.detach().cpu().numpy().reshape(-1) if gg is not None else np.zeros(6 * self.n_g))
-        return np.concatenate(parts).astype(np.float64) * self.scales
+    def field(self, vec: np.ndarray) -> np.ndarray:
+        with torch.no_grad():
+            return self.group.assemble([self.row], [vec], False)[0][0].cpu().numpy()
 
 
 class _Request(NamedTuple):
@@ -119,7 +135,8 @@ class _Request(NamedTuple):
     precondition: bool
 
 
-def _program(scene: _Scene, budget_solves: int, bands_hz: Sequence[float | None], use_pc: bool):
+def _program(scene: _Scene, budget_solves: int, bands_hz: Sequence[float | None], use_pc: bool,
+             pc_mode: str = "first"):
     """One model's projected L-BFGS with the band schedule (refine.py:92-169).
     Yields _Request; receives (loss, plain_grad, pc_grad) for gradient
     requests (pc_grad None when not preconditioning) and the loss otherwise.
@@ -131,18 +148,30 @@ def _program(scene: _Scene, budget_solves: int, bands_hz: Sequence[float | None]
     solves += 1
     per_band = budget_solves // len(bands_hz)
     scales = scene.scales
+    first = use_pc and pc_mode == "first"
+    if first:
+        use_pc = False
 
     for band in bands_hz:
         band_budget = solves + per_band
-        cur_loss, g_plain, g_pc = yield _Request(True, x, band, use_pc)
+        cur_loss, g_plain, g_pc = yield _Request(True, x, band, use_pc or first)
         solves += 2
         cur_grad = g_pc if use_pc else g_plain
         s_hist: list = []
         y_hist: list = []
         rho: list = []
+        learned = first and g_pc is not None
         while solves + 1 < band_budget:
             d = lbfgs_direction(cur_grad, s_hist, y_hist, rho)
             slope = float(cur_grad @ d)
+            if learned:
+                # the band's first step goes along the learned direction,
+                # judged by the true misfit slope
+                learned = False
+                dl = -g_pc * (np.linalg.norm(cur_grad) / max(float(np.linalg.norm(g_pc)), 1e-300))
+                sl = float(cur_grad @ dl)
+                if np.isfinite(sl) and sl < 0.0:
+                    d, slope = dl, sl
             if not np.isfinite(slope) or slope >= 0.0:
                 s_hist, y_hist, rho = [], [], []
                 d = -cur_grad
@@ -198,6 +227,36 @@ def _misfit_f64(pred: torch.Tensor, observed: torch.Tensor) -> torch.Tensor:
     return 0.5 * (d * d).sum(1) / (o * o).sum(1)
 
 
+def _fields(scenes, pending, ids, need_grad):
+    """Fields [n, 70, 70] of the requests `ids` (in that order), assembled per
+    group; returns them with the (group, leaf, positions) needed to chain-rule."""
+    by_group: dict = {}
+    for pos, i in enumerate(ids):
+        by_group.setdefault(id(scenes[i].group), []).append(pos)
+    out = [None] * len(ids)
+    parts = []
+    for positions in by_group.values():
+        grp = scenes[ids[positions[0]]].group
+        f, leaf = grp.assemble([scenes[ids[p]].row for p in positions], [pending[ids[p]].vec for p in positions],
+                               need_grad)
+        for k, p in enumerate(positions):
+            out[p] = f[k]
+        parts.append((f, leaf, positions))
+    return torch.stack(out), parts
+
+
+def _chain(field_t, parts, gv: torch.Tensor, retain: bool) -> list[np.ndarray]:
+    """Per request, the parameter gradient (unscaled) of <field, gv>."""
+    res = [None] * gv.shape[0]
+    for f, leaf, positions in parts:
+        w = gv[torch.tensor(positions, device=gv.device)].to(f.device)
+        (g,) = torch.autograd.grad((f * w).sum(), [leaf], retain_graph=retain)
+        g = g.detach().cpu().numpy().astype(np.float64)
+        for k, p in enumerate(positions):
+            res[p] = g[k]
+    return res
+
+
 def _drive(scenes, programs, physics: WavePhysics, observed_full: torch.Tensor, precondition):
     """Runs the programs in lockstep; returns their return values."""
     dev = physics.device
@@ -223,7 +282,7 @@ def _drive(scenes, programs, physics: WavePhysics, observed_full: torch.Tensor, 
         grad_ids = [i for i, r in pending.items() if r.need_grad]
         if loss_ids:
             with torch.no_grad():
-                fields = torch.stack([scenes[i].assemble(pending[i].vec, False)[0] for i in loss_ids]).to(dev)
+                fields = _fields(scenes, pending, loss_ids, False)[0].to(dev)
             wav = torch.stack([wavelet(pending[i].band) for i in loss_ids])
             pred = physics.simulate_torch(fields, wavelets=wav)
             obs = torch.stack([observed(i, pending[i].band) for i in loss_ids])
@@ -231,8 +290,8 @@ def _drive(scenes, programs, physics: WavePhysics, observed_full: torch.Tensor, 
             for k, i in enumerate(loss_ids):
                 answers[i] = float(losses[k])
         if grad_ids:
-            built = [scenes[i].assemble(pending[i].vec, True) for i in grad_ids]
-            fields = torch.stack([b[0].detach() for b in built]).to(dev)
+            field_t, parts = _fields(scenes, pending, grad_ids, True)
+            fields = field_t.detach().to(dev)
             wav = torch.stack([wavelet(pending[i].band) for i in grad_ids])
             obs = torch.stack([observed(i, pending[i].band) for i in grad_ids])
             losses, grads = physics.loss_and_grad_torch(fields, obs, wavelets=wav)
@@ -242,12 +301,16 @@ def _drive(scenes, programs, physics: WavePhysics, observed_full: torch.Tensor, 
                 sel = torch.tensor(pc_rows, device=dev)
                 pcg = precondition(grads[sel], fields[sel])
             losses = losses.cpu().numpy()
+            plain = _chain(field_t, parts, grads, retain=pcg is not None)
+            pc_full = None
+            if pcg is not None:
+                gv = torch.zeros_like(grads)
+                gv[torch.tensor(pc_rows, device=dev)] = pcg
+                pc_full = _chain(field_t, parts, gv, retain=False)
             for k, i in enumerate(grad_ids):
-                field, coarse, g = built[k]
                 want_pc = pending[i].precondition and pcg is not None
-                plain = scenes[i].chain(field, coarse, g, grads[k], retain=want_pc)
-                pc = scenes[i].chain(field, coarse, g, pcg[pc_rows.index(k)], retain=False) if want_pc else None
-                answers[i] = (float(losses[k]), plain, pc)
+                sc = scenes[i].scales
+                answers[i] = (float(losses[k]), plain[k] * sc, pc_full[k] * sc if want_pc else None)
         for i, ans in answers.items():
             try:
                 pending[i] = programs[i].send(ans)
@@ -259,9 +322,8 @@ def _drive(scenes, programs, physics: WavePhysics, observed_full: torch.Tensor, 
 
 def _result(scene: _Scene, out, seconds: float) -> RefineResult:
     x, initial_loss, final_loss, solves, history, pc = out
-    with torch.no_grad():
-        initial_field = scene.assemble(scene.project(scene.x0), False)[0].cpu().numpy()
-        final_field = scene.assemble(x, False)[0].cpu().numpy()
+    initial_field = scene.field(scene.project(scene.x0))
+    final_field = scene.field(x)
     n_c, n_g = scene.n_c, scene.n_g
     return RefineResult(
         initial_field_km=initial_field,
@@ -287,6 +349,7 @@ def refine_batch(
     amp_keep_km: float = ACTIVE_AMP_KM,
     device: torch.device | str = "cpu",
     precondition: Callable[[torch.Tensor, torch.Tensor], torch.Tensor] | None = None,
+    pc_mode: str = "first",
 ) -> list[RefineResult]:
     """Refines B scenes together: dense_km [B, 70, 70], gaussians [B, 1225, 6]
     activated, observed_full [B, shots, nt, 70] (numpy or tensor).  Scene
@@ -295,8 +358,19 @@ def refine_batch(
     obs = observed_full if isinstance(observed_full, torch.Tensor) else torch.from_numpy(
         np.ascontiguousarray(observed_full, np.float32))
     obs = obs.to(physics.device, torch.float32)
-    scenes = [_Scene(dense_km[b], gaussians[b], amp_keep_km, device) for b in range(len(dense_km))]
-    programs = [_program(sc, budget_solves, tuple(bands_hz), precondition is not None) for sc in scenes]
+    B = len(dense_km)
+    actives = [np.flatnonzero(np.abs(gaussians[b][:, 5]) >= amp_keep_km) for b in range(B)]
+    groups: dict = {}
+    for b in range(B):
+        groups.setdefault(actives[b].tobytes(), []).append(b)
+    scenes: list = [None] * B
+    for members in groups.values():
+        grp = _Group(np.stack([dense_km[b] for b in members]), actives[members[0]], device)
+        for row, b in enumerate(members):
+            scenes[b] = _Scene(grp, row, gaussians[b])
+    if pc_mode not in ("first", "replace"):
+        raise ValueError("pc_mode must be 'first' or 'replace'")
+    programs = [_program(sc, budget_solves, tuple(bands_hz), precondition is not None, pc_mode) for sc in scenes]
     outs = _drive(scenes, programs, physics, obs, precondition)
     el = time.perf_counter() - t0
     return [_result(sc, out, el) for sc, out in zip(scenes, outs)]
