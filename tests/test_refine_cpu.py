@@ -148,11 +148,17 @@ def test_refine_batch_driver_on_a_linear_problem():
         assert batch[k].solves == one.solves <= 32
         assert batch[k].final_loss == one.final_loss
         assert batch[k].final_loss < batch[k].initial_loss
-    # a "preconditioner" that points uphill: every preconditioned line search
-    # fails, so each model switches to plain gradients and still descends
+    # a "preconditioner" that points uphill.  replace mode: every
+    # preconditioned line search fails, so each model switches to plain
+    # gradients and still descends.  first mode: the uphill direction is
+    # never taken, so the run is plain L-BFGS's exactly.
     anti = refine_batch(dense, init, obs, phys, budget_solves=30, bands_hz=bands,
-                        precondition=lambda g, f: -g)
+                        precondition=lambda g, f: -g, pc_mode="replace")
     for k in range(3):
         assert not anti[k].preconditioned
         assert any(h["preconditioned"] and not h["accepted"] for h in anti[k].history)
         assert anti[k].final_loss < anti[k].initial_loss
+    first = refine_batch(dense, init, obs, phys, budget_solves=30, bands_hz=bands,
+                         precondition=lambda g, f: -g, pc_mode="first")
+    for k in range(3):
+        assert first[k].final_loss == batch[k].final_loss and first[k].solves == batch[k].solves
