@@ -118,6 +118,33 @@ int32_t wg_loss_grad_wavelets_dev(wg_engine *e, int32_t B, const float *vel_dev,
                                   float *traces_dev, double *energy_dev,
                                   void *stream);
 
+/* Uneven batches (no reference counterpart; the reference solves one model
+ * with the engine's own source list per call, wave_wgpu.rs:761-785).  Each
+ * of the B models has `shots` unit slots; unit u = b * shots + k carries its
+ * own source position and wavelet, and units whose active flag is 0 are
+ * padding (a model with fewer shots): they cost no time steps and give zero
+ * traces and zero gradient, so the misfit and gradient of each model are
+ * those of its active units alone.
+ *   wg_source_corners: host helper, bilinear corners (padded cell index) and
+ *     weights of n positions xz [n, 2] in metres, as the kernels use them.
+ *   src_cell_dev [B*shots*4], src_w_dev [B*shots*4]: per-unit corners;
+ *   active_dev [B*shots] (NULL: all active); wav_dev [B, shots, nt]. */
+int32_t wg_source_corners(const wg_engine *e, int32_t n, const float *xz,
+                          int32_t *cell, float *w);
+int32_t wg_loss_grad_units_dev(wg_engine *e, int32_t B,
+                               const int32_t *src_cell_dev,
+                               const float *src_w_dev,
+                               const uint8_t *active_dev, const float *vel_dev,
+                               const float *wav_dev, const float *obs_dev,
+                               float *loss_dev, float *grad_dev,
+                               float *traces_dev, double *energy_dev,
+                               void *stream);
+int32_t wg_simulate_units_dev(wg_engine *e, int32_t B,
+                              const int32_t *src_cell_dev,
+                              const float *src_w_dev, const uint8_t *active_dev,
+                              const float *vel_dev, const float *wav_dev,
+                              float *traces_dev, void *stream);
+
 /* Transpose of the linearised engine at vel_dev, for the dot-product test
  * and gradient checks (no reference counterpart: the reference exposes the
  * adjoint only through evaluate_adjoint, splatAdjoint.wgsl:58-131).
@@ -152,6 +179,12 @@ int32_t wg_set_snapshot_budget(wg_engine *e, uint64_t bytes);
 /* 1 when the engine runs the HBM-streamed kernels (grid too large for the
  * resident ones), 0 otherwise. */
 int32_t wg_engine_streamed(const wg_engine *e);
+/* Tile shapes of the HBM-streamed kernels (no reference counterpart): a
+ * 256-thread workgroup computes 64x16 (0, default), 128x8 (1), 32x32 (2) or
+ * 64x32 (3) cells.  Results do not depend on the shape; an autotuner times
+ * them per grid and sets the fastest. */
+int32_t wg_stream_tiles(void);
+int32_t wg_set_stream_tile(wg_engine *e, int32_t tile);
 
 /* Kernel timing (no reference counterpart; the reference records no
  * timestamps, wave_wgpu.rs:829).  When enabled, HIP events are recorded on the

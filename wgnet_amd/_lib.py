@@ -61,11 +61,16 @@ SIGNATURES = {
     "wg_simulate_dev": (_i32, [_vp, _i32, _vp, _vp, _vp]),
     "wg_loss_grad_dev": (_i32, [_vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "wg_loss_grad_wavelets_dev": (_i32, [_vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "wg_source_corners": (_i32, [_vp, _i32, _vp, _vp, _vp]),
+    "wg_loss_grad_units_dev": (_i32, [_vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "wg_simulate_units_dev": (_i32, [_vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "wg_adjoint_apply_dev": (_i32, [_vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp]),
     "wg_born_dev": (_i32, [_vp, _i32, _vp, _vp, _vp, _vp]),
     "wg_simulate_wavelets_dev": (_i32, [_vp, _i32, _vp, _vp, _vp, _vp]),
     "wg_set_snapshot_budget": (_i32, [_vp, ctypes.c_uint64]),
     "wg_engine_streamed": (_i32, [_vp]),
+    "wg_stream_tiles": (_i32, []),
+    "wg_set_stream_tile": (_i32, [_vp, _i32]),
     "wg_timing_enable": (_i32, [_vp, _i32]),
     "wg_timing_read": (_i32, [_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64),
                               ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]),

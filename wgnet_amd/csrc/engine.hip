@@ -110,6 +110,12 @@ struct wg_engine {
     // per-(model, shot) wavelets [B, shots, nt] of the call in progress, or
     // null for the engine wavelet (set by WavScope under the engine mutex)
     const float* wav_dev = nullptr;
+    int tile = 0;  // tile shape of the HBM-streamed kernels (wg::StepParams::tile)
+    // per-unit source corners and activity mask of the call in progress
+    // (uneven batches), or null (set by UnitScope under the engine mutex)
+    const int* unit_src_cell = nullptr;
+    const float* unit_src_w = nullptr;
+    const unsigned char* unit_active = nullptr;
     std::mutex mu;
     // kernel timing: (start, stop, launches) records of forward / adjoint launches
     bool timing = false;
@@ -128,6 +134,13 @@ struct wg_engine {
         G.band_z0 = band_z0;
         G.band_rows = band_rows;
         G.n_ent = (int)ent_r.size();
+        G.src_by_unit = 0;
+        G.active = unit_active;
+        if (unit_src_cell) {
+            G.src_cell = unit_src_cell;
+            G.src_w = unit_src_w;
+            G.src_by_unit = 1;
+        }
         return G;
     }
 };
@@ -246,6 +259,20 @@ struct WavScope {
     wg_engine* e;
     WavScope(wg_engine* en, const float* w) : e(en) { e->wav_dev = w; }
     ~WavScope() { e->wav_dev = nullptr; }
+};
+
+struct UnitScope {
+    wg_engine* e;
+    UnitScope(wg_engine* en, const int* cell, const float* w, const unsigned char* act) : e(en) {
+        e->unit_src_cell = cell;
+        e->unit_src_w = w;
+        e->unit_active = act;
+    }
+    ~UnitScope() {
+        e->unit_src_cell = nullptr;
+        e->unit_src_w = nullptr;
+        e->unit_active = nullptr;
+    }
 };
 
 wg::FwdParams fwd_params(wg_engine* e, const float* vel, float* traces) {
@@ -378,6 +405,7 @@ wg::StepParams step_params(wg_engine* e, const float* vel, int units, int unit0)
     p.wav_stride = wav_stride(e);
     p.units = units;
     p.unit0 = unit0;
+    p.tile = e->tile;
     return p;
 }
 
@@ -399,7 +427,7 @@ int stream_forward(wg_engine* e, const StreamPlan& P, int units, int unit0, cons
                    long long wav_stride, float* traces, bool store, hipStream_t st) {
     const size_t cells = (size_t)e->g.nx * e->g.nz;
     const size_t sbytes = 8 * units * cells * sizeof(float);
-    HIPCK(hipMemsetAsync(e->fstate.p, 0, sbytes, st));
+    LAUNCH(wg::launch_zero((float*)e->fstate.p, sbytes / sizeof(float), st));
     wg::StepParams p = step_params(e, vel, units, unit0);
     p.wavelet = wav;
     p.wav_stride = wav_stride;
@@ -428,8 +456,8 @@ int stream_adjoint(wg_engine* e, const StreamPlan& P, int units, int unit0, cons
     const wg::Grid& g = e->g;
     const size_t cells = (size_t)g.nx * g.nz;
     const size_t sbytes = 8 * units * cells * sizeof(float);
-    HIPCK(hipMemsetAsync(e->astate.p, 0, 10 * units * cells * sizeof(float), st));
-    HIPCK(hipMemsetAsync(gunit + (size_t)unit0 * cells, 0, units * cells * sizeof(float), st));
+    LAUNCH(wg::launch_zero((float*)e->astate.p, 10 * units * cells, st));
+    LAUNCH(wg::launch_zero(gunit + (size_t)unit0 * cells, units * cells, st));
     wg::StepParams pf = step_params(e, vel, units, unit0);
     pf.snaps = (float*)e->snaps.p;
     pf.seg_len = P.seg_len;
@@ -813,10 +841,12 @@ int32_t wg_simulate_dev(wg_engine* e, int32_t B, const float* vel_dev, float* tr
 
 static int32_t loss_grad_entry(wg_engine* e, int32_t B, const float* vel_dev, const float* wav_dev,
                                const float* obs_dev, float* loss_dev, float* grad_dev, float* traces_dev,
-                               double* energy_dev, void* stream) {
+                               double* energy_dev, void* stream, const int* src_cell = nullptr,
+                               const float* src_w = nullptr, const unsigned char* active = nullptr) {
     if (!e || !vel_dev || !obs_dev || !loss_dev || !grad_dev || B < 0) return fail(WG_E_INVALID, "invalid argument");
     Guard G(e);
     WavScope W(e, wav_dev);
+    UnitScope U(e, src_cell, src_w, active);
     HIPCK(hipSetDevice(e->device));
     hipStream_t st = (hipStream_t)stream;
     int rc;
@@ -843,6 +873,26 @@ int32_t wg_loss_grad_wavelets_dev(wg_engine* e, int32_t B, const float* vel_dev,
     if (!wav_dev) return fail(WG_E_INVALID, "invalid argument");
     return loss_grad_entry(e, B, vel_dev, wav_dev, obs_dev, loss_dev, grad_dev, traces_dev, energy_dev, stream);
 }
+
+int32_t wg_source_corners(const wg_engine* e, int32_t n, const float* xz, int32_t* cell, float* w) {
+    if (!e || n < 0 || (n > 0 && (!xz || !cell || !w))) return fail(WG_E_INVALID, "invalid argument");
+    for (int k = 0; k < n; ++k) {
+        if (!std::isfinite(xz[2 * k]) || !std::isfinite(xz[2 * k + 1]))
+            return fail(WG_E_INVALID, "source positions must be finite");
+        corners(xz[2 * k], xz[2 * k + 1], e->cfg.h, e->g.pad, e->g.nx, e->g.nz, &cell[4 * k], &w[4 * k]);
+    }
+    return WG_OK;
+}
+
+int32_t wg_loss_grad_units_dev(wg_engine* e, int32_t B, const int32_t* src_cell_dev, const float* src_w_dev,
+                               const uint8_t* active_dev, const float* vel_dev, const float* wav_dev,
+                               const float* obs_dev, float* loss_dev, float* grad_dev, float* traces_dev,
+                               double* energy_dev, void* stream) {
+    if (!src_cell_dev || !src_w_dev || !wav_dev) return fail(WG_E_INVALID, "invalid argument");
+    return loss_grad_entry(e, B, vel_dev, wav_dev, obs_dev, loss_dev, grad_dev, traces_dev, energy_dev, stream,
+                           src_cell_dev, src_w_dev, active_dev);
+}
+
 
 int32_t wg_adjoint_apply_dev(wg_engine* e, int32_t B, const float* vel_dev, const float* resid_dev, float* grad_dev,
                              float* dwav_dev, float* traces_dev, void* stream) {
@@ -907,7 +957,7 @@ int32_t wg_born_dev(wg_engine* e, int32_t B, const float* vel_dev, const float* 
             const int nb = std::min(P.models, B - b0);
             if ((rc = stream_forward(e, P, nb * S, b0 * S, vel_dev, wav_ptr(e), wav_stride(e), (float*)e->traces.p, true, st)))
                 return rc;
-            HIPCK(hipMemsetAsync(e->astate.p, 0, 8 * nb * S * cells * sizeof(float), st));
+            LAUNCH(wg::launch_zero((float*)e->astate.p, 8 * (size_t)nb * S * cells, st));
             wg::StepParams p = step_params(e, vel_dev, nb * S, b0 * S);
             p.traces = dtraces_dev;
             p.seg_len = P.seg_len;
@@ -935,10 +985,9 @@ int32_t wg_born_dev(wg_engine* e, int32_t B, const float* vel_dev, const float* 
     return WG_OK;
 }
 
-int32_t wg_simulate_wavelets_dev(wg_engine* e, int32_t B, const float* vel_dev, const float* wav_dev,
-                                 float* traces_dev, void* stream) {
-    if (!e || !vel_dev || !wav_dev || !traces_dev || B < 0) return fail(WG_E_INVALID, "invalid argument");
-    Guard G(e);
+static int32_t simulate_wavelets_locked(wg_engine* e, int32_t B, const float* vel_dev, const float* wav_dev,
+                                        float* traces_dev, void* stream) {
+    if (!vel_dev || !wav_dev || !traces_dev || B < 0) return fail(WG_E_INVALID, "invalid argument");
     HIPCK(hipSetDevice(e->device));
     hipStream_t st = (hipStream_t)stream;
     int rc;
@@ -952,6 +1001,22 @@ int32_t wg_simulate_wavelets_dev(wg_engine* e, int32_t B, const float* vel_dev, 
     return WG_OK;
 }
 
+int32_t wg_simulate_wavelets_dev(wg_engine* e, int32_t B, const float* vel_dev, const float* wav_dev,
+                                 float* traces_dev, void* stream) {
+    if (!e) return fail(WG_E_INVALID, "invalid argument");
+    Guard G(e);
+    return simulate_wavelets_locked(e, B, vel_dev, wav_dev, traces_dev, stream);
+}
+
+int32_t wg_simulate_units_dev(wg_engine* e, int32_t B, const int32_t* src_cell_dev, const float* src_w_dev,
+                              const uint8_t* active_dev, const float* vel_dev, const float* wav_dev,
+                              float* traces_dev, void* stream) {
+    if (!e || !src_cell_dev || !src_w_dev) return fail(WG_E_INVALID, "invalid argument");
+    Guard G(e);
+    UnitScope U(e, src_cell_dev, src_w_dev, active_dev);
+    return simulate_wavelets_locked(e, B, vel_dev, wav_dev, traces_dev, stream);
+}
+
 int32_t wg_set_snapshot_budget(wg_engine* e, uint64_t bytes) {
     if (!e || bytes == 0) return fail(WG_E_INVALID, "budget must be positive");
     Guard G(e);
@@ -960,6 +1025,15 @@ int32_t wg_set_snapshot_budget(wg_engine* e, uint64_t bytes) {
 }
 
 int32_t wg_engine_streamed(const wg_engine* e) { return e ? (e->streamed ? 1 : 0) : -1; }
+
+int32_t wg_stream_tiles(void) { return wg::kStreamTiles; }
+
+int32_t wg_set_stream_tile(wg_engine* e, int32_t tile) {
+    if (!e || tile < 0 || tile >= wg::kStreamTiles) return fail(WG_E_INVALID, "tile must be in [0, wg_stream_tiles())");
+    Guard G(e);
+    e->tile = tile;
+    return WG_OK;
+}
 
 int32_t wg_timing_enable(wg_engine* e, int32_t on) {
     if (!e) return fail(WG_E_INVALID, "null engine");

@@ -4,6 +4,7 @@
 // These are small HBM-bound passes over the gathers [B, shots, nt, nrec] and
 // the per-unit padded gradients; the float32 tree of the reference's 256-wide
 // residual workgroup is reproduced exactly so the loss matches bit for bit.
+#include <algorithm>
 #include "wave_kernels.h"
 
 #pragma clang fp contract(off)
@@ -137,6 +138,20 @@ hipError_t launch_fold(const Grid& g, const float* grad_unit, const float* vel, 
     const long long n = (long long)B * g.nvx * g.nvz;
     hipLaunchKernelGGL(k_fold, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, grad_unit, vel,
                        energy, B, grad_vis);
+    return hipGetLastError();
+}
+
+// Zero fill (grid-stride), used instead of memset nodes so that the engine's
+// work can be captured in a HIP graph.
+__global__ void k_zero(float* p, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0.0f;
+}
+
+hipError_t launch_zero(float* p, size_t n, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    const size_t blocks = std::min<size_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_zero, dim3((unsigned)blocks), dim3(256), 0, st, p, n);
     return hipGetLastError();
 }
 

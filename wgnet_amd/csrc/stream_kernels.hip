@@ -6,8 +6,10 @@
 // (double-buffered, neighbours read it) and zeta (own cell only, in place);
 // adjoint lambda (double-buffered likewise), P and Z (double-buffered by the
 // step parity, as splatAdjoint.wgsl:70-73) and the gradient.  A 256-thread
-// workgroup computes a 64 x 16 tile of inner cells, four cells per thread
-// (rows lz, lz + 4, lz + 8, lz + 12 of column lx); the neighbour values it
+// workgroup computes a TX x TZ tile of inner cells (64 x 16 by default;
+// 128 x 8, 32 x 32 and 64 x 32 are the other shapes an engine can select),
+// TX * TZ / 256 cells per thread (rows lz, lz + RZ, ... of column lx,
+// RZ = 256 / TX); the neighbour values it
 // needs (u or V = v^2 dt^2 lambda, a*psi, T, U1) are staged in LDS with a
 // 2-cell halo.  Tiles with no C-PML cell along an axis skip that axis's strips
 // and memory variables (they are zero there).  Per-cell arithmetic is
@@ -18,10 +20,20 @@
 namespace wg {
 namespace st {
 
-constexpr int TX = 64, TZ = 16, NT = 256;
-constexpr int RZ = NT / TX;   // rows per pass
-constexpr int CPT = TZ / RZ;  // cells per thread
-constexpr int HX = TX + 4, HZ = TZ + 4;
+constexpr int NT = 256;
+
+template <int TX_, int TZ_>
+struct Tile {
+    static constexpr int TX = TX_, TZ = TZ_;
+    static constexpr int RZ = NT / TX;   // rows per pass
+    static constexpr int CPT = TZ / RZ;  // cells per thread
+    static constexpr int HX = TX + 4, HZ = TZ + 4;
+    static_assert(NT % TX == 0 && TZ % RZ == 0, "tile must tile the workgroup");
+};
+
+#define WG_TILE_CONSTANTS                                              \
+    constexpr int TX = TL::TX, TZ = TL::TZ, RZ = TL::RZ, CPT = TL::CPT; \
+    constexpr int HX = TL::HX, HZ = TL::HZ;
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 __device__ __forceinline__ float clampv(float v, float lo, float hi) { return v < lo ? lo : (v > hi ? hi : v); }
@@ -39,15 +51,16 @@ __device__ __forceinline__ float vis_raw(const float* vvis, const Grid& g, int x
 // --------------------------------------------------------------------------
 // Forward step (lib.rs:145-220 for one t, every unit of the chunk).
 // --------------------------------------------------------------------------
-template <bool STORE, bool BORN>
+template <class TL, bool STORE, bool BORN>
 __global__ __launch_bounds__(NT) void k_fwd_step(StepParams P) {
+    WG_TILE_CONSTANTS
     __shared__ float sU[HZ * HX];
     __shared__ float sAX[TZ * HX];
     __shared__ float sAZ[HZ * TX];
     const Grid& g = P.g;
     const int ul = blockIdx.z;             // unit within the chunk
     const int unit = P.unit0 + ul;
-    const int b = unit / g.shots, s = unit - (unit / g.shots) * g.shots;
+    const int b = unit / g.shots, s = src_slot(P.geo, unit, unit - (unit / g.shots) * g.shots);
     const size_t cells = (size_t)g.nx * g.nz;
     const int t = P.t, par = t & 1;
     float* Ucur = P.state + ((size_t)(0 + par) * P.units + ul) * cells;
@@ -68,6 +81,11 @@ __global__ __launch_bounds__(NT) void k_fwd_step(StepParams P) {
     const int tid = threadIdx.x;
     const int xi0 = blockIdx.x * TX, zi0 = blockIdx.y * TZ;  // tile origin, inner coords
 
+    if (!unit_active(P.geo, unit)) {  // padding unit of an uneven batch
+        if (P.traces && blockIdx.x == 0 && blockIdx.y == 0)
+            for (int r = tid; r < g.nrec; r += NT) P.traces[((size_t)unit * g.nt + t) * g.nrec + r] = 0.0f;
+        return;
+    }
     // Receivers record u^t before the update (lib.rs:146-149): one tile per unit.
     if (P.traces && blockIdx.x == 0 && blockIdx.y == 0) {
         float* tr = P.traces + ((size_t)unit * g.nt + t) * g.nrec;
@@ -172,15 +190,16 @@ __global__ __launch_bounds__(NT) void k_fwd_step(StepParams P) {
 // --------------------------------------------------------------------------
 // Adjoint step (splatAdjoint.wgsl:58-131 for one t, every unit of the chunk).
 // --------------------------------------------------------------------------
-template <bool DWAV>
+template <class TL, bool DWAV>
 __global__ __launch_bounds__(NT) void k_adj_step(StepParams P) {
+    WG_TILE_CONSTANTS
     __shared__ float sV[HZ * HX];
     __shared__ float sTX[TZ * HX], sUX[TZ * HX];
     __shared__ float sTZ[HZ * TX], sUZ[HZ * TX];
     const Grid& g = P.g;
     const int ul = blockIdx.z;
     const int unit = P.unit0 + ul;
-    const int b = unit / g.shots, s = unit - (unit / g.shots) * g.shots;
+    const int b = unit / g.shots, s = src_slot(P.geo, unit, unit - (unit / g.shots) * g.shots);
     const size_t cells = (size_t)g.nx * g.nz;
     const int t = P.t, q = (g.nt - 1 - t) & 1;
     // arrays: lam0 lam1 | Px0 Px1 | Zx0 Zx1 | Pz0 Pz1 | Zz0 Zz1 (halves by q)
@@ -206,6 +225,10 @@ __global__ __launch_bounds__(NT) void k_adj_step(StepParams P) {
     const int tid = threadIdx.x;
     const int xi0 = blockIdx.x * TX, zi0 = blockIdx.y * TZ;
 
+    if (!unit_active(P.geo, unit)) {  // padding unit; its gradient rows are zeroed by the host
+        if (DWAV && blockIdx.x == 0 && blockIdx.y == 0 && tid == 0) P.dwav[(size_t)unit * g.nt + t] = 0.0f;
+        return;
+    }
     if (DWAV && blockIdx.x == 0 && blockIdx.y == 0 && tid == 0) {
         float acc = 0.0f;
 #pragma unroll
@@ -312,20 +335,50 @@ __global__ __launch_bounds__(NT) void k_adj_step(StepParams P) {
 
 }  // namespace st
 
-hipError_t launch_fwd_step(const StepParams& p, int units, bool store, bool born, hipStream_t st) {
-    if (units < 1 || units > 65535) return hipErrorInvalidValue;
-    const dim3 grid((p.g.ni + st::TX - 1) / st::TX, (p.g.nzi + st::TZ - 1) / st::TZ, units);
-    void (*fn)(StepParams) = born ? st::k_fwd_step<false, true> : (store ? st::k_fwd_step<true, false> : st::k_fwd_step<false, false>);
+namespace {
+
+template <class T>
+hipError_t fwd_step_t(const StepParams& p, int units, bool store, bool born, hipStream_t st) {
+    const dim3 grid((p.g.ni + T::TX - 1) / T::TX, (p.g.nzi + T::TZ - 1) / T::TZ, units);
+    void (*fn)(StepParams) =
+        born ? st::k_fwd_step<T, false, true> : (store ? st::k_fwd_step<T, true, false> : st::k_fwd_step<T, false, false>);
     hipLaunchKernelGGL(fn, grid, dim3(st::NT), 0, st, p);
     return hipGetLastError();
 }
 
-hipError_t launch_adj_step(const StepParams& p, int units, hipStream_t st) {
-    if (units < 1 || units > 65535) return hipErrorInvalidValue;
-    const dim3 grid((p.g.ni + st::TX - 1) / st::TX, (p.g.nzi + st::TZ - 1) / st::TZ, units);
-    void (*fn)(StepParams) = p.dwav ? st::k_adj_step<true> : st::k_adj_step<false>;
+template <class T>
+hipError_t adj_step_t(const StepParams& p, int units, hipStream_t st) {
+    const dim3 grid((p.g.ni + T::TX - 1) / T::TX, (p.g.nzi + T::TZ - 1) / T::TZ, units);
+    void (*fn)(StepParams) = p.dwav ? st::k_adj_step<T, true> : st::k_adj_step<T, false>;
     hipLaunchKernelGGL(fn, grid, dim3(st::NT), 0, st, p);
     return hipGetLastError();
+}
+
+using T0 = st::Tile<64, 16>;
+using T1 = st::Tile<128, 8>;
+using T2 = st::Tile<32, 32>;
+using T3 = st::Tile<64, 32>;
+
+}  // namespace
+
+hipError_t launch_fwd_step(const StepParams& p, int units, bool store, bool born, hipStream_t st) {
+    if (units < 1 || units > 65535) return hipErrorInvalidValue;
+    switch (p.tile) {
+        case 1: return fwd_step_t<T1>(p, units, store, born, st);
+        case 2: return fwd_step_t<T2>(p, units, store, born, st);
+        case 3: return fwd_step_t<T3>(p, units, store, born, st);
+        default: return fwd_step_t<T0>(p, units, store, born, st);
+    }
+}
+
+hipError_t launch_adj_step(const StepParams& p, int units, hipStream_t st) {
+    if (units < 1 || units > 65535) return hipErrorInvalidValue;
+    switch (p.tile) {
+        case 1: return adj_step_t<T1>(p, units, st);
+        case 2: return adj_step_t<T2>(p, units, st);
+        case 3: return adj_step_t<T3>(p, units, st);
+        default: return adj_step_t<T0>(p, units, st);
+    }
 }
 
 }  // namespace wg

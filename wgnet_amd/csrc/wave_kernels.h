@@ -36,7 +36,25 @@ struct Geometry {
     const int* ent_r;
     const float* ent_w;
     int band_z0, band_rows, n_ent;
+    // Uneven batches: per-unit source slots (src_cell / src_w hold one entry
+    // of 4 corners per unit) and an activity mask; inactive units are padding
+    // and produce zero traces and gradients.  Defaults: slot = unit % shots,
+    // every unit active.
+    int src_by_unit;
+    const unsigned char* active;
 };
+
+__device__ __forceinline__ int src_slot(const Geometry& geo, int unit, int shot) {
+    return geo.src_by_unit ? unit : shot;
+}
+__device__ __forceinline__ bool unit_active(const Geometry& geo, int unit) { return !geo.active || geo.active[unit]; }
+
+// Outputs of a padding unit: zero `n` floats at a (and `m` at b), block-strided.
+// Kept out of line so that it costs the resident kernels no registers.
+static __device__ __noinline__ void zero_unit_outputs(float* a, int n, float* b, int m) {
+    for (int j = threadIdx.x; j < n; j += blockDim.x) a[j] = 0.0f;
+    for (int j = threadIdx.x; j < m; j += blockDim.x) b[j] = 0.0f;
+}
 
 struct FwdParams {
     Grid g;
@@ -85,8 +103,11 @@ struct StepParams {
     const float* resid;      // adjoint: [all units, nt, nrec]
     float* grad;             // adjoint: [all units, nx * nz], accumulated
     float* dwav;             // adjoint: [all units, nt] or null
+    int tile;                // tile shape: 0 64x16, 1 128x8, 2 32x32, 3 64x32
 };
+constexpr int kStreamTiles = 4;
 
+hipError_t launch_zero(float* p, size_t n, hipStream_t st);
 hipError_t launch_fwd_step(const StepParams& p, int units, bool store, bool born, hipStream_t st);
 hipError_t launch_adj_step(const StepParams& p, int units, hipStream_t st);
 

@@ -358,7 +358,11 @@ __global__ __launch_bounds__(NT) void k_forward(FwdParams P) {
     const int tid = threadIdx.x;
     const int unit = P.unit0 + blockIdx.x;
     const int b = unit / g.shots;
-    const int s = unit - b * g.shots;
+    const int s = src_slot(P.geo, unit, unit - b * g.shots);
+    if (!unit_active(P.geo, unit)) {  // padding unit of an uneven batch
+        zero_unit_outputs(P.traces + (size_t)unit * g.nt * g.nrec, g.nt * g.nrec, nullptr, 0);
+        return;
+    }
 
     for (int j = tid; j < L.PR; j += NT) lds[j] = 0.0f;
     for (int j = tid; j < 3 * (g.nx + g.nz); j += NT) PR[j] = P.geo.prof[j];
@@ -687,7 +691,12 @@ __global__ __launch_bounds__(NT) void k_adjoint(AdjParams P) {
     const int tid = threadIdx.x;
     const int unit = P.unit0 + blockIdx.x;
     const int b = unit / g.shots;
-    const int s = unit - b * g.shots;
+    const int s = src_slot(P.geo, unit, unit - b * g.shots);
+    if (!unit_active(P.geo, unit)) {  // padding unit of an uneven batch
+        zero_unit_outputs(P.grad + (size_t)unit * g.nx * g.nz, g.nx * g.nz,
+                          DWAV ? P.dwav + (size_t)unit * g.nt : nullptr, DWAV ? g.nt : 0);
+        return;
+    }
 
     for (int j = tid; j < L.PR; j += NT) lds[j] = 0.0f;
     for (int j = tid; j < 3 * (g.nx + g.nz); j += NT) PR[j] = P.geo.prof[j];
