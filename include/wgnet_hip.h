@@ -197,6 +197,15 @@ int32_t wg_timing_enable(wg_engine *e, int32_t on);
 int32_t wg_timing_read(wg_engine *e, double *fwd_ms, int64_t *fwd_launches,
                        double *adj_ms, int64_t *adj_launches);
 
+/* Causal low-pass of gathers along time, the data half of frequency
+ * continuation (python/wgnet/physics.py:48-57, a conv1d with the taps
+ * left-padded by ntaps - 1 zeros): for each of the nseries blocks of x_dev
+ * [nseries, nt, nrec], y[t][r] = sum_k taps[k] x[t + k - (ntaps - 1)][r],
+ * k ascending, float32.  y_dev must not alias x_dev.  Enqueued on stream. */
+int32_t wg_causal_filter_dev(const float *x_dev, int64_t nseries, int32_t nt,
+                             int32_t nrec, const float *taps_dev,
+                             int32_t ntaps, float *y_dev, void *stream);
+
 /* Reference Ricker (geology-core lib.rs:12-21), float32 with expf. */
 void wg_ricker_wavelet(int32_t samples, float dt, float peak_hz, float *out);
 

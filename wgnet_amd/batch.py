@@ -29,7 +29,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .physics import lowpass_kernel
+from .physics import causal_filter, lowpass_kernel
 from .wave_gauss_rs import PyWaveEngine
 
 
@@ -260,12 +260,9 @@ def _stream() -> ctypes.c_void_p:
 
 
 def _bandlimit(d: torch.Tensor, fmax_hz: float, dt: float) -> torch.Tensor:
-    """Causal low-pass along time of [..., nt, nrec] (physics.py:48-57)."""
-    k = torch.from_numpy(lowpass_kernel(fmax_hz, dt)).to(d.device).view(1, 1, -1)
-    nt, nr = d.shape[-2], d.shape[-1]
-    x = d.reshape(-1, nt, nr).transpose(1, 2).reshape(-1, 1, nt)
-    y = torch.nn.functional.conv1d(torch.nn.functional.pad(x, (k.shape[-1] - 1, 0)), k)
-    return y.view(-1, nr, nt).transpose(1, 2).reshape(d.shape).contiguous()
+    """Causal low-pass along time of device gathers [..., nt, nrec]
+    (physics.py:48-57), HIP kernel."""
+    return causal_filter(d, lowpass_kernel(fmax_hz, dt))
 
 
 def random_solves(n: int, seed: int, grids=(70, 100, 141), nts=(1000, 1500, 2000), bands=(6.0, 9.0, 12.0, 15.0),

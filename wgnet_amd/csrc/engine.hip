@@ -1026,6 +1026,15 @@ int32_t wg_set_snapshot_budget(wg_engine* e, uint64_t bytes) {
 
 int32_t wg_engine_streamed(const wg_engine* e) { return e ? (e->streamed ? 1 : 0) : -1; }
 
+int32_t wg_causal_filter_dev(const float* x_dev, int64_t nseries, int32_t nt, int32_t nrec, const float* taps_dev,
+                             int32_t ntaps, float* y_dev, void* stream) {
+    if (!x_dev || !y_dev || !taps_dev || nseries < 0 || nt < 1 || nrec < 1 || ntaps < 1 || x_dev == y_dev)
+        return fail(WG_E_INVALID, "invalid argument");
+    const hipError_t err = wg::launch_causal_fir(x_dev, nseries, nt, nrec, taps_dev, ntaps, y_dev, (hipStream_t)stream);
+    if (err != hipSuccess) return fail(WG_E_DEVICE, std::string("causal filter launch: ") + hipGetErrorString(err));
+    return WG_OK;
+}
+
 int32_t wg_stream_tiles(void) { return wg::kStreamTiles; }
 
 int32_t wg_set_stream_tile(wg_engine* e, int32_t tile) {

@@ -155,4 +155,32 @@ hipError_t launch_zero(float* p, size_t n, hipStream_t st) {
     return hipGetLastError();
 }
 
+// Causal low-pass along time (python/wgnet/physics.py:48-57: conv1d of each
+// receiver trace with the taps h, left-padded by L - 1 zeros):
+//   y[n][t][r] = sum_{k = max(0, L-1-t)}^{L-1} h[k] x[n][t + k - (L - 1)][r].
+// One thread per output sample; threads of a wave run along r, so every tap
+// reads consecutive floats.  HBM-bound (x is re-read from L2 per tap).
+__global__ __launch_bounds__(256) void k_causal_fir(const float* __restrict__ x, long long total, int nt, int nrec,
+                                                    const float* __restrict__ h, int L, float* __restrict__ y) {
+    const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= total) return;
+    const int r = (int)(idx % nrec);
+    const long long q = idx / nrec;
+    const int t = (int)(q % nt);
+    const long long n = q / nt;
+    const float* xs = x + (size_t)n * nt * nrec + r;
+    float acc = 0.0f;
+    for (int k = (L - 1 - t > 0 ? L - 1 - t : 0); k < L; ++k) acc += h[k] * xs[(size_t)(t + k - (L - 1)) * nrec];
+    y[idx] = acc;
+}
+
+hipError_t launch_causal_fir(const float* x, long long nseries, int nt, int nrec, const float* h, int L, float* y,
+                             hipStream_t st) {
+    const long long total = nseries * nt * nrec;
+    if (total == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_causal_fir, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, x, total, nt, nrec, h, L,
+                       y);
+    return hipGetLastError();
+}
+
 }  // namespace wg

@@ -68,6 +68,19 @@ def _device_f32(x, device) -> torch.Tensor:
     return torch.from_numpy(np.ascontiguousarray(x, dtype=np.float32)).to(device)
 
 
+def causal_filter(traces: torch.Tensor, taps: np.ndarray) -> torch.Tensor:
+    """Causal FIR along time of device gathers [..., nt, nrec] with the HIP
+    kernel (physics.py:48-57's conv1d, left padding len(taps) - 1)."""
+    x = traces.to(dtype=torch.float32).contiguous()
+    nt, nr = x.shape[-2], x.shape[-1]
+    y = torch.empty_like(x)
+    h = torch.from_numpy(np.ascontiguousarray(taps, np.float32)).to(x.device)
+    n = x.numel() // (nt * nr)
+    _lib.check(_lib.load().wg_causal_filter_dev(_dev_ptr(x), n, nt, nr, _dev_ptr(h), h.numel(), _dev_ptr(y),
+                                                _stream()))
+    return y
+
+
 class WaveMisfit(torch.autograd.Function):
     """losses[b] = 0.5 ||p_b - d_b||^2 / ||d_b||^2 for fields in km/s.
 
@@ -130,6 +143,8 @@ class WavePhysics:
         if fmax_hz is None:
             return traces
         kernel = lowpass_kernel(fmax_hz, self.dt)
+        if isinstance(traces, torch.Tensor) and traces.is_cuda:
+            return causal_filter(traces, kernel)
         is_np = not isinstance(traces, torch.Tensor)
         t = torch.from_numpy(np.ascontiguousarray(traces, dtype=np.float32)) if is_np else traces.float()
         lead = t.shape[:-2]
