@@ -45,7 +45,7 @@ def cpu_baseline(start, true, nt, dt, threads, units):
     w = O.ricker_f32(nt, dt, 15.0)
     S = SURFACE_SRC.shape[0]
     idx = np.arange(units)
-    b, s = idx // S, idx % S
+    b, s = (idx // S) % start.shape[0], idx % S
     src = SURFACE_SRC[s]
     vel = np.ascontiguousarray(start[b].reshape(units, -1))
     obs = np.ascontiguousarray(O.batch_forward_f32(g, src, np.ascontiguousarray(true[b].reshape(units, -1)), w, threads))
@@ -156,9 +156,12 @@ def main() -> int:
     inner, R = (nx - 4) * (nz - 4), ph.engine.nrec
     # Algorithmic HBM bytes per unit: forward writes the snapshot (4 B per inner
     # cell-step) and the gathers (4 B per receiver-step); the adjoint reads the
-    # snapshot and the residual.
-    fwd_bytes = (4 * inner * nt + 4 * R * nt) * units
-    adj_bytes = (4 * inner * nt + 4 * R * nt) * units
+    # snapshot and the residual.  A step may take several launches (chunks of
+    # the overlapped pipeline): bytes per launch = bytes of the units of all
+    # timed steps / launches.
+    unit_bytes = 4 * inner * nt + 4 * R * nt
+    fwd_bytes = unit_bytes * units * args.steps / max(fn.value, 1)
+    adj_bytes = unit_bytes * units * args.steps / max(an.value, 1)
     f_avg = fms.value / max(fn.value, 1) / 1e3
     a_avg = ams.value / max(an.value, 1) / 1e3
     if a_avg >= f_avg:
@@ -194,14 +197,15 @@ def main() -> int:
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic_from_profiles(kern),
-            "algorithmic_bytes_per_launch": kbytes,
+            "algorithmic_bytes_per_launch": int(kbytes),
             "launch_ms": {"k_forward_fx": round(1e3 * f_avg, 3), "k_adjoint_fx": round(1e3 * a_avg, 3)},
+            "launches_per_step": {"k_forward_fx": fn.value / args.steps, "k_adjoint_fx": an.value / args.steps},
             "shot_solve_hbm_frac": round(ss_bytes * value / 1e9 / HBM_PEAK_GBS, 4),
         },
     }
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
-        units_cpu = 2 * threads
+        units_cpu = 64 * threads  # about 10-15 s of CPU work
         rate, cel = cpu_baseline(start, true, nt, dt, threads, units_cpu)
         out["cpu_baseline"] = {
             "value": round(rate, 3),
@@ -510,7 +514,7 @@ def bench_large(args) -> int:
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
         nt_cpu = 500
-        units_cpu = 2 * threads
+        units_cpu = 64 * threads  # about 10-15 s of CPU work
         rate, cel = cpu_baseline_large(start, src, nvx, nvz, nt_cpu, dt, threads, units_cpu)
         out["cpu_baseline"] = {
             "value": round(rate * nt_cpu / nt, 5),

@@ -184,6 +184,12 @@ int32_t wg_engine_streamed(const wg_engine *e);
  * 64x32 (3) cells.  Results do not depend on the shape; an autotuner times
  * them per grid and sets the fastest. */
 int32_t wg_stream_tiles(void);
+/* Overlapped chunk pipeline of the resident loss/gradient (no reference
+ * counterpart; default on): a batch larger than one unit per compute unit is
+ * cut into device-sized model chunks, and each chunk's forward runs on a
+ * second stream beside the previous chunk's adjoint.  Results are identical
+ * either way; off runs the chunks one after another on the caller's stream. */
+int32_t wg_set_overlap(wg_engine *e, int32_t on);
 int32_t wg_set_stream_tile(wg_engine *e, int32_t tile);
 
 /* Kernel timing (no reference counterpart; the reference records no

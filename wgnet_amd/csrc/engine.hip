@@ -104,6 +104,10 @@ struct wg_engine {
     bool streamed = false;
     DevBuf fstate, astate, ckpt;
     hipStream_t stream = nullptr;  // host-pointer calls
+    hipStream_t side = nullptr;    // second stream of the overlapped chunk pipeline
+    hipEvent_t ev_pipe[2] = {nullptr, nullptr};
+    int cus = 256;                 // compute units of the device
+    bool overlap = true;           // overlap chunk forwards with the previous chunk's adjoint
     hipStream_t last = nullptr;    // stream of the last enqueued call
     bool last_valid = false;
     size_t snap_budget = 0;
@@ -342,21 +346,22 @@ struct Timed {
 int stream_simulate(wg_engine* e, int b0, int nb, const float* vel, const float* wav, long long wav_stride,
                     float* traces, hipStream_t st);
 
-int run_chunk_forward(wg_engine* e, int b0, int nb, const float* vel, float* traces, bool store, hipStream_t st) {
+int run_chunk_forward(wg_engine* e, int b0, int nb, const float* vel, float* traces, bool store, hipStream_t st,
+                      size_t snap_unit_off = 0) {
     if (e->streamed && !store) return stream_simulate(e, b0, nb, vel, wav_ptr(e), wav_stride(e), traces, st);
     wg::FwdParams p = fwd_params(e, vel, traces);
     p.unit0 = b0 * e->cfg.shots;
-    p.snaps = store ? (float*)e->snaps.p : nullptr;
+    p.snaps = store ? (float*)e->snaps.p + snap_unit_off * e->g.nt * e->g.inner : nullptr;
     Timed tm(e, &e->ev_fwd, st);
     LAUNCH(wg::launch_forward(p, nb * e->cfg.shots, store, false, st));
     return WG_OK;
 }
 
 int run_chunk_adjoint(wg_engine* e, int b0, int nb, const float* vel, const float* resid, float* gunit, float* dwav,
-                      hipStream_t st) {
+                      hipStream_t st, size_t snap_unit_off = 0) {
     wg::AdjParams p = adj_params(e, vel, resid, gunit);
     p.unit0 = b0 * e->cfg.shots;
-    p.snaps = (const float*)e->snaps.p;
+    p.snaps = (const float*)e->snaps.p + snap_unit_off * e->g.nt * e->g.inner;
     p.dwav = dwav;
     Timed tm(e, &e->ev_adj, st);
     LAUNCH(wg::launch_adjoint(p, nb * e->cfg.shots, st));
@@ -505,6 +510,15 @@ int stream_simulate(wg_engine* e, int b0, int nb, const float* vel, const float*
     return WG_OK;
 }
 
+// The side stream (non-blocking, default priority) and the two pipeline
+// events, created on first use.
+int ensure_pipeline(wg_engine* e) {
+    if (!e->side) HIPCK(hipStreamCreateWithFlags(&e->side, hipStreamNonBlocking));
+    for (auto& ev : e->ev_pipe)
+        if (!ev) HIPCK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    return WG_OK;
+}
+
 // Loss and gradient of B models on stream st.  scale/energy may be preset by
 // the caller (host path) or computed here (device path).
 int loss_grad_core(wg_engine* e, int B, const float* vel, const float* obs, float* loss, float* grad,
@@ -540,6 +554,37 @@ int loss_grad_core(wg_engine* e, int B, const float* vel, const float* obs, floa
         return WG_OK;
     }
     const int mc = models_per_chunk(e, B);
+    // Overlapped pipeline: chunks of one device's worth of units (one unit per
+    // CU), chunk i on stream i % 2 with snapshot region i % 2; chunk i's
+    // forward waits for chunk i - 1's forward, so it runs beside chunk i - 1's
+    // adjoint instead of after it.  Per-unit arithmetic is unchanged.
+    const int wave = std::max(1, e->cus / S);
+    const size_t per_model = snap_bytes_per_unit(e) * S;
+    const size_t fit = per_model ? e->snap_budget / per_model : (size_t)B;  // models whose snapshots fit
+    if (e->overlap && B > wave && fit >= (size_t)std::min(B, 2 * wave)) {
+        if ((rc = ensure_pipeline(e))) return rc;
+        if ((rc = ensure(e, e->snaps, per_model * std::min(B, 2 * wave)))) return rc;
+        hipStream_t sts[2] = {st, e->side};
+        for (int i = 0, b0 = 0; b0 < B; ++i, b0 += wave) {
+            const int nb = std::min(wave, B - b0);
+            hipStream_t cs = sts[i & 1];
+            const size_t off = (size_t)(i & 1) * wave * S;
+            if (i > 0) HIPCK(hipStreamWaitEvent(cs, e->ev_pipe[(i - 1) & 1], 0));
+            if ((rc = run_chunk_forward(e, b0, nb, vel, traces, true, cs, off))) return rc;
+            HIPCK(hipEventRecord(e->ev_pipe[i & 1], cs));
+            LAUNCH(wg::launch_residual(traces + b0 * per, obs + b0 * per, scale + b0, per, nb,
+                                       (float*)e->resid.p + b0 * per, (float*)e->partials.p + b0 * chunks, cs));
+            if ((rc = run_chunk_adjoint(e, b0, nb, vel, (const float*)e->resid.p, (float*)e->gunit.p, nullptr, cs,
+                                        off)))
+                return rc;
+        }
+        // join: the caller's stream waits for everything queued on the side stream
+        HIPCK(hipEventRecord(e->ev_pipe[1], e->side));
+        HIPCK(hipStreamWaitEvent(st, e->ev_pipe[1], 0));
+        LAUNCH(wg::launch_loss_sum((const float*)e->partials.p, chunks, B, energy, loss, st));
+        LAUNCH(wg::launch_fold(g, (const float*)e->gunit.p, vel, energy, B, grad, st));
+        return WG_OK;
+    }
     if ((rc = ensure(e, e->snaps, snap_bytes_per_unit(e) * S * mc))) return rc;
     for (int b0 = 0; b0 < B; b0 += mc) {
         const int nb = std::min(mc, B - b0);
@@ -697,6 +742,11 @@ int32_t wg_engine_create(const wg_config* cfg, wg_engine** out) {
         if (he != hipSuccess) rc = fail(WG_E_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(he));
     }
     if (!rc) {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, e->device) == hipSuccess && cus > 0)
+            e->cus = cus;
+    }
+    if (!rc) {
         size_t fr = 0, tot = 0;
         if (hipMemGetInfo(&fr, &tot) != hipSuccess) fr = (size_t)16 << 30;
         e->snap_budget = std::min<size_t>(fr / 2, (size_t)96 << 30);
@@ -725,6 +775,12 @@ void wg_engine_destroy(wg_engine* e) {
                       &e->obs, &e->loss, &e->grad, &e->fstate, &e->astate, &e->ckpt})
         if (b->p) (void)hipFree(b->p);
     if (e->stream) (void)hipStreamDestroy(e->stream);
+    if (e->side) {
+        (void)hipStreamSynchronize(e->side);
+        (void)hipStreamDestroy(e->side);
+    }
+    for (auto& ev : e->ev_pipe)
+        if (ev) (void)hipEventDestroy(ev);
     delete e;
 }
 
@@ -1036,6 +1092,13 @@ int32_t wg_causal_filter_dev(const float* x_dev, int64_t nseries, int32_t nt, in
 }
 
 int32_t wg_stream_tiles(void) { return wg::kStreamTiles; }
+
+int32_t wg_set_overlap(wg_engine* e, int32_t on) {
+    if (!e) return fail(WG_E_INVALID, "null engine");
+    Guard G(e);
+    e->overlap = on != 0;
+    return WG_OK;
+}
 
 int32_t wg_set_stream_tile(wg_engine* e, int32_t tile) {
     if (!e || tile < 0 || tile >= wg::kStreamTiles) return fail(WG_E_INVALID, "tile must be in [0, wg_stream_tiles())");
