@@ -37,7 +37,7 @@ template <bool PX, bool PZ>
 __device__ __forceinline__ float fwd_w(const float* U, int i, int sz, float u, float rh, float rh2,
                                        const float* AXc, const Pml& px_, float& psx, float& zex,
                                        const float* AZc, int az_stride, const Pml& pz_, float& psz, float& zez) {
-    float w = 0.0f;
+    float w;  // the x term, then += the z term (0 + x skipped: equal up to the sign of zero)
     const float um1x = U[i - 1], up1x = U[i + 1];
     const float um2x = U[i - 2], up2x = U[i + 2];
     const float d2x = (kD0 * u + kD1 * (up1x + um1x) - kD2 * (up2x + um2x)) * rh2;
@@ -47,11 +47,11 @@ __device__ __forceinline__ float fwd_w(const float* U, int i, int sz, float u, f
         const float ap2 = AXc[2], am2 = AXc[-2];
         const float d1_apsi = (kC1 * (ap1 - am1) - kC2 * (ap2 - am2)) * rh;
         const float tmp = px_.opb * d2x + px_.db * d1x + d1_apsi;
-        w += px_.opb * tmp + px_.a * zex;
+        w = px_.opb * tmp + px_.a * zex;
         psx = px_.b * d1x + px_.a * psx;
         zex = px_.b * tmp + px_.a * zex;
     } else {
-        w += d2x;
+        w = d2x;
     }
     const float um1z = U[i - sz], up1z = U[i + sz];
     const float um2z = U[i - 2 * sz], up2z = U[i + 2 * sz];

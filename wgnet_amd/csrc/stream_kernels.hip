@@ -295,15 +295,15 @@ __global__ __launch_bounds__(NT) void k_adj_step(StepParams P) {
         const int li = (lz + 2) * HX + (lx + 2);
         const float v0 = sV[li];
         const float rh = g.rh;
-        float w = 0.0f;
+        float w;  // x term, then += z term (0 + x skipped: equal up to the sign of zero)
         if (x < g.region_adj || x >= g.nx - g.region_adj) {
             const float opb[5] = {1.0f + pbx[x - 2], 1.0f + pbx[x - 1], 1.0f + pbx[x], 1.0f + pbx[x + 1], 1.0f + pbx[x + 2]};
             float Pv = PXr[i], Zv = ZXr[i];
-            w += adj_axis_pml(sTX + lz * HX + (lx + 2), sUX + lz * HX + (lx + 2), 1, opb, pax[x], v0, rh, Pv, Zv);
+            w = adj_axis_pml(sTX + lz * HX + (lx + 2), sUX + lz * HX + (lx + 2), 1, opb, pax[x], v0, rh, Pv, Zv);
             PXw[i] = Pv;
             ZXw[i] = Zv;
         } else {
-            w += adj_axis_int(sV, li, 1, v0, rh);
+            w = adj_axis_int(sV, li, 1, v0, rh);
         }
         if (z < g.region_adj || z >= g.nz - g.region_adj) {
             const float opb[5] = {1.0f + pbz[z - 2], 1.0f + pbz[z - 1], 1.0f + pbz[z], 1.0f + pbz[z + 1], 1.0f + pbz[z + 2]};

@@ -24,6 +24,10 @@ __device__ __forceinline__ float clampv(float v, float lo, float hi) { return v 
 // Opaque copies, taken once per phase: they stop the compiler from hoisting
 // per-slot addresses and profile loads out of the time loop, which would cost
 // one register per slot each; per slot, addresses are base + immediate.
+// Exact product of small ints (|a|, |b| < 2^23) on the full-rate 24-bit
+// multiplier; the laundered offsets below are recomputed every step.
+__device__ __forceinline__ int m24(int a, int b) { return __mul24(a, b); }
+
 __device__ __forceinline__ int opq(int v) {
     asm volatile("" : "+v"(v));
     return v;
@@ -190,12 +194,12 @@ __device__ __forceinline__ void f_compute(FSt<R::K, R::BX, R::BZ, STORE, BORN>& 
     const float* PR = opq_s(C.PR);
     const int x = xi + 2;
     const int z0 = zi0 + 2;
-    const int i0 = z0 * G::NX + x;
-    const int q0 = zi0 * G::NI + xi;
+    const int i0 = m24(z0, G::NX) + x;
+    const int q0 = m24(zi0, G::NI) + xi;
     constexpr int SW2 = 2 * G::SWF;
     // a*psi strip entries of this thread's first slot
-    const float* AX0 = C.AX + (zi0 * SW2 + (x < G::REG ? x : x - G::NX + SW2));
-    const float* AZ0 = C.AZ + ((R::ZOFF == 0 ? z0 : z0 - G::NZ + SW2) * G::NI + xi);
+    const float* AX0 = C.AX + (m24(zi0, SW2) + (x < G::REG ? x : x - G::NX + SW2));
+    const float* AZ0 = C.AZ + (m24(R::ZOFF == 0 ? z0 : z0 - G::NZ + SW2, G::NI) + xi);
     float dummy = 0.0f;
 #pragma unroll
     for (int k = 0; k < R::K; ++k) {
@@ -227,10 +231,10 @@ __device__ __forceinline__ void f_write(FSt<R::K, R::BX, R::BZ, STORE, BORN>& S,
     const float* PR = opq_s(C.PR);
     const int x = xi + 2;
     const int z0 = zi0 + 2;
-    const int i0 = z0 * G::NX + x;
+    const int i0 = m24(z0, G::NX) + x;
     constexpr int SW2 = 2 * G::SWF;
-    float* AX0 = C.AX + (zi0 * SW2 + (x < G::REG ? x : x - G::NX + SW2));
-    float* AZ0 = C.AZ + ((R::ZOFF == 0 ? z0 : z0 - G::NZ + SW2) * G::NI + xi);
+    float* AX0 = C.AX + (m24(zi0, SW2) + (x < G::REG ? x : x - G::NX + SW2));
+    float* AZ0 = C.AZ + (m24(R::ZOFF == 0 ? z0 : z0 - G::NZ + SW2, G::NI) + xi);
 #pragma unroll
     for (int k = 0; k < R::K; ++k) {
         if (k == R::K - 1 && !pl.last_ok) break;
@@ -444,11 +448,11 @@ __device__ __forceinline__ void a_phase1(ASt<R::K, R::BX, R::BZ>& S, const Place
     const int sk = opq(so.slot);
     const int x = xi + 2;
     const int z0 = zi0 + 2;
-    const int i0 = z0 * G::NX + x;
+    const int i0 = m24(z0, G::NX) + x;
     const bool xstrip = x < G::SWA || x >= G::NX - G::SWA;
     const int xcol = x < G::SWA ? x : x - G::NX + SW2;
-    const int tx0 = zi0 * SW2 + xcol;
-    const int tz0 = (R::ZOFF == 0 ? z0 : z0 - G::NZ + SW2) * G::NI + xi;
+    const int tx0 = m24(zi0, SW2) + xcol;
+    const int tz0 = m24(R::ZOFF == 0 ? z0 : z0 - G::NZ + SW2, G::NI) + xi;
 #pragma unroll
     for (int k = 0; k < R::K; ++k) {
         if (k == R::K - 1 && !pl.last_ok) break;
@@ -472,8 +476,8 @@ __device__ __forceinline__ void a_phase1(ASt<R::K, R::BX, R::BZ>& S, const Place
             C.TZ[tz] = T;
             C.UZ[tz] = U1;
         } else {
-            if (z < G::SWA) C.TZ[z * G::NI + xi] = Vc;
-            else if (z >= G::NZ - G::SWA) C.TZ[(z - G::NZ + SW2) * G::NI + xi] = Vc;
+            if (z < G::SWA) C.TZ[m24(z, G::NI) + xi] = Vc;
+            else if (z >= G::NZ - G::SWA) C.TZ[m24(z - G::NZ + SW2, G::NI) + xi] = Vc;
         }
         if (k == sk) C.SRC[so.corner] = S.lam[k];
     }
@@ -491,11 +495,11 @@ __device__ __forceinline__ void a_phase2(ASt<R::K, R::BX, R::BZ>& S, const Place
     const int rk = opq(rslot);
     const int x = xi + 2;
     const int z0 = zi0 + 2;
-    const int i0 = z0 * G::NX + x;
-    const int q0 = zi0 * G::NI + xi;
-    const float* TX0 = C.TX + (zi0 * SW2 + (x < G::REGA ? x : x - G::NX + SW2));
-    const float* UX0 = C.UX + (zi0 * SW2 + (x < G::REGA ? x : x - G::NX + SW2));
-    const int tz0 = (R::ZOFF == 0 ? z0 : z0 - G::NZ + SW2) * G::NI + xi;
+    const int i0 = m24(z0, G::NX) + x;
+    const int q0 = m24(zi0, G::NI) + xi;
+    const float* TX0 = C.TX + (m24(zi0, SW2) + (x < G::REGA ? x : x - G::NX + SW2));
+    const float* UX0 = C.UX + (m24(zi0, SW2) + (x < G::REGA ? x : x - G::NX + SW2));
+    const int tz0 = m24(R::ZOFF == 0 ? z0 : z0 - G::NZ + SW2, G::NI) + xi;
     const float rh = C.rh;
 #pragma unroll
     for (int k = 0; k < R::K; ++k) {
@@ -504,11 +508,11 @@ __device__ __forceinline__ void a_phase2(ASt<R::K, R::BX, R::BZ>& S, const Place
         const int z = z0 + k * R::RP;
         const float Sn = snap_t[q0 + k * R::RP * G::NI];
         const float v0 = C.V[i];
-        float w = 0.0f;
+        float w;  // x term, then += z term (0 + x skipped: equal up to the sign of zero)
         if constexpr (R::BX) {
-            w += adj_axis_pml(TX0 + k * R::RP * SW2, UX0 + k * R::RP * SW2, 1, cx.opb, cx.a, v0, rh, S.Px[k], S.Zx[k]);
+            w = adj_axis_pml(TX0 + k * R::RP * SW2, UX0 + k * R::RP * SW2, 1, cx.opb, cx.a, v0, rh, S.Px[k], S.Zx[k]);
         } else {
-            w += adj_axis_int(C.V, i, 1, v0, rh);
+            w = adj_axis_int(C.V, i, 1, v0, rh);
         }
         if constexpr (R::BZ) {
             const float opb[5] = {PR[Prof<G>::OPBZ + z - 2], PR[Prof<G>::OPBZ + z - 1], PR[Prof<G>::OPBZ + z],
@@ -521,7 +525,7 @@ __device__ __forceinline__ void a_phase2(ASt<R::K, R::BX, R::BZ>& S, const Place
         float next = w + 2.0f * S.lam[k] - S.lamp[k];
         float gk = S.lam[k] * Sn;  // store_every = 1 (wgsl:234-236)
         if (k == rk) {  // receiver residuals, CSR order (wgsl:239-242)
-            const int o = (z - C.bz0) * G::NX + x;
+            const int o = m24(z - C.bz0, G::NX) + x;
             for (int e = C.BOFF[o]; e < C.BOFF[o + 1]; ++e) next += C.EW[e] * C.RES[C.ER[e]];
         }
         if (k == sk) gk += S.lam[k] * (so.coef * f_t);  // wgsl:245-249

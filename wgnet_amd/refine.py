@@ -164,6 +164,7 @@ def _program(scene: _Scene, budget_solves: int, bands_hz: Sequence[float | None]
         while solves + 1 < band_budget:
             d = lbfgs_direction(cur_grad, s_hist, y_hist, rho)
             slope = float(cur_grad @ d)
+            took_learned = False
             if learned:
                 # the band's first step goes along the learned direction,
                 # judged by the true misfit slope
@@ -172,6 +173,7 @@ def _program(scene: _Scene, budget_solves: int, bands_hz: Sequence[float | None]
                 sl = float(cur_grad @ dl)
                 if np.isfinite(sl) and sl < 0.0:
                     d, slope = dl, sl
+                    took_learned = True
             if not np.isfinite(slope) or slope >= 0.0:
                 s_hist, y_hist, rho = [], [], []
                 d = -cur_grad
@@ -205,7 +207,7 @@ def _program(scene: _Scene, budget_solves: int, bands_hz: Sequence[float | None]
                     break
                 step *= 0.5
             history.append({"band_hz": band, "solves": solves, "loss": cur_loss, "accepted": accepted,
-                            "preconditioned": use_pc})
+                            "preconditioned": use_pc or took_learned})
             if not accepted:
                 if use_pc:
                     # plain L-BFGS from here on: the last plain gradient is at x
