@@ -243,7 +243,7 @@ def bench_refine(args) -> int:
     import torch
 
     from wgnet_amd.physics import WavePhysics
-    from wgnet_amd.precond import Preconditioner, make_pairs, train
+    from wgnet_amd.precond import Preconditioner, make_pairs, pseudo_hessian_hook, train
     from wgnet_amd.refine import refine_batch
 
     S, nt, dt = 5, 1000, 0.001
@@ -289,11 +289,28 @@ def bench_refine(args) -> int:
         res = refine_batch(start_km, gauss, obs, ph, budget_solves=int(B * f), bands_hz=bands,
                            precondition=hook, pc_mode=args.pc_mode)
         pc_final[f] = np.array([r.final_loss for r in res])
-        print(f"refine: preconditioned {int(B * f)} solves: median final misfit {np.median(pc_final[f]):.4e}, "
-              f"reaches plain in {np.mean(pc_final[f] <= m_plain):.3f} of models", file=sys.stderr, flush=True)
-    need = np.full(len(m_plain), np.inf)
-    for f in reversed(fracs):
-        need[pc_final[f] <= m_plain] = f
+        print(f"refine: learned preconditioner {int(B * f)} solves: median final misfit "
+              f"{np.median(pc_final[f]):.4e}, reaches plain in {np.mean(pc_final[f] <= m_plain):.3f} of models",
+              file=sys.stderr, flush=True)
+    # the physics baseline: diagonal pseudo-Hessian (illumination) scaling; its
+    # one extra forward per model is counted as one solve
+    ph_hook = pseudo_hessian_hook(ph, torch.from_numpy(start_km).cuda())
+    phs_final = {}
+    for f in fracs:
+        res = refine_batch(start_km, gauss, obs, ph, budget_solves=int(B * f) - 1, bands_hz=bands,
+                           precondition=ph_hook, pc_mode="replace")
+        phs_final[f] = np.array([r.final_loss for r in res])
+        print(f"refine: pseudo-Hessian {int(B * f)} solves: median final misfit {np.median(phs_final[f]):.4e}, "
+              f"reaches plain in {np.mean(phs_final[f] <= m_plain):.3f} of models", file=sys.stderr, flush=True)
+
+    def solves_needed(final):
+        need = np.full(len(m_plain), np.inf)
+        for f in reversed(fracs):
+            need[final[f] <= m_plain] = f
+        return need
+
+    need = solves_needed(pc_final)
+    need_ph = solves_needed(phs_final)
     init = np.array([r.initial_loss for r in plain])
     out = {
         "metric": "refine solves-to-misfit: preconditioned solves to reach plain L-BFGS's final misfit / plain budget",
@@ -311,8 +328,13 @@ def bench_refine(args) -> int:
         "reached_at": {str(f): float(np.mean(need <= f)) for f in fracs},
         "median_final_misfit": {"initial": float(np.median(init)),
                                 **{f"plain_{f}": float(np.median(plain_final[f])) for f in fracs},
-                                **{f"pc_{f}": float(np.median(pc_final[f])) for f in fracs}},
-        "pc_better_at_equal_budget": {str(f): float(np.mean(pc_final[f] < plain_final[f])) for f in fracs},
+                                **{f"learned_{f}": float(np.median(pc_final[f])) for f in fracs},
+                                **{f"pseudo_hessian_{f}": float(np.median(phs_final[f])) for f in fracs}},
+        "learned_better_at_equal_budget": {str(f): float(np.mean(pc_final[f] < plain_final[f])) for f in fracs},
+        "pseudo_hessian": {"solves_needed_median": float(np.median(need_ph)),
+                           "reached_at": {str(f): float(np.mean(need_ph <= f)) for f in fracs},
+                           "better_at_equal_budget": {str(f): float(np.mean(phs_final[f] < plain_final[f]))
+                                                      for f in fracs}},
         "train": {"pairs_s": round(t_pairs, 2), "train_s": round(t_train, 2), "loss_first": hist[0],
                   "loss_last": hist[-1], "zero_prediction_loss": baseline, "grad_scale": net.grad_scale},
         "plain_refine_s": round(t_plain, 2),

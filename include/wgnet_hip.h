@@ -203,6 +203,14 @@ int32_t wg_timing_enable(wg_engine *e, int32_t on);
 int32_t wg_timing_read(wg_engine *e, double *fwd_ms, int64_t *fwd_launches,
                        double *adj_ms, int64_t *adj_launches);
 
+/* Source illumination illum_dev [B, nvz, nvx]: for each visible cell, the
+ * sum over the model's shots and all steps of the squared snapshot 2 v dt^2 w
+ * at that cell (the pseudo-Hessian diagonal that splatAdjoint.wgsl:113-117
+ * accumulates and the reference never reads back).  Runs one storing forward;
+ * resident grids only (WG_E_TOO_LARGE otherwise). */
+int32_t wg_illumination_dev(wg_engine *e, int32_t B, const float *vel_dev,
+                            float *illum_dev, void *stream);
+
 /* Causal low-pass of gathers along time, the data half of frequency
  * continuation (python/wgnet/physics.py:48-57, a conv1d with the taps
  * left-padded by ntaps - 1 zeros): for each of the nseries blocks of x_dev

@@ -355,3 +355,30 @@ def test_overlapped_chunk_pipeline_is_bit_identical(B):
     torch.cuda.synchronize()
     assert torch.equal(l_on, l_off)
     assert torch.equal(g_on, g_off)
+
+
+def test_illumination_matches_oracle_snapshots(orc):
+    """wg_illumination_dev: sum over shots and steps of the squared snapshot at
+    each visible cell, against the oracle's snapshots summed in float64."""
+    import torch
+
+    from wgnet_amd.physics import WavePhysics
+
+    nt = 300
+    ph = WavePhysics(shots=2, nt=nt, dt=0.001)
+    src = np.array([[100.0, 10.0], [555.0, 10.0]], np.float32)
+    ph.engine.set_sources(src)
+    vel = np.full((70, 70), 1800.0, np.float32)
+    vel[30:] = 2600.0
+    vel[50:] = 3400.0
+    il = ph.illumination_torch(torch.from_numpy(vel[None] / 1000.0).cuda())[0].cpu().numpy()
+    ref = np.zeros((70, 70))
+    for s in range(2):
+        g = orc.Geometry(nt=nt, dt=0.001, src_x=float(src[s, 0]), src_z=float(src[s, 1]))
+        _, snaps = orc.forward_f32(g, vel, ph.engine.wavelet(), snaps=True)
+        d = g.dims()
+        full = np.asarray(snaps, np.float64).reshape(nt, d.nz, d.nx)  # [nt, padded cells]
+        ref += (full[:, d.pad:d.pad + 70, d.pad:d.pad + 70] ** 2).sum(0)
+    err = np.linalg.norm(il - ref) / np.linalg.norm(ref)
+    print(f"illumination rel-L2 vs oracle {err:.2e}")
+    assert err <= 1e-6

@@ -70,6 +70,7 @@ SIGNATURES = {
     "wg_set_snapshot_budget": (_i32, [_vp, ctypes.c_uint64]),
     "wg_engine_streamed": (_i32, [_vp]),
     "wg_stream_tiles": (_i32, []),
+    "wg_illumination_dev": (_i32, [_vp, _i32, _vp, _vp, _vp]),
     "wg_set_overlap": (_i32, [_vp, _i32]),
     "wg_causal_filter_dev": (_i32, [_vp, ctypes.c_int64, _i32, _i32, _vp, _i32, _vp, _vp]),
     "wg_set_stream_tile": (_i32, [_vp, _i32]),

@@ -1091,6 +1091,29 @@ int32_t wg_causal_filter_dev(const float* x_dev, int64_t nseries, int32_t nt, in
     return WG_OK;
 }
 
+int32_t wg_illumination_dev(wg_engine* e, int32_t B, const float* vel_dev, float* illum_dev, void* stream) {
+    if (!e || !vel_dev || !illum_dev || B < 0) return fail(WG_E_INVALID, "invalid argument");
+    Guard G(e);
+    HIPCK(hipSetDevice(e->device));
+    hipStream_t st = (hipStream_t)stream;
+    int rc;
+    if ((rc = fence(e, st))) return rc;
+    if (B == 0) return WG_OK;
+    if (e->streamed) return fail(WG_E_TOO_LARGE, "illumination needs the resident kernels (grid fits one workgroup)");
+    const wg::Grid& g = e->g;
+    const int S = e->cfg.shots;
+    const size_t per = (size_t)S * g.nt * g.nrec;
+    if ((rc = ensure(e, e->traces, B * per * sizeof(float)))) return rc;
+    const int mc = models_per_chunk(e, B);
+    if ((rc = ensure(e, e->snaps, snap_bytes_per_unit(e) * S * mc))) return rc;
+    for (int b0 = 0; b0 < B; b0 += mc) {
+        const int nb = std::min(mc, B - b0);
+        if ((rc = run_chunk_forward(e, b0, nb, vel_dev, (float*)e->traces.p, true, st))) return rc;
+        LAUNCH(wg::launch_illum(g, (const float*)e->snaps.p, nb, illum_dev + (size_t)b0 * g.nvx * g.nvz, st));
+    }
+    return WG_OK;
+}
+
 int32_t wg_stream_tiles(void) { return wg::kStreamTiles; }
 
 int32_t wg_set_overlap(wg_engine* e, int32_t on) {

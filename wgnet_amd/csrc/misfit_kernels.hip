@@ -183,4 +183,35 @@ hipError_t launch_causal_fir(const float* x, long long nseries, int nt, int nrec
     return hipGetLastError();
 }
 
+// Source illumination of each visible cell: sum over the model's shots and
+// all steps of the squared snapshot 2 v dt^2 w at the cell's own padded
+// position (the pseudo-Hessian diagonal that splatAdjoint.wgsl:113-117
+// accumulates as stored^2 and never reads back), float32 in (shot, t) order.
+// snaps [units of the chunk, nt, inner]; one thread per visible cell.
+__global__ void k_illum(Grid g, const float* __restrict__ snaps, int nb, float* __restrict__ out) {
+    const int nvis = g.nvx * g.nvz;
+    const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= (long long)nb * nvis) return;
+    const int b = (int)(gid / nvis);
+    const int vi = (int)(gid - (long long)b * nvis);
+    const int vz = vi / g.nvx, vx = vi - (vi / g.nvx) * g.nvx;
+    const size_t q = (size_t)(vz + g.pad - 2) * g.ni + (vx + g.pad - 2);
+    float acc = 0.0f;
+    for (int s = 0; s < g.shots; ++s) {
+        const float* sn = snaps + ((size_t)b * g.shots + s) * g.nt * g.inner + q;
+        for (int t = 0; t < g.nt; ++t) {
+            const float v = sn[(size_t)t * g.inner];
+            acc += v * v;
+        }
+    }
+    out[gid] = acc;
+}
+
+hipError_t launch_illum(const Grid& g, const float* snaps, int nb, float* out, hipStream_t st) {
+    if (nb <= 0) return hipSuccess;
+    const long long n = (long long)nb * g.nvx * g.nvz;
+    hipLaunchKernelGGL(k_illum, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, snaps, nb, out);
+    return hipGetLastError();
+}
+
 }  // namespace wg

@@ -108,6 +108,7 @@ struct StepParams {
 constexpr int kStreamTiles = 4;
 
 hipError_t launch_zero(float* p, size_t n, hipStream_t st);
+hipError_t launch_illum(const Grid& g, const float* snaps, int nb, float* out, hipStream_t st);
 hipError_t launch_causal_fir(const float* x, long long nseries, int nt, int nrec, const float* h, int L, float* y,
                              hipStream_t st);
 hipError_t launch_fwd_step(const StepParams& p, int units, bool store, bool born, hipStream_t st);

@@ -185,6 +185,15 @@ class WavePhysics:
             return self.simulate_torch(fields_km)
         return self.simulate_torch(torch.from_numpy(np.asarray(fields_km, np.float32))).cpu().numpy()
 
+    def illumination_torch(self, fields_km: torch.Tensor) -> torch.Tensor:
+        """[B, nvz, nvx] km/s -> source illumination [B, nvz, nvx] (sum over
+        shots and steps of the squared snapshot at each cell)."""
+        vel = (_device_f32(fields_km, self.device) * 1000.0).contiguous()
+        out = torch.empty_like(vel)
+        _lib.check(self.engine._lib.wg_illumination_dev(self.engine.handle, vel.shape[0], _dev_ptr(vel),
+                                                        _dev_ptr(out), _stream()))
+        return out
+
     def loss_and_grad_torch(self, fields_km: torch.Tensor, observed: torch.Tensor,
                             traces_out: torch.Tensor | None = None,
                             wavelets=None) -> tuple[torch.Tensor, torch.Tensor]:

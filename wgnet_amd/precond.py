@@ -142,6 +142,21 @@ class Preconditioner:
         return precondition
 
 
+def pseudo_hessian_hook(physics, fields_km: torch.Tensor, eps: float = 0.01):
+    """refine_batch `precondition` callable dividing each model's gradient by
+    its normalised source illumination P + eps (P = illumination / its max,
+    computed once on `fields_km`, the starting fields of the batch): the
+    diagonal pseudo-Hessian scaling of the gradient, with no learning.  The
+    illumination costs one forward per model."""
+    illum = physics.illumination_torch(fields_km)
+    P = illum / illum.flatten(1).amax(1).clamp_min(1e-30)[:, None, None]
+
+    def precondition(grads: torch.Tensor, fields: torch.Tensor, ids) -> torch.Tensor:
+        return grads / (P[torch.as_tensor(ids, device=grads.device)] + eps)
+
+    return precondition
+
+
 def make_pairs(physics, n: int, seed: int, bands_hz=(6.0, 9.0, 12.0, 15.0), batch: int = 64):
     """Training pairs from the batched solver: synthetic (start, true) models
     (the flat / curved / faulted families plus a Gaussian anomaly), the

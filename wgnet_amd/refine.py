@@ -17,7 +17,8 @@ decisions is the same either way.
 
 `precondition`, when given, maps the velocity-space gradient (the hook point of
 refine.py:105-106) and the current field, both km/s-based [n, 70, 70] device
-tensors, to a velocity-space vector that the chain rule carries to the search
+tensors (and, if it takes a third argument, the models' indices in the batch),
+to a velocity-space vector that the chain rule carries to the search
 parameters.  Two ways of using it (`pc_mode`):
   "first"    L-BFGS runs on plain gradients; the first step of each band goes
              along the preconditioned direction (rescaled to the gradient's
@@ -222,6 +223,19 @@ def _program(scene: _Scene, budget_solves: int, bands_hz: Sequence[float | None]
     return x, initial_loss, final_loss, solves, history, use_pc
 
 
+def _takes_ids(fn) -> bool:
+    """Whether a precondition hook takes a third argument: the indices of
+    the models (in the refined batch) whose gradients it receives."""
+    import inspect
+
+    try:
+        params = inspect.signature(fn).parameters.values()
+    except (TypeError, ValueError):
+        return False
+    return sum(p.kind in (p.POSITIONAL_ONLY, p.POSITIONAL_OR_KEYWORD) for p in params) >= 3 or any(
+        p.kind == p.VAR_POSITIONAL for p in params)
+
+
 def _misfit_f64(pred: torch.Tensor, observed: torch.Tensor) -> torch.Tensor:
     """Per-model 0.5 ||p - d||^2 / ||d||^2 accumulated in float64."""
     d = (pred.double() - observed.double()).flatten(1)
@@ -301,7 +315,10 @@ def _drive(scenes, programs, physics: WavePhysics, observed_full: torch.Tensor, 
             pcg = None
             if pc_rows and precondition is not None:
                 sel = torch.tensor(pc_rows, device=dev)
-                pcg = precondition(grads[sel], fields[sel])
+                if _takes_ids(precondition):
+                    pcg = precondition(grads[sel], fields[sel], [grad_ids[k] for k in pc_rows])
+                else:
+                    pcg = precondition(grads[sel], fields[sel])
             losses = losses.cpu().numpy()
             plain = _chain(field_t, parts, grads, retain=pcg is not None)
             pc_full = None
