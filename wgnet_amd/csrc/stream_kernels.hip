@@ -154,12 +154,12 @@ __global__ __launch_bounds__(NT) void k_fwd_step(StepParams P) {
         const float vr = vis_raw(vvis, g, x, z);
         const float v = clampv(vr, g.vmin, g.vmax);
         const size_t q = (size_t)(z - 2) * g.ni + (x - 2);
-        if (STORE) P.snaps[((size_t)ul * P.seg_len + (t - P.snap_t0)) * g.inner + q] = 2.0f * v * g.dt2 * w;
+        if (STORE) P.snaps[((size_t)(t - P.snap_t0) * P.units + ul) * g.inner + q] = 2.0f * v * g.dt2 * w;
         float next = v * v * g.dt2 * w + 2.0f * u - Unxt[i];
         if (BORN) {
             const float m = (vr > g.vmin && vr < g.vmax) ? 1.0f : 0.0f;
             next += (vis_raw(P.dvel + (size_t)b * g.nvx * g.nvz, g, x, z) * m) *
-                    P.born_snaps[((size_t)ul * P.seg_len + (t - P.snap_t0)) * g.inner + q];
+                    P.born_snaps[((size_t)(t - P.snap_t0) * P.units + ul) * g.inner + q];
         }
         // point source after the update (lib.rs:209-215); Born: its derivative
 #pragma unroll
@@ -317,7 +317,7 @@ __global__ __launch_bounds__(NT) void k_adj_step(StepParams P) {
         const float lam = Lcur[i];
         float next = w + 2.0f * lam - Lnxt[i];
         const size_t qq = (size_t)(z - 2) * g.ni + (x - 2);
-        float gk = lam * P.snaps[((size_t)ul * P.seg_len + (t - P.snap_t0)) * g.inner + qq];
+        float gk = lam * P.snaps[((size_t)(t - P.snap_t0) * P.units + ul) * g.inner + qq];
         if (z >= P.geo.band_z0 && z < P.geo.band_z0 + P.geo.band_rows) {
             const int o = (z - P.geo.band_z0) * g.nx + x;
             for (int e = P.geo.band_off[o]; e < P.geo.band_off[o + 1]; ++e) next += P.geo.ent_w[e] * res[P.geo.ent_r[e]];

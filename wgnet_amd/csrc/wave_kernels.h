@@ -94,7 +94,7 @@ struct StepParams {
     const float* wavelet;    // [units or 1, nt]
     long long wav_stride;
     float* traces;           // forward: [all units, nt, nrec] or null (no recording)
-    float* snaps;            // [units, seg_len, inner]: written (forward STORE) or read (adjoint)
+    float* snaps;            // [seg_len, units, inner]: written (forward STORE) or read (adjoint)
     int snap_t0, seg_len;    // snapshot of step t is entry t - snap_t0
     float* state;
     int units, unit0, t;
