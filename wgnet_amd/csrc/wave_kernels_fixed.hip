@@ -197,23 +197,26 @@ __device__ __forceinline__ void f_compute(FSt<R::K, R::BX, R::BZ, STORE, BORN>& 
     const int i0 = m24(z0, G::NX) + x;
     const int q0 = m24(zi0, G::NI) + xi;
     constexpr int SW2 = 2 * G::SWF;
-    // a*psi strip entries of this thread's first slot
-    const float* AX0 = C.AX + (m24(zi0, SW2) + (x < G::REG ? x : x - G::NX + SW2));
-    const float* AZ0 = C.AZ + (m24(R::ZOFF == 0 ? z0 : z0 - G::NZ + SW2, G::NI) + xi);
+    // Base pointers at the lowest address each slot reads (LDS offsets are
+    // unsigned immediates): field at i0 - 2 NX - 2, strips at -2 / -2 NI.
+    const float* Ub = C.U + (i0 - 2 * G::NX - 2);
+    const float* AXb = C.AX + (m24(zi0, SW2) + (x < G::REG ? x : x - G::NX + SW2) - 2);
+    const float* AZb = C.AZ + (m24(R::ZOFF == 0 ? z0 : z0 - G::NZ + SW2, G::NI) + xi - 2 * G::NI);
     float dummy = 0.0f;
 #pragma unroll
     for (int k = 0; k < R::K; ++k) {
         if (k == R::K - 1 && !pl.last_ok) break;
         const int i = i0 + k * R::RP * G::NX;
-        const float u = C.U[i];
+        const int li = k * R::RP * G::NX + 2 * G::NX + 2;  // slot k relative to Ub
+        const float u = Ub[li];
         Pml pmz{0, 0, 0, 0};
         if constexpr (R::BZ) pmz = pml_z<G>(PR, z0 + k * R::RP);
         float& px = R::BX ? S.px[R::BX ? k : 0] : dummy;
         float& zx = R::BX ? S.zx[R::BX ? k : 0] : dummy;
         float& pz = R::BZ ? S.pz[R::BZ ? k : 0] : dummy;
         float& zz = R::BZ ? S.zz[R::BZ ? k : 0] : dummy;
-        const float w = fwd_w<R::BX, R::BZ>(C.U, i, G::NX, u, C.rh, C.rh2, AX0 + k * R::RP * SW2, pmx, px, zx,
-                                            AZ0 + k * R::RP * G::NI, G::NI, pmz, pz, zz);
+        const float w = fwd_w<R::BX, R::BZ>(Ub, li, G::NX, u, C.rh, C.rh2, AXb + (k * R::RP * SW2 + 2), pmx, px, zx,
+                                            AZb + (k * R::RP * G::NI + 2 * G::NI), G::NI, pmz, pz, zz);
         const int q = q0 + k * R::RP * G::NI;
         if constexpr (STORE) snap_t[q] = S.c2[k] * w;  // 2 v dt^2 w (splatForward.wgsl:108-110)
         float next = S.c1[k] * w + 2.0f * u - S.uprev[k];
@@ -497,30 +500,38 @@ __device__ __forceinline__ void a_phase2(ASt<R::K, R::BX, R::BZ>& S, const Place
     const int z0 = zi0 + 2;
     const int i0 = m24(z0, G::NX) + x;
     const int q0 = m24(zi0, G::NI) + xi;
-    const float* TX0 = C.TX + (m24(zi0, SW2) + (x < G::REGA ? x : x - G::NX + SW2));
-    const float* UX0 = C.UX + (m24(zi0, SW2) + (x < G::REGA ? x : x - G::NX + SW2));
-    const int tz0 = m24(R::ZOFF == 0 ? z0 : z0 - G::NZ + SW2, G::NI) + xi;
+    // Base pointers at the lowest address each slot reads (LDS offsets are
+    // unsigned immediates): V at i0 - 2 NX - 2, x strips at -2, z strips at -2 NI.
+    const float* Vb = C.V + (i0 - 2 * G::NX - 2);
+    const int txb = m24(zi0, SW2) + (x < G::REGA ? x : x - G::NX + SW2) - 2;
+    const float* TXb = C.TX + txb;
+    const float* UXb = C.UX + txb;
+    const int tzb = m24(R::ZOFF == 0 ? z0 : z0 - G::NZ + SW2, G::NI) + xi - 2 * G::NI;
+    const float* TZb = C.TZ + tzb;
+    const float* UZb = C.UZ + tzb;
     const float rh = C.rh;
 #pragma unroll
     for (int k = 0; k < R::K; ++k) {
         if (k == R::K - 1 && !pl.last_ok) break;
         const int i = i0 + k * R::RP * G::NX;
+        const int li = k * R::RP * G::NX + 2 * G::NX + 2;  // slot k relative to Vb
         const int z = z0 + k * R::RP;
         const float Sn = snap_t[q0 + k * R::RP * G::NI];
-        const float v0 = C.V[i];
+        const float v0 = Vb[li];
         float w;  // x term, then += z term (0 + x skipped: equal up to the sign of zero)
         if constexpr (R::BX) {
-            w = adj_axis_pml(TX0 + k * R::RP * SW2, UX0 + k * R::RP * SW2, 1, cx.opb, cx.a, v0, rh, S.Px[k], S.Zx[k]);
+            w = adj_axis_pml(TXb + (k * R::RP * SW2 + 2), UXb + (k * R::RP * SW2 + 2), 1, cx.opb, cx.a, v0, rh,
+                             S.Px[k], S.Zx[k]);
         } else {
-            w = adj_axis_int(C.V, i, 1, v0, rh);
+            w = adj_axis_int(Vb, li, 1, v0, rh);
         }
         if constexpr (R::BZ) {
             const float opb[5] = {PR[Prof<G>::OPBZ + z - 2], PR[Prof<G>::OPBZ + z - 1], PR[Prof<G>::OPBZ + z],
                                   PR[Prof<G>::OPBZ + z + 1], PR[Prof<G>::OPBZ + z + 2]};
-            const int tz = tz0 + k * R::RP * G::NI;
-            w += adj_axis_pml(C.TZ + tz, C.UZ + tz, G::NI, opb, PR[Prof<G>::AZ + z], v0, rh, S.Pz[k], S.Zz[k]);
+            const int tz = k * R::RP * G::NI + 2 * G::NI;
+            w += adj_axis_pml(TZb + tz, UZb + tz, G::NI, opb, PR[Prof<G>::AZ + z], v0, rh, S.Pz[k], S.Zz[k]);
         } else {
-            w += adj_axis_int(C.V, i, G::NX, v0, rh);
+            w += adj_axis_int(Vb, li, G::NX, v0, rh);
         }
         float next = w + 2.0f * S.lam[k] - S.lamp[k];
         float gk = S.lam[k] * Sn;  // store_every = 1 (wgsl:234-236)
