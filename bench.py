@@ -96,9 +96,11 @@ def main() -> int:
     ap.add_argument("--budget", type=int, default=160)
     ap.add_argument("--epochs", type=int, default=100)
     ap.add_argument("--save-params", default="")
-    ap.add_argument("--pc-mode", choices=("first", "replace"), default="first")
+    ap.add_argument("--pc-mode", choices=("jump", "first", "replace"), default="jump")
     ap.add_argument("--lr", type=float, default=3e-3)
-    ap.add_argument("--compress", action="store_true")
+    ap.add_argument("--no-compress", action="store_true")
+    ap.add_argument("--stages", type=int, choices=(1, 2), default=2)
+    ap.add_argument("--jumps", type=int, default=1)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only; this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -237,6 +239,19 @@ def refine_scenes(n: int, seed: int):
     return start / 1000.0, true / 1000.0, g
 
 
+def refine_lattice_field():
+    """[70, 70] km/s field that refine_scenes' initial active Gaussians render
+    (the same for every scene): the scene's initial field is the starting
+    model plus this."""
+    import torch
+
+    from wgnet_amd.render import ACTIVE_AMP_KM, render_residual
+
+    lat = refine_scenes(1, seed=0)[2][0]
+    act = np.flatnonzero(np.abs(lat[:, 5]) >= ACTIVE_AMP_KM)
+    return render_residual(torch.from_numpy(lat[act])[None], torch.from_numpy(act))[0]
+
+
 def bench_refine(args) -> int:
     """Learned-preconditioned refinement against plain projected L-BFGS:
     solves needed to reach plain L-BFGS's final data misfit."""
@@ -250,21 +265,33 @@ def bench_refine(args) -> int:
     bands = (6.0, 9.0, 12.0, 15.0)
     ph = WavePhysics(shots=S, nt=nt, dt=dt)
     ph.engine.set_sources(SURFACE_SRC)
-    t0 = time.perf_counter()
-    grads, fields, target = make_pairs(ph, args.train_pairs, seed=4, bands_hz=bands)
-    torch.cuda.synchronize()
-    t_pairs = time.perf_counter() - t0
-    baseline = float((target ** 2).mean())
-    print(f"refine: {grads.shape[0]} training pairs in {t_pairs:.1f} s (zero-prediction loss {baseline:.4f})",
-          file=sys.stderr, flush=True)
+    # stage 1: pairs at the scenes' initial fields (starting model + the
+    # rendered initial Gaussians); stage 2 (--stages 2): pairs at the points
+    # the stage-1 network's step reaches from the same models, trained
+    # together with stage 1's; --train-pairs counts both
+    n1 = args.train_pairs // args.stages
+    offset = refine_lattice_field()
     net = Preconditioner(max_batch=256, seed=1)
-    net.compress = args.compress
-    t0 = time.perf_counter()
-    hist = train(net, grads, fields, target, epochs=args.epochs, lr=args.lr)
-    torch.cuda.synchronize()
-    t_train = time.perf_counter() - t0
-    print(f"refine: trained {args.epochs} epochs in {t_train:.1f} s, loss {hist[0]:.4f} -> {hist[-1]:.4f}",
-          file=sys.stderr, flush=True)
+    net.compress = not args.no_compress
+    t_pairs = t_train = 0.0
+    pairs = []
+    for stage in range(args.stages):
+        t0 = time.perf_counter()
+        pairs.append(make_pairs(ph, n1, seed=4, bands_hz=bands, net=net if stage else None, offset_km=offset))
+        torch.cuda.synchronize()
+        t_pairs += time.perf_counter() - t0
+        grads, fields, target = (torch.cat([p[k] for p in pairs]) for k in range(3))
+        if stage == 0:
+            baseline = float((target ** 2).mean())
+        print(f"refine: stage {stage + 1}: {grads.shape[0]} training pairs (zero-prediction loss "
+              f"{float((target ** 2).mean()):.4f})", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        hist = train(net, grads, fields, target, epochs=args.epochs, lr=args.lr, fit_scale=stage == 0)
+        torch.cuda.synchronize()
+        t_train += time.perf_counter() - t0
+        print(f"refine: stage {stage + 1}: trained {args.epochs} epochs, loss {hist[0]:.4f} -> {hist[-1]:.4f}",
+              file=sys.stderr, flush=True)
+    del pairs
     if args.save_params:
         net.save(args.save_params)
     del grads, fields, target
@@ -284,10 +311,12 @@ def bench_refine(args) -> int:
               f"{np.median(plain_final[f]):.4e}", file=sys.stderr, flush=True)
     m_plain = plain_final[1.0]
     pc_final = {}
-    hook = net.hook()
+    if args.pc_mode == "jump":
+        learned = {"update": net.update_hook(), "jumps_per_band": args.jumps}
+    else:
+        learned = {"precondition": net.hook(), "pc_mode": args.pc_mode}
     for f in fracs:
-        res = refine_batch(start_km, gauss, obs, ph, budget_solves=int(B * f), bands_hz=bands,
-                           precondition=hook, pc_mode=args.pc_mode)
+        res = refine_batch(start_km, gauss, obs, ph, budget_solves=int(B * f), bands_hz=bands, **learned)
         pc_final[f] = np.array([r.final_loss for r in res])
         print(f"refine: learned preconditioner {int(B * f)} solves: median final misfit "
               f"{np.median(pc_final[f]):.4e}, reaches plain in {np.mean(pc_final[f] <= m_plain):.3f} of models",
@@ -324,7 +353,9 @@ def bench_refine(args) -> int:
         "config": {"workload": f"{args.eval_models} held-out models, 5 shots, 70x70, nt={nt}, bands {bands} Hz, "
                                f"plain budget {B} solves; network trained on {args.train_pairs} solver pairs",
                    "budget": B, "eval_models": args.eval_models, "train_pairs": args.train_pairs,
-                   "epochs": args.epochs, "pc_mode": args.pc_mode},
+                   "epochs": args.epochs, "stages": args.stages, "jumps_per_band": args.jumps,
+                   "pc_mode": args.pc_mode, "compress": net.compress,
+                   "layers": net.layers()},
         "reached_at": {str(f): float(np.mean(need <= f)) for f in fracs},
         "median_final_misfit": {"initial": float(np.median(init)),
                                 **{f"plain_{f}": float(np.median(plain_final[f])) for f in fracs},

@@ -233,8 +233,9 @@ void wg_cpml_profiles(int32_t n, float h, float dt, int32_t width,
  * refines with plain projected L-BFGS, python/wgnet/refine.py:92-169; this
  * network is applied at its gradient hook, refine.py:105-106).
  *
- * A 3x3 convolutional network 2 -> 32 -> 32 -> 32 -> 1 channels with GELU
- * after the first three layers and zero padding, mapping
+ * A network of dilated 3x3 convolutions, 2 -> 32 -> 32 -> 32 -> 1 channels
+ * with dilations 4, 16, 8, 1, GELU after the first three layers and zero
+ * padding equal to the dilation (wg_pc_layer_shape reports each layer), mapping
  * x [N, 2, H, W] (gradient, normalised velocity) to y [N, H, W].  bfloat16
  * weights and activations, float32 MFMA accumulation, float32 master weights
  * and AdamW moments.  Parameters in PyTorch Conv2d order (per layer
@@ -250,6 +251,10 @@ int32_t wg_pc_create(int32_t max_batch, int32_t height, int32_t width,
                      uint64_t seed, wg_pc **out);
 void wg_pc_destroy(wg_pc *pc);
 int32_t wg_pc_num_params(void);
+int32_t wg_pc_num_layers(void);
+/* input and output channels and dilation of layer 0 .. num_layers-1 */
+int32_t wg_pc_layer_shape(int32_t layer, int32_t *cin, int32_t *cout,
+                          int32_t *dilation);
 int32_t wg_pc_get_params(wg_pc *pc, float *host, int32_t n);
 /* also resets the AdamW moments and step count */
 int32_t wg_pc_set_params(wg_pc *pc, const float *host, int32_t n);

@@ -1,9 +1,10 @@
 """CPU reference of the learned preconditioner network (test infrastructure:
 only tests/ may import it).
 
-The network has no counterpart in the reference repository; it is the 3x3
-convolutional network 2 -> 32 -> 32 -> 32 -> 1 (GELU after the first three
-layers, zero padding 1) of the HIP kernels.  This module restates its forward
+The network has no counterpart in the reference repository; it is the network
+of dilated 3x3 convolutions 2 -> 32 -> 32 -> 32 -> 1 (dilations 4, 16, 8, 1,
+GELU after the first three layers, zero padding equal to the dilation) of the
+HIP kernels.  This module restates its forward
 pass, mean-squared-error loss and parameter gradient in float64 with torch's
 CPU convolution routines (torch.nn.functional.conv2d and torch.nn.grad.*),
 rounding to bfloat16 at the same points as the HIP kernels (network input,
@@ -21,6 +22,7 @@ import torch
 import torch.nn.functional as F
 
 LAYERS = ((2, 32), (32, 32), (32, 32), (32, 1))
+DILATIONS = (4, 16, 8, 1)
 
 
 def bf16_round(x: torch.Tensor) -> torch.Tensor:
@@ -58,7 +60,8 @@ def forward(params: np.ndarray, x: np.ndarray, bf16: bool = True):
     for li, (w, b) in enumerate(layers):
         wq = r(w)
         ws.append(wq)
-        pre = F.conv2d(a, wq, b, padding=1)
+        d = DILATIONS[li]
+        pre = F.conv2d(a, wq, b, padding=d, dilation=d)
         pres.append(pre)
         if li < len(layers) - 1:
             a = r(_gelu(pre))
@@ -78,11 +81,12 @@ def loss_and_grad(params: np.ndarray, x: np.ndarray, target: np.ndarray, bf16: b
     grads = []
     for li in range(len(LAYERS) - 1, -1, -1):
         a_in = acts[li]
-        gw = torch.nn.grad.conv2d_weight(a_in, ws[li].shape, g, padding=1)
+        d = DILATIONS[li]
+        gw = torch.nn.grad.conv2d_weight(a_in, ws[li].shape, g, padding=d, dilation=d)
         gb = g.sum((0, 2, 3))
         grads.append((gw, gb))
         if li > 0:
-            gin = torch.nn.grad.conv2d_input(a_in.shape, ws[li], g, padding=1)
+            gin = torch.nn.grad.conv2d_input(a_in.shape, ws[li], g, padding=d, dilation=d)
             g = r(gin * _gelu_d(pres[li - 1]))
     flat = []
     for gw, gb in reversed(grads):

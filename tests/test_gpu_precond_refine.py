@@ -1,8 +1,10 @@
 """The trained preconditioner inside refine: a small network trained on pairs
-from the batched solver, then plain and preconditioned projected L-BFGS on
-held-out models at equal budgets.  The test checks the invariants (budgets,
-descent, fallback accounting) and prints what the network gains or loses
-against plain L-BFGS; the full-size measurement is bench.py --workload refine."""
+from the batched solver (at the scenes' initial fields), then plain and
+preconditioned projected L-BFGS on held-out models at equal budgets.  The
+test checks the invariants (budgets, descent within a band, fallback
+accounting) and that the learned update steps lower the median misfit below
+plain L-BFGS's at the same budget; the full-size measurement is
+bench.py --workload refine."""
 
 from __future__ import annotations
 
@@ -22,7 +24,8 @@ def test_preconditioned_refine_against_plain():
     bands = (6.0, 9.0, 12.0, 15.0)
     ph = WavePhysics(shots=5, nt=1000, dt=0.001)
     ph.engine.set_sources(bench.SURFACE_SRC)
-    grads, fields, target = make_pairs(ph, 2048, seed=21, bands_hz=bands)
+    offset = bench.refine_lattice_field()
+    grads, fields, target = make_pairs(ph, 2048, seed=21, bands_hz=bands, offset_km=offset)
     net = Preconditioner(max_batch=256, seed=2)
     hist = train(net, grads, fields, target, epochs=30, lr=3e-3)
     zero = float((target ** 2).mean())
@@ -33,13 +36,20 @@ def test_preconditioned_refine_against_plain():
     obs = ph.simulate_torch(torch.from_numpy(true).cuda())
     budget = 40
     plain = refine_batch(start, gauss, obs, ph, budget_solves=budget, bands_hz=bands)
-    pc = refine_batch(start, gauss, obs, ph, budget_solves=budget, bands_hz=bands, precondition=net.hook())
+    pc = refine_batch(start, gauss, obs, ph, budget_solves=budget, bands_hz=bands, update=net.update_hook())
+    pd = refine_batch(start, gauss, obs, ph, budget_solves=budget, bands_hz=bands, precondition=net.hook())
     mp = np.array([r.final_loss for r in plain])
     mq = np.array([r.final_loss for r in pc])
     init = np.array([r.initial_loss for r in plain])
     print(f"median misfit: initial {np.median(init):.4e}, plain {np.median(mp):.4e}, preconditioned "
           f"{np.median(mq):.4e}; preconditioned lower in {np.mean(mq < mp):.2f} of models")
-    for r in plain + pc:
+    for r in plain + pc + pd:
         assert r.solves <= budget + 2
+        # accepted steps never raise the misfit within a band
+        for a, b in zip(r.history, r.history[1:]):
+            if a["band_hz"] == b["band_hz"] and b["accepted"]:
+                assert b["loss"] <= a["loss"]
     assert np.median(mq) < np.median(init)
-    assert any(h["preconditioned"] for r in pc for h in r.history)
+    assert np.median(mq) < np.median(mp)  # the learned first steps pay for themselves
+    assert any(h["jump"] for r in pc for h in r.history)
+    assert any(h["preconditioned"] for r in pd for h in r.history)

@@ -83,6 +83,8 @@ SIGNATURES = {
     "wg_pc_create": (_i32, [_i32, _i32, _i32, ctypes.c_uint64, ctypes.POINTER(_vp)]),
     "wg_pc_destroy": (None, [_vp]),
     "wg_pc_num_params": (_i32, []),
+    "wg_pc_num_layers": (_i32, []),
+    "wg_pc_layer_shape": (_i32, [_i32, _vp, _vp, _vp]),
     "wg_pc_get_params": (_i32, [_vp, _vp, _i32]),
     "wg_pc_set_params": (_i32, [_vp, _vp, _i32]),
     "wg_pc_forward_dev": (_i32, [_vp, _i32, _vp, _vp, _vp]),

@@ -1,6 +1,8 @@
-// Learned gradient preconditioner: a 4-layer 3x3 convolutional network
-// (2 -> 32 -> 32 -> 32 -> 1 channels, GELU after the first three layers,
-// zero padding), inference and training on MFMA.
+// Learned gradient preconditioner: a 4-layer network of dilated 3x3
+// convolutions (2 -> 32 -> 32 -> 32 -> 1 channels, dilations 4, 16, 8, 1,
+// GELU after the first three layers, zero padding equal to the dilation, so
+// each layer keeps the image size), inference and training on MFMA.  The
+// dilations give a 59 x 59 receptive field on the 70 x 70 models.
 //
 // No reference counterpart: the reference refines with plain projected L-BFGS
 // (python/wgnet/refine.py:92-169); the network maps (velocity-space gradient,
@@ -13,7 +15,8 @@
 // float32.
 //
 // Layouts: activations NHWC, [P = N*H*W pixels][C]; layer weights
-// [COUTP][KP] with k = (ky*3 + kx)*CIN + ci, COUTP and KP rounded up to 16
+// [COUTP][KP] with k = (ky*3 + kx)*CIN + ci (tap (ky, kx) reads the pixel
+// offset by ((ky - 1) D, (kx - 1) D) for dilation D), COUTP and KP rounded up to 16
 // (padding stays zero); their transposes [CINP][K2P] with k2 = tap*COUT + co
 // for the data gradient.  Convolutions are implicit GEMMs: M = pixels,
 // N = output channels, K = 9*CIN (forward); M = pixels, N = CIN, K = 9*COUT
@@ -60,12 +63,12 @@ __device__ inline float gelu_d(float x) {
 __host__ __device__ constexpr int rup16(int v) { return (v + 15) / 16 * 16; }
 
 // --------------------------------------------------------------------------
-// Forward: pre[p][co] = bias[co] + sum_k W[co][k] * in[nbr(p, tap(k))][ci(k)];
+// Forward: pre[p][co] = bias[co] + sum_k W[co][k] * in[nbr(p, tap(k), D)][ci(k)];
 // act = bf16(gelu(pre)) (or bf16(pre) without GELU).  One wave computes 16
 // pixels x COUTP channels.
 // --------------------------------------------------------------------------
 template <int CIN, int COUT, bool GELU>
-__global__ __launch_bounds__(NT) void k_fwd(const uint16_t* __restrict__ in, int P, int H, int W,
+__global__ __launch_bounds__(NT) void k_fwd(const uint16_t* __restrict__ in, int P, int H, int W, int D,
                                             const uint16_t* __restrict__ wt, const float* __restrict__ bias,
                                             float* __restrict__ pre, uint16_t* __restrict__ act) {
     constexpr int K = 9 * CIN, KP = rup16(K), COUTP = rup16(COUT), NTL = COUTP / 16;
@@ -88,7 +91,7 @@ __global__ __launch_bounds__(NT) void k_fwd(const uint16_t* __restrict__ in, int
             uint16_t v = 0;
             if (pv && k < K) {
                 const int tap = k / CIN, ci = k - tap * CIN;
-                const int yy = y + tap / 3 - 1, xx = x + tap % 3 - 1;
+                const int yy = y + (tap / 3 - 1) * D, xx = x + (tap % 3 - 1) * D;
                 if (yy >= 0 && yy < H && xx >= 0 && xx < W) v = in[((size_t)img * HW + yy * W + xx) * CIN + ci];
             }
             a[j] = (short)v;
@@ -119,11 +122,11 @@ __global__ __launch_bounds__(NT) void k_fwd(const uint16_t* __restrict__ in, int
 }
 
 // --------------------------------------------------------------------------
-// Data gradient: d_in[p][ci] = sum_{tap, co} dpre[p - off(tap)][co] * W[co][tap*CIN + ci];
+// Data gradient: d_in[p][ci] = sum_{tap, co} dpre[p - D off(tap)][co] * W[co][tap*CIN + ci];
 // dpre_prev = bf16(d_in * gelu'(pre_prev)).  wtT is [CINP][K2P], k2 = tap*COUT + co.
 // --------------------------------------------------------------------------
 template <int CIN, int COUT>
-__global__ __launch_bounds__(NT) void k_bwd_data(const uint16_t* __restrict__ dpre, int P, int H, int W,
+__global__ __launch_bounds__(NT) void k_bwd_data(const uint16_t* __restrict__ dpre, int P, int H, int W, int D,
                                                  const uint16_t* __restrict__ wtT, const float* __restrict__ pre_prev,
                                                  uint16_t* __restrict__ dpre_prev) {
     constexpr int K2 = 9 * COUT, K2P = rup16(K2), CINP = rup16(CIN), NTL = CINP / 16;
@@ -146,7 +149,7 @@ __global__ __launch_bounds__(NT) void k_bwd_data(const uint16_t* __restrict__ dp
             uint16_t v = 0;
             if (pv && k < K2) {
                 const int tap = k / COUT, co = k - tap * COUT;
-                const int yy = y - (tap / 3 - 1), xx = x - (tap % 3 - 1);
+                const int yy = y - (tap / 3 - 1) * D, xx = x - (tap % 3 - 1) * D;
                 if (yy >= 0 && yy < H && xx >= 0 && xx < W) v = dpre[((size_t)img * HW + yy * W + xx) * COUT + co];
             }
             a[j] = (short)v;
@@ -176,12 +179,12 @@ __global__ __launch_bounds__(NT) void k_bwd_data(const uint16_t* __restrict__ dp
 
 // --------------------------------------------------------------------------
 // Weight and bias gradient partials over a chunk of `chunk` pixels per block:
-// part[blk][co*KP + k] = sum_p dpre[p][co] * in[nbr(p, tap(k))][ci(k)],
+// part[blk][co*KP + k] = sum_p dpre[p][co] * in[nbr(p, tap(k), D)][ci(k)],
 // part[blk][COUTP*KP + co] = sum_p dpre[p][co].
 // --------------------------------------------------------------------------
 template <int CIN, int COUT>
 __global__ __launch_bounds__(NT) void k_bwd_w(const uint16_t* __restrict__ dpre, const uint16_t* __restrict__ in,
-                                              int P, int H, int W, int chunk, float* __restrict__ part) {
+                                              int P, int H, int W, int D, int chunk, float* __restrict__ part) {
     constexpr int K = 9 * CIN, KP = rup16(K), COUTP = rup16(COUT), MT = COUTP / 16, NTL = KP / 16;
     constexpr int NPW = (NTL + 3) / 4;  // column tiles per wave
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -230,7 +233,7 @@ __global__ __launch_bounds__(NT) void k_bwd_w(const uint16_t* __restrict__ dpre,
                 for (int j = 0; j < 4; ++j) {
                     uint16_t v = 0;
                     if (ok[j] && k < K) {
-                        const int yy = py[j] + tap / 3 - 1, xx = px[j] + tap % 3 - 1;
+                        const int yy = py[j] + (tap / 3 - 1) * D, xx = px[j] + (tap % 3 - 1) * D;
                         if (yy >= 0 && yy < H && xx >= 0 && xx < W)
                             v = in[((size_t)pimg[j] * HW + yy * W + xx) * CIN + ci];
                     }
@@ -365,7 +368,7 @@ int pc_fail(int code, const std::string& m) {
     } while (0)
 
 struct Layer {
-    int cin, cout;
+    int cin, cout, dil;
     int KP() const { return rup16(9 * cin); }
     int COUTP() const { return rup16(cout); }
     int CINP() const { return rup16(cin); }
@@ -375,7 +378,7 @@ struct Layer {
 };
 
 constexpr int kLayers = 4;
-const Layer kNet[kLayers] = {{2, 32}, {32, 32}, {32, 32}, {32, 1}};
+const Layer kNet[kLayers] = {{2, 32, 4}, {32, 32, 16}, {32, 32, 8}, {32, 1, 1}};
 constexpr int kChunk = 4096;  // pixels per weight-gradient block
 
 }  // namespace
@@ -437,14 +440,14 @@ int forward(wg_pc* pc, int N, const float* x, hipStream_t st) {
     const int HW = pc->H * pc->W, P = N * HW;
     const dim3 g4((P + 63) / 64);
     hipLaunchKernelGGL(k_prep, dim3((P + 255) / 256), dim3(256), 0, st, x, N, HW, pc->x16);
-    hipLaunchKernelGGL((k_fwd<2, 32, true>), g4, dim3(NT), 0, st, pc->x16, P, pc->H, pc->W, pc->wt[0],
-                       pc->th + pc->off[0] + kNet[0].COUTP() * kNet[0].KP(), pc->pre[0], pc->act[0]);
-    hipLaunchKernelGGL((k_fwd<32, 32, true>), g4, dim3(NT), 0, st, pc->act[0], P, pc->H, pc->W, pc->wt[1],
-                       pc->th + pc->off[1] + kNet[1].COUTP() * kNet[1].KP(), pc->pre[1], pc->act[1]);
-    hipLaunchKernelGGL((k_fwd<32, 32, true>), g4, dim3(NT), 0, st, pc->act[1], P, pc->H, pc->W, pc->wt[2],
-                       pc->th + pc->off[2] + kNet[2].COUTP() * kNet[2].KP(), pc->pre[2], pc->act[2]);
-    hipLaunchKernelGGL((k_fwd<32, 1, false>), g4, dim3(NT), 0, st, pc->act[2], P, pc->H, pc->W, pc->wt[3],
-                       pc->th + pc->off[3] + kNet[3].COUTP() * kNet[3].KP(), pc->out, (uint16_t*)nullptr);
+    hipLaunchKernelGGL((k_fwd<2, 32, true>), g4, dim3(NT), 0, st, pc->x16, P, pc->H, pc->W, kNet[0].dil,
+                       pc->wt[0], pc->th + pc->off[0] + kNet[0].COUTP() * kNet[0].KP(), pc->pre[0], pc->act[0]);
+    hipLaunchKernelGGL((k_fwd<32, 32, true>), g4, dim3(NT), 0, st, pc->act[0], P, pc->H, pc->W, kNet[1].dil,
+                       pc->wt[1], pc->th + pc->off[1] + kNet[1].COUTP() * kNet[1].KP(), pc->pre[1], pc->act[1]);
+    hipLaunchKernelGGL((k_fwd<32, 32, true>), g4, dim3(NT), 0, st, pc->act[1], P, pc->H, pc->W, kNet[2].dil,
+                       pc->wt[2], pc->th + pc->off[2] + kNet[2].COUTP() * kNet[2].KP(), pc->pre[2], pc->act[2]);
+    hipLaunchKernelGGL((k_fwd<32, 1, false>), g4, dim3(NT), 0, st, pc->act[2], P, pc->H, pc->W, kNet[3].dil,
+                       pc->wt[3], pc->th + pc->off[3] + kNet[3].COUTP() * kNet[3].KP(), pc->out, (uint16_t*)nullptr);
     PCK(hipGetLastError());
     return WG_OK;
 }
@@ -459,28 +462,28 @@ int backward(wg_pc* pc, int N, const float* target, hipStream_t st) {
     // layer 4 (32 -> 1)
     float* p4 = pc->part;
     hipLaunchKernelGGL((k_bwd_w<32, 1>), dim3(nchunk), dim3(NT), 0, st, pc->dout, pc->act[2], P, pc->H, pc->W,
-                       kChunk, p4);
+                       kNet[3].dil, kChunk, p4);
     hipLaunchKernelGGL((k_reduce), dim3((kNet[3].flat() + 255) / 256), dim3(256), 0, st, p4, nchunk, kNet[3].flat(),
                        pc->grad + pc->off[3]);
-    hipLaunchKernelGGL((k_bwd_data<32, 1>), g4, dim3(NT), 0, st, pc->dout, P, pc->H, pc->W, pc->wtT[3], pc->pre[2],
-                       pc->dpre[2]);
+    hipLaunchKernelGGL((k_bwd_data<32, 1>), g4, dim3(NT), 0, st, pc->dout, P, pc->H, pc->W, kNet[3].dil,
+                       pc->wtT[3], pc->pre[2], pc->dpre[2]);
     // layer 3 (32 -> 32)
     hipLaunchKernelGGL((k_bwd_w<32, 32>), dim3(nchunk), dim3(NT), 0, st, pc->dpre[2], pc->act[1], P, pc->H, pc->W,
-                       kChunk, pc->part);
+                       kNet[2].dil, kChunk, pc->part);
     hipLaunchKernelGGL((k_reduce), dim3((kNet[2].flat() + 255) / 256), dim3(256), 0, st, pc->part, nchunk,
                        kNet[2].flat(), pc->grad + pc->off[2]);
-    hipLaunchKernelGGL((k_bwd_data<32, 32>), g4, dim3(NT), 0, st, pc->dpre[2], P, pc->H, pc->W, pc->wtT[2],
-                       pc->pre[1], pc->dpre[1]);
+    hipLaunchKernelGGL((k_bwd_data<32, 32>), g4, dim3(NT), 0, st, pc->dpre[2], P, pc->H, pc->W, kNet[2].dil,
+                       pc->wtT[2], pc->pre[1], pc->dpre[1]);
     // layer 2
     hipLaunchKernelGGL((k_bwd_w<32, 32>), dim3(nchunk), dim3(NT), 0, st, pc->dpre[1], pc->act[0], P, pc->H, pc->W,
-                       kChunk, pc->part);
+                       kNet[1].dil, kChunk, pc->part);
     hipLaunchKernelGGL((k_reduce), dim3((kNet[1].flat() + 255) / 256), dim3(256), 0, st, pc->part, nchunk,
                        kNet[1].flat(), pc->grad + pc->off[1]);
-    hipLaunchKernelGGL((k_bwd_data<32, 32>), g4, dim3(NT), 0, st, pc->dpre[1], P, pc->H, pc->W, pc->wtT[1],
-                       pc->pre[0], pc->dpre[0]);
+    hipLaunchKernelGGL((k_bwd_data<32, 32>), g4, dim3(NT), 0, st, pc->dpre[1], P, pc->H, pc->W, kNet[1].dil,
+                       pc->wtT[1], pc->pre[0], pc->dpre[0]);
     // layer 1 (2 -> 32)
     hipLaunchKernelGGL((k_bwd_w<2, 32>), dim3(nchunk), dim3(NT), 0, st, pc->dpre[0], pc->x16, P, pc->H, pc->W,
-                       kChunk, pc->part);
+                       kNet[0].dil, kChunk, pc->part);
     hipLaunchKernelGGL((k_reduce), dim3((kNet[0].flat() + 255) / 256), dim3(256), 0, st, pc->part, nchunk,
                        kNet[0].flat(), pc->grad + pc->off[0]);
     PCK(hipGetLastError());
@@ -580,6 +583,16 @@ void wg_pc_destroy(wg_pc* pc) {
         if (pc->dpre[l]) (void)hipFree(pc->dpre[l]);
     }
     delete pc;
+}
+
+int32_t wg_pc_num_layers(void) { return kLayers; }
+
+int32_t wg_pc_layer_shape(int32_t layer, int32_t* cin, int32_t* cout, int32_t* dilation) {
+    if (layer < 0 || layer >= kLayers || !cin || !cout || !dilation) return pc_fail(WG_E_INVALID, "invalid argument");
+    *cin = kNet[layer].cin;
+    *cout = kNet[layer].cout;
+    *dilation = kNet[layer].dil;
+    return WG_OK;
 }
 
 int32_t wg_pc_num_params(void) {

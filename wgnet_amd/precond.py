@@ -6,9 +6,14 @@ No reference counterpart: the reference refines with plain projected L-BFGS
 (python/wgnet/refine.py:92-169).  The network sees (velocity-space gradient,
 normalised velocity) and predicts the model update (true - current), the pairs
 coming from the batched wave solver itself.  In refine() it is applied at the
-gradient hook of refine.py:105-106, where it returns minus the predicted
-update rescaled to the gradient's norm (refine_batch's `pc_mode` decides how
-L-BFGS uses it; plain gradients remain the fallback).
+gradient hook of refine.py:105-106 in one of two ways:
+  update_hook()  the predicted update itself (km/s), which refine_batch fits
+                 onto the scene parameters and tries as the first step of
+                 each band (`update=`);
+  hook()         minus the predicted update rescaled to the gradient's norm,
+                 a gradient-like direction (`precondition=`, `pc_mode`).
+Plain L-BFGS steps remain the fallback whenever a learned step fails its
+line search.
 """
 
 from __future__ import annotations
@@ -23,6 +28,7 @@ from . import _lib
 VEL_CENTRE_KM = 3.0
 VEL_SCALE_KM = 1.5
 TARGET_SCALE_KM = 0.3
+ASINH_GAIN = 3.0  # compressed inputs: asinh(3 g / median RMS)
 
 
 def _p(t: torch.Tensor) -> ctypes.c_void_p:
@@ -45,7 +51,8 @@ def features(grads: torch.Tensor, fields_km: torch.Tensor, grad_scale: float, co
 
 
 class Preconditioner:
-    """bf16 3x3 conv network 2 -> 32 -> 32 -> 32 -> 1 (GELU) on MFMA."""
+    """bf16 network of dilated 3x3 convolutions 2 -> 32 -> 32 -> 32 -> 1
+    (dilations 4, 16, 8, 1; GELU) on MFMA."""
 
     def __init__(self, max_batch: int = 256, height: int = 70, width: int = 70, seed: int = 0):
         self._lib = _lib.load()
@@ -55,7 +62,7 @@ class Preconditioner:
         self.max_batch, self.height, self.width = int(max_batch), int(height), int(width)
         self.num_params = int(self._lib.wg_pc_num_params())
         self.grad_scale = 1.0  # input scaling of the gradient channel (fit_scale)
-        self.compress = False  # asinh compression of the gradient channel
+        self.compress = True  # asinh compression of the gradient channel
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -74,10 +81,21 @@ class Preconditioner:
             raise ValueError(f"expected {self.num_params} parameters")
         _lib.check(self._lib.wg_pc_set_params(self._h, ctypes.c_void_p(p.ctypes.data), self.num_params), pc=True)
 
+    def layers(self) -> list[tuple[int, int, int]]:
+        """(input channels, output channels, dilation) of each layer."""
+        out = []
+        for l in range(int(self._lib.wg_pc_num_layers())):
+            v = [ctypes.c_int32() for _ in range(3)]
+            _lib.check(self._lib.wg_pc_layer_shape(l, *[ctypes.byref(c) for c in v]), pc=True)
+            out.append(tuple(c.value for c in v))
+        return out
+
     def fit_scale(self, grads: torch.Tensor) -> float:
-        """Sets grad_scale to 1 / (median per-model RMS of `grads`)."""
+        """Sets grad_scale to g / (median per-model RMS of `grads`), with gain
+        g = ASINH_GAIN for compressed inputs and 1 otherwise."""
         rms = grads.float().flatten(1).pow(2).mean(1).sqrt()
-        self.grad_scale = float(1.0 / max(float(rms.median()), 1e-30))
+        gain = ASINH_GAIN if self.compress else 1.0
+        self.grad_scale = float(gain / max(float(rms.median()), 1e-30))
         return self.grad_scale
 
     def save(self, path: str) -> None:
@@ -129,6 +147,14 @@ class Preconditioner:
         """Predicted model update (km/s) for velocity-space gradients."""
         return self.forward(features(grads, fields_km, self.grad_scale, self.compress)) * TARGET_SCALE_KM
 
+    def update_hook(self):
+        """refine_batch `update` callable: the predicted model update (km/s)."""
+
+        def update(grads: torch.Tensor, fields_km: torch.Tensor) -> torch.Tensor:
+            return self.direction(grads, fields_km)
+
+        return update
+
     def hook(self):
         """refine_batch `precondition` callable: minus the predicted update,
         rescaled to the norm of the gradient it replaces."""
@@ -157,12 +183,19 @@ def pseudo_hessian_hook(physics, fields_km: torch.Tensor, eps: float = 0.01):
     return precondition
 
 
-def make_pairs(physics, n: int, seed: int, bands_hz=(6.0, 9.0, 12.0, 15.0), batch: int = 64):
+def make_pairs(physics, n: int, seed: int, bands_hz=(6.0, 9.0, 12.0, 15.0), batch: int = 64,
+               net: "Preconditioner | None" = None, offset_km: torch.Tensor | None = None):
     """Training pairs from the batched solver: synthetic (start, true) models
     (the flat / curved / faulted families plus a Gaussian anomaly), the
-    velocity-space gradient at the start model in a passband drawn per model,
-    and the target (true - start) / 0.3 km/s.  Returns device tensors
-    grads [n, 70, 70], fields_km [n, 70, 70] and target [n, 70, 70]."""
+    velocity-space gradient at the current model in a passband drawn per
+    model, and the target (true - current) / 0.3 km/s.  The current model is
+    the start model plus `offset_km` ([70, 70], e.g. the field the scene's
+    initial Gaussians render; default none), or, with `net`, that model
+    moved by the network's predicted update (clipped to 1.5..4.5 km/s):
+    pairs at the points a first learned step reaches, so that a second one
+    learns to correct the first.
+    Returns device tensors grads [n, 70, 70], fields_km [n, 70, 70] and
+    target [n, 70, 70]."""
     from . import synthetic
 
     gs, fs, ts = [], [], []
@@ -170,24 +203,31 @@ def make_pairs(physics, n: int, seed: int, bands_hz=(6.0, 9.0, 12.0, 15.0), batc
     for s in range(0, n, batch):
         m = min(batch, n - s)
         start, true = synthetic.batch(m, seed=seed * 1000003 + s)
-        start_km = torch.from_numpy(start / 1000.0).cuda()
+        cur_km = torch.from_numpy(start / 1000.0).cuda()
+        if offset_km is not None:
+            cur_km = cur_km + offset_km.to(cur_km)
         true_km = torch.from_numpy(true / 1000.0).cuda()
         bands = [bands_hz[k] for k in rng.integers(0, len(bands_hz), m)]
         wav = torch.stack([torch.from_numpy(physics.band_wavelet(b)) for b in bands]).cuda()
         obs = physics.simulate_torch(true_km, wavelets=wav)
-        _, grads = physics.loss_and_grad_torch(start_km, obs, wavelets=wav)
+        _, grads = physics.loss_and_grad_torch(cur_km, obs, wavelets=wav)
+        if net is not None:
+            cur_km = (cur_km + net.direction(grads, cur_km)).clamp(1.5, 4.5)
+            _, grads = physics.loss_and_grad_torch(cur_km, obs, wavelets=wav)
         gs.append(grads)
-        fs.append(start_km.float())
-        ts.append(((true_km - start_km) / TARGET_SCALE_KM).float())
+        fs.append(cur_km.float())
+        ts.append(((true_km - cur_km) / TARGET_SCALE_KM).float())
     return torch.cat(gs), torch.cat(fs), torch.cat(ts)
 
 
 def train(net: Preconditioner, grads: torch.Tensor, fields_km: torch.Tensor, target: torch.Tensor,
           epochs: int = 20, batch: int = 128, lr: float = 2e-3, weight_decay: float = 1e-4,
-          seed: int = 0) -> list[float]:
-    """Fits grad_scale, then mini-batch AdamW with a cosine learning-rate
-    decay; returns the mean loss of each epoch."""
-    net.fit_scale(grads)
+          seed: int = 0, fit_scale: bool = True) -> list[float]:
+    """Fits grad_scale (unless `fit_scale` is False: a later training stage
+    keeps the first stage's input scaling), then mini-batch AdamW with a
+    cosine learning-rate decay; returns the mean loss of each epoch."""
+    if fit_scale:
+        net.fit_scale(grads)
     x = features(grads, fields_km, net.grad_scale, net.compress)
     n = x.shape[0]
     g = torch.Generator(device="cpu").manual_seed(seed)

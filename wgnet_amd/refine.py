@@ -27,6 +27,15 @@ parameters.  Two ways of using it (`pc_mode`):
   "replace"  L-BFGS runs on the preconditioned gradients; a model whose
              preconditioned line search fails to reduce the misfit continues
              with plain gradients (its L-BFGS memory is reset).
+
+`update`, when given, maps the same inputs to a predicted model update (km/s,
+[n, 70, 70]).  It is fitted onto the scene parameters (least squares between
+the scene's field change and the update; no solves) and the first
+`jumps_per_band` steps of each band try the fitted parameter step at full
+length, each from the point the previous one reached, judged by the Armijo
+test with the true misfit slope.  A learned step (`update`, or `precondition` in
+"first" mode) that fails its line search costs its trial solves and the band
+continues with L-BFGS.
 """
 
 from __future__ import annotations
@@ -92,13 +101,17 @@ class _Group:
     def assemble(self, rows: list[int], vecs: list[np.ndarray], need_grad: bool):
         """fields [n, 70, 70] of members `rows` at parameter vectors `vecs`,
         and the leaf tensor of those vectors."""
-        n, n_c = len(rows), COARSE * COARSE
         v = torch.tensor(np.stack(vecs), dtype=torch.float32, device=self.device, requires_grad=need_grad)
+        return self.field_of(rows, v), v
+
+    def field_of(self, rows: list[int], v: torch.Tensor) -> torch.Tensor:
+        """fields [n, 70, 70] of members `rows` at parameters v [n, 81 + 6 K]."""
+        n, n_c = len(rows), COARSE * COARSE
         coarse = v[:, :n_c].reshape(n, 1, COARSE, COARSE)
         field = self.dense[rows] + F.interpolate(coarse, size=(GRID, GRID), mode="bilinear", align_corners=True)[:, 0]
         if self.n_g:
             field = field + render_residual(v[:, n_c:].reshape(n, self.n_g, 6), self.slots)
-        return field, v
+        return field
 
 
 class _Scene:
@@ -134,13 +147,15 @@ class _Request(NamedTuple):
     vec: np.ndarray
     band: float | None
     precondition: bool
+    jump: bool = False
 
 
 def _program(scene: _Scene, budget_solves: int, bands_hz: Sequence[float | None], use_pc: bool,
-             pc_mode: str = "first"):
+             pc_mode: str = "first", jumps_per_band: int = 0):
     """One model's projected L-BFGS with the band schedule (refine.py:92-169).
-    Yields _Request; receives (loss, plain_grad, pc_grad) for gradient
-    requests (pc_grad None when not preconditioning) and the loss otherwise.
+    Yields _Request; receives (loss, plain_grad, pc_grad, jump_step) for
+    gradient requests (pc_grad / jump_step None when not asked for; all in
+    the scene's scaled units) and the loss otherwise.
     Returns (x, initial_loss, final_loss, solves, history, preconditioned)."""
     solves = 0
     history: list = []
@@ -155,7 +170,8 @@ def _program(scene: _Scene, budget_solves: int, bands_hz: Sequence[float | None]
 
     for band in bands_hz:
         band_budget = solves + per_band
-        cur_loss, g_plain, g_pc = yield _Request(True, x, band, use_pc or first)
+        jumps_left = jumps_per_band
+        cur_loss, g_plain, g_pc, g_jump = yield _Request(True, x, band, use_pc or first, jumps_left > 0)
         solves += 2
         cur_grad = g_pc if use_pc else g_plain
         s_hist: list = []
@@ -165,14 +181,22 @@ def _program(scene: _Scene, budget_solves: int, bands_hz: Sequence[float | None]
         while solves + 1 < band_budget:
             d = lbfgs_direction(cur_grad, s_hist, y_hist, rho)
             slope = float(cur_grad @ d)
-            took_learned = False
-            if learned:
+            took_learned = took_jump = False
+            if jumps_left > 0 and g_jump is not None:
+                # the band's first steps try the fitted predicted update, at
+                # full length, judged by the true misfit slope
+                jumps_left -= 1
+                sl = float(g_plain @ g_jump)
+                if np.all(np.isfinite(g_jump)) and np.isfinite(sl) and sl < 0.0:
+                    d, slope = g_jump, sl
+                    took_jump = True
+            elif learned:
                 # the band's first step goes along the learned direction,
                 # judged by the true misfit slope
                 learned = False
                 dl = -g_pc * (np.linalg.norm(cur_grad) / max(float(np.linalg.norm(g_pc)), 1e-300))
                 sl = float(cur_grad @ dl)
-                if np.isfinite(sl) and sl < 0.0:
+                if np.all(np.isfinite(dl)) and np.isfinite(sl) and sl < 0.0:
                     d, slope = dl, sl
                     took_learned = True
             if not np.isfinite(slope) or slope >= 0.0:
@@ -180,7 +204,7 @@ def _program(scene: _Scene, budget_solves: int, bands_hz: Sequence[float | None]
                 d = -cur_grad
                 slope = -float(cur_grad @ cur_grad)
             norm = float(np.linalg.norm(d))
-            if norm > 1.0:
+            if norm > 1.0 and not took_jump:
                 d, slope = d / norm, slope / norm
             step, accepted = 1.0, False
             for _ in range(LINE_SEARCH_TRIALS):
@@ -194,7 +218,7 @@ def _program(scene: _Scene, budget_solves: int, bands_hz: Sequence[float | None]
                         x, cur_loss = cand, cand_loss
                         accepted = True
                         break
-                    cand_loss, c_plain, c_pc = yield _Request(True, cand, band, use_pc)
+                    cand_loss, c_plain, c_pc, g_jump = yield _Request(True, cand, band, use_pc, jumps_left > 0)
                     solves += 2
                     cand_grad = c_pc if use_pc else c_plain
                     s, y = (cand - x) / scales, cand_grad - cur_grad
@@ -208,8 +232,11 @@ def _program(scene: _Scene, budget_solves: int, bands_hz: Sequence[float | None]
                     break
                 step *= 0.5
             history.append({"band_hz": band, "solves": solves, "loss": cur_loss, "accepted": accepted,
-                            "preconditioned": use_pc or took_learned})
+                            "preconditioned": use_pc or took_learned or took_jump, "jump": took_jump})
             if not accepted:
+                if took_jump or took_learned:
+                    jumps_left = 0
+                    continue  # L-BFGS from the same point
                 if use_pc:
                     # plain L-BFGS from here on: the last plain gradient is at x
                     use_pc = False
@@ -273,7 +300,40 @@ def _chain(field_t, parts, gv: torch.Tensor, retain: bool) -> list[np.ndarray]:
     return res
 
 
-def _drive(scenes, programs, physics: WavePhysics, observed_full: torch.Tensor, precondition):
+def _fit_update(scenes, pending, ids, update: torch.Tensor, iters: int = 40) -> list[np.ndarray]:
+    """Per request `ids[k]`, the parameter step u (unscaled) whose field change
+    best matches the velocity update `update[k]` (km/s) in least squares:
+    min_u ||field(x + u) - field(x) - update||^2, solved per group by L-BFGS
+    in the scene's scaled units (u = scales * delta), no solves."""
+    out = [None] * len(ids)
+    by_group: dict = {}
+    for pos, i in enumerate(ids):
+        by_group.setdefault(id(scenes[i].group), []).append(pos)
+    for positions in by_group.values():
+        grp = scenes[ids[positions[0]]].group
+        rows = [scenes[ids[p]].row for p in positions]
+        x = torch.tensor(np.stack([pending[ids[p]].vec for p in positions]), dtype=torch.float32, device=grp.device)
+        sc = torch.tensor(np.stack([scenes[ids[p]].scales for p in positions]), dtype=torch.float32,
+                          device=grp.device)
+        with torch.no_grad():
+            want = grp.field_of(rows, x) + update[torch.tensor(positions, device=update.device)].to(grp.device)
+        delta = torch.zeros_like(x, requires_grad=True)
+        opt = torch.optim.LBFGS([delta], lr=1.0, max_iter=iters, history_size=10, line_search_fn="strong_wolfe")
+
+        def closure():
+            opt.zero_grad()
+            loss = ((grp.field_of(rows, x + sc * delta) - want) ** 2).sum()
+            loss.backward()
+            return loss
+
+        opt.step(closure)
+        u = (sc * delta).detach().cpu().numpy().astype(np.float64)
+        for k, p in enumerate(positions):
+            out[p] = u[k]
+    return out
+
+
+def _drive(scenes, programs, physics: WavePhysics, observed_full: torch.Tensor, precondition, update=None):
     """Runs the programs in lockstep; returns their return values."""
     dev = physics.device
     band_obs: dict = {}
@@ -319,6 +379,14 @@ def _drive(scenes, programs, physics: WavePhysics, observed_full: torch.Tensor, 
                     pcg = precondition(grads[sel], fields[sel], [grad_ids[k] for k in pc_rows])
                 else:
                     pcg = precondition(grads[sel], fields[sel])
+            jumps = [None] * len(grad_ids)
+            jump_rows = [k for k, i in enumerate(grad_ids) if pending[i].jump]
+            if jump_rows and update is not None:
+                sel = torch.tensor(jump_rows, device=dev)
+                ids = [grad_ids[k] for k in jump_rows]
+                upd = update(grads[sel], fields[sel], ids) if _takes_ids(update) else update(grads[sel], fields[sel])
+                for k, u in zip(jump_rows, _fit_update(scenes, pending, ids, upd.detach().float())):
+                    jumps[k] = u / scenes[grad_ids[k]].scales
             losses = losses.cpu().numpy()
             plain = _chain(field_t, parts, grads, retain=pcg is not None)
             pc_full = None
@@ -329,7 +397,7 @@ def _drive(scenes, programs, physics: WavePhysics, observed_full: torch.Tensor, 
             for k, i in enumerate(grad_ids):
                 want_pc = pending[i].precondition and pcg is not None
                 sc = scenes[i].scales
-                answers[i] = (float(losses[k]), plain[k] * sc, pc_full[k] * sc if want_pc else None)
+                answers[i] = (float(losses[k]), plain[k] * sc, pc_full[k] * sc if want_pc else None, jumps[k])
         for i, ans in answers.items():
             try:
                 pending[i] = programs[i].send(ans)
@@ -369,6 +437,8 @@ def refine_batch(
     device: torch.device | str = "cpu",
     precondition: Callable[[torch.Tensor, torch.Tensor], torch.Tensor] | None = None,
     pc_mode: str = "first",
+    update: Callable[[torch.Tensor, torch.Tensor], torch.Tensor] | None = None,
+    jumps_per_band: int = 1,
 ) -> list[RefineResult]:
     """Refines B scenes together: dense_km [B, 70, 70], gaussians [B, 1225, 6]
     activated, observed_full [B, shots, nt, 70] (numpy or tensor).  Scene
@@ -389,8 +459,12 @@ def refine_batch(
             scenes[b] = _Scene(grp, row, gaussians[b])
     if pc_mode not in ("first", "replace"):
         raise ValueError("pc_mode must be 'first' or 'replace'")
-    programs = [_program(sc, budget_solves, tuple(bands_hz), precondition is not None, pc_mode) for sc in scenes]
-    outs = _drive(scenes, programs, physics, obs, precondition)
+    if update is not None and precondition is not None and pc_mode == "first":
+        raise ValueError("update and a 'first'-mode precondition both claim each band's first step")
+    jumps = int(jumps_per_band) if update is not None else 0
+    programs = [_program(sc, budget_solves, tuple(bands_hz), precondition is not None, pc_mode, jumps)
+                for sc in scenes]
+    outs = _drive(scenes, programs, physics, obs, precondition, update)
     el = time.perf_counter() - t0
     return [_result(sc, out, el) for sc, out in zip(scenes, outs)]
 
