@@ -191,6 +191,14 @@ int32_t wg_stream_tiles(void);
  * either way; off runs the chunks one after another on the caller's stream. */
 int32_t wg_set_overlap(wg_engine *e, int32_t on);
 int32_t wg_set_stream_tile(wg_engine *e, int32_t tile);
+/* Temporal blocking of the HBM-streamed adjoint (no reference counterpart):
+ * time steps per launch, 1 (default) or 2.  A two-step launch recomputes a
+ * 2-cell ring around each tile and reads and writes the wavefield, memory
+ * variables and gradient once per two steps.  Results are identical either
+ * way; which is faster depends on the grid (an autotuner times both); the
+ * wavelet-gradient adjoint (wg_adjoint_apply_dev with dwav) always runs
+ * single steps. */
+int32_t wg_set_time_block(wg_engine *e, int32_t steps);
 
 /* Kernel timing (no reference counterpart; the reference records no
  * timestamps, wave_wgpu.rs:829).  When enabled, HIP events are recorded on the

@@ -10,7 +10,8 @@ source position and its own band-limited wavelet; slots beyond a solve's shot
 count are inactive padding (zero traces, zero gradient, no time steps).  A
 group runs as one launch of the resident kernels (one workgroup per unit, all
 nt steps, LDS-resident wavefield, snapshots to HBM) when its grid fits a
-workgroup, else as the HBM-streamed per-step kernels with the tile shape the
+workgroup, else as the HBM-streamed per-step kernels with the tile shape and
+adjoint steps per launch (one, or two with temporal blocking) that the
 autotuner picked for that grid.  Groups are issued longest first, and the
 whole batch (every group's forward, misfit and adjoint) is captured once as a
 HIP graph and replayed.
@@ -140,9 +141,23 @@ class UnevenBatch:
         return sum(s.shots for s in self.solves)
 
     def autotune(self, repeats: int = 2) -> dict:
-        """Times every tile shape of the HBM-streamed kernels on each streamed
-        group's own forward and keeps the fastest; returns {grid: tile}."""
+        """For each streamed group: times every tile shape of the HBM-streamed
+        kernels on the group's own forward and keeps the fastest, then times
+        the group's loss and gradient with one and two adjoint steps per
+        launch and keeps the faster; returns {(nvx, nvz, nt): (tile, steps)}.
+        Results do not depend on either choice."""
         lib = self._lib
+
+        def timed(fn) -> float:
+            fn()  # warm-up
+            ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            ev0.record()
+            for _ in range(repeats):
+                fn()
+            ev1.record()
+            torch.cuda.synchronize()
+            return ev0.elapsed_time(ev1)
+
         chosen = {}
         for g in self.groups:
             if not g.engine.streamed:
@@ -150,18 +165,18 @@ class UnevenBatch:
             best, best_t = 0, math.inf
             for tile in range(lib.wg_stream_tiles()):
                 _lib.check(lib.wg_set_stream_tile(g.engine.handle, tile))
-                self._simulate_group(g)  # warm-up
-                ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-                ev0.record()
-                for _ in range(repeats):
-                    self._simulate_group(g)
-                ev1.record()
-                torch.cuda.synchronize()
-                t = ev0.elapsed_time(ev1)
+                t = timed(lambda: self._simulate_group(g))
                 if t < best_t:
                     best, best_t = tile, t
             _lib.check(lib.wg_set_stream_tile(g.engine.handle, best))
-            chosen[g.key] = best
+            steps, steps_t = 1, math.inf
+            for tb in (1, 2):
+                _lib.check(lib.wg_set_time_block(g.engine.handle, tb))
+                t = timed(lambda: self._run_group(g))
+                if t < steps_t:
+                    steps, steps_t = tb, t
+            _lib.check(lib.wg_set_time_block(g.engine.handle, steps))
+            chosen[g.key] = (best, steps)
         self.graph = None
         return chosen
 
@@ -204,12 +219,15 @@ class UnevenBatch:
         return out
 
     # -- loss and gradient ---------------------------------------------------
+    def _run_group(self, g: _Group) -> None:
+        b = g.buf
+        _lib.check(self._lib.wg_loss_grad_units_dev(
+            g.engine.handle, len(g.ids), _p(b["cell"]), _p(b["w"]), _p(b["active"]), _p(b["vel"]), _p(b["wav"]),
+            _p(b["obs"]), _p(b["loss"]), _p(b["grad"]), _p(b["traces"]), _p(b["energy"]), _stream()))
+
     def _run(self) -> None:
         for g in self.groups:
-            b = g.buf
-            _lib.check(self._lib.wg_loss_grad_units_dev(
-                g.engine.handle, len(g.ids), _p(b["cell"]), _p(b["w"]), _p(b["active"]), _p(b["vel"]), _p(b["wav"]),
-                _p(b["obs"]), _p(b["loss"]), _p(b["grad"]), _p(b["traces"]), _p(b["energy"]), _stream()))
+            self._run_group(g)
 
     def run_eager(self) -> None:
         """Loss and gradient of every solve, launched directly."""

@@ -108,6 +108,7 @@ struct wg_engine {
     hipEvent_t ev_pipe[2] = {nullptr, nullptr};
     int cus = 256;                 // compute units of the device
     bool overlap = true;           // overlap chunk forwards with the previous chunk's adjoint
+    int time_block = 1;            // adjoint steps per streamed launch (1 or 2)
     hipStream_t last = nullptr;    // stream of the last enqueued call
     bool last_valid = false;
     size_t snap_budget = 0;
@@ -385,7 +386,7 @@ constexpr int kMaxStepUnits = 65535;   // units of one step launch (grid z dimen
 StreamPlan plan_stream(wg_engine* e, int B, bool need_snaps) {
     const wg::Grid& g = e->g;
     const size_t S = e->cfg.shots, cells = (size_t)g.nx * g.nz, inner = g.inner;
-    const size_t state = 18 * cells * sizeof(float);
+    const size_t state = 20 * cells * sizeof(float);
     StreamPlan P{1, g.nt, 1, need_snaps};
     const size_t full = state + (need_snaps ? (size_t)g.nt * inner * sizeof(float) : 0);
     if (!need_snaps || full * S <= e->snap_budget) {
@@ -419,7 +420,7 @@ int stream_alloc(wg_engine* e, const StreamPlan& P, int units) {
     int rc;
     if ((rc = ensure(e, e->fstate, 8 * units * cells * sizeof(float)))) return rc;
     if (!P.snaps) return WG_OK;
-    if ((rc = ensure(e, e->astate, 10 * units * cells * sizeof(float)))) return rc;
+    if ((rc = ensure(e, e->astate, 12 * units * cells * sizeof(float)))) return rc;
     if ((rc = ensure(e, e->snaps, (size_t)units * P.seg_len * e->g.inner * sizeof(float)))) return rc;
     if (P.nseg > 1 && (rc = ensure(e, e->ckpt, (size_t)P.nseg * 8 * units * cells * sizeof(float)))) return rc;
     return WG_OK;
@@ -461,7 +462,7 @@ int stream_adjoint(wg_engine* e, const StreamPlan& P, int units, int unit0, cons
     const wg::Grid& g = e->g;
     const size_t cells = (size_t)g.nx * g.nz;
     const size_t sbytes = 8 * units * cells * sizeof(float);
-    LAUNCH(wg::launch_zero((float*)e->astate.p, 10 * units * cells, st));
+    LAUNCH(wg::launch_zero((float*)e->astate.p, 12 * units * cells, st));
     LAUNCH(wg::launch_zero(gunit + (size_t)unit0 * cells, units * cells, st));
     wg::StepParams pf = step_params(e, vel, units, unit0);
     pf.snaps = (float*)e->snaps.p;
@@ -474,6 +475,11 @@ int stream_adjoint(wg_engine* e, const StreamPlan& P, int units, int unit0, cons
     pa.resid = resid;
     pa.grad = gunit;
     pa.dwav = dwav;
+    // lambda buffers (0..3) holding lambda^t and lambda^{t+1}, and the half the
+    // memory variables are read from; a single step updates lambda^{t+1} in
+    // place, a two-step launch writes the two other buffers
+    const bool two = e->time_block == 2 && !dwav;
+    int cur = 0, prev = 1, half = 0;
     for (int seg = P.nseg - 1; seg >= 0; --seg) {
         const int t0 = seg * P.seg_len, t1 = std::min(g.nt, t0 + P.seg_len);
         if (P.nseg > 1) {
@@ -487,10 +493,29 @@ int stream_adjoint(wg_engine* e, const StreamPlan& P, int units, int unit0, cons
             }
         }
         pa.snap_t0 = P.nseg > 1 ? t0 : 0;
-        Timed tm(e, &e->ev_adj, st, t1 - t0);
-        for (int t = t1 - 1; t >= t0; --t) {
+        const int launches = two ? (t1 - t0 + 1) / 2 : t1 - t0;
+        Timed tm(e, &e->ev_adj, st, launches);
+        for (int t = t1 - 1; t >= t0;) {
             pa.t = t;
-            LAUNCH(wg::launch_adj_step(pa, units, st));
+            pa.lam_cur = cur;
+            pa.lam_prev = prev;
+            pa.half = half;
+            if (two && t - 1 >= t0) {
+                int o[2], n = 0;
+                for (int k = 0; k < 4; ++k)
+                    if (k != cur && k != prev) o[n++] = k;
+                pa.lam_out0 = o[0];
+                pa.lam_out1 = o[1];
+                LAUNCH(wg::launch_adj_step2(pa, units, st));
+                prev = o[0];
+                cur = o[1];
+                t -= 2;
+            } else {
+                LAUNCH(wg::launch_adj_step(pa, units, st));
+                std::swap(cur, prev);
+                t -= 1;
+            }
+            half ^= 1;
         }
     }
     return WG_OK;
@@ -1120,6 +1145,13 @@ int32_t wg_set_overlap(wg_engine* e, int32_t on) {
     if (!e) return fail(WG_E_INVALID, "null engine");
     Guard G(e);
     e->overlap = on != 0;
+    return WG_OK;
+}
+
+int32_t wg_set_time_block(wg_engine* e, int32_t steps) {
+    if (!e || steps < 1 || steps > 2) return fail(WG_E_INVALID, "steps must be 1 or 2");
+    Guard G(e);
+    e->time_block = steps;
     return WG_OK;
 }
 

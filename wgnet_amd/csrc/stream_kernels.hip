@@ -96,6 +96,19 @@ __global__ __launch_bounds__(NT) void k_fwd_step(StepParams P) {
             tr[r] = acc;
         }
     }
+    // own cells' u^{t-1} and velocity, loaded before the staging so that
+    // their latency overlaps it
+    const int lx = tid % TX;
+    float pu[CPT], pv[CPT];
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+        const int x = xi0 + lx + 2, z = zi0 + tid / TX + RZ * k + 2;
+        pu[k] = pv[k] = 0.0f;
+        if (x < g.nx - 2 && z < g.nz - 2) {
+            pu[k] = Unxt[z * g.nx + x];
+            pv[k] = vis_raw(vvis, g, x, z);
+        }
+    }
     // Stage u (tile + 2-cell halo), a*psi_x (x halo) and a*psi_z (z halo).
     for (int j = tid; j < HZ * HX; j += NT) {
         const int x = xi0 + (j % HX), z = zi0 + (j / HX);  // padded coords: halo starts at inner - 2
@@ -115,8 +128,8 @@ __global__ __launch_bounds__(NT) void k_fwd_step(StepParams P) {
             sAZ[j] = (x < g.nx && z < g.nz) ? paz[z] * PZc[z * g.nx + x] : 0.0f;
         }
     __syncthreads();
-    const int lx = tid % TX;
     const float f_t = P.wavelet[(size_t)unit * P.wav_stride + t];
+#pragma unroll
     for (int k = 0; k < CPT; ++k) {
         const int lz = tid / TX + RZ * k;
         const int x = xi0 + lx + 2, z = zi0 + lz + 2;
@@ -151,11 +164,11 @@ __global__ __launch_bounds__(NT) void k_fwd_step(StepParams P) {
             w = fwd_w<false, true>(sU, li, HX, u, g.rh, g.rh2, AXc, mx, psx, zex, AZc, TX, mz, psz, zez);
         else
             w = fwd_w<false, false>(sU, li, HX, u, g.rh, g.rh2, AXc, mx, psx, zex, AZc, TX, mz, psz, zez);
-        const float vr = vis_raw(vvis, g, x, z);
+        const float vr = pv[k];
         const float v = clampv(vr, g.vmin, g.vmax);
         const size_t q = (size_t)(z - 2) * g.ni + (x - 2);
         if (STORE) P.snaps[((size_t)(t - P.snap_t0) * P.units + ul) * g.inner + q] = 2.0f * v * g.dt2 * w;
-        float next = v * v * g.dt2 * w + 2.0f * u - Unxt[i];
+        float next = v * v * g.dt2 * w + 2.0f * u - pu[k];
         if (BORN) {
             const float m = (vr > g.vmin && vr < g.vmax) ? 1.0f : 0.0f;
             next += (vis_raw(P.dvel + (size_t)b * g.nvx * g.nvz, g, x, z) * m) *
@@ -201,11 +214,13 @@ __global__ __launch_bounds__(NT) void k_adj_step(StepParams P) {
     const int unit = P.unit0 + ul;
     const int b = unit / g.shots, s = src_slot(P.geo, unit, unit - (unit / g.shots) * g.shots);
     const size_t cells = (size_t)g.nx * g.nz;
-    const int t = P.t, q = (g.nt - 1 - t) & 1;
-    // arrays: lam0 lam1 | Px0 Px1 | Zx0 Zx1 | Pz0 Pz1 | Zz0 Zz1 (halves by q)
+    const int t = P.t, q = P.half;
+    // arrays: lam0 lam1 | Px0 Px1 | Zx0 Zx1 | Pz0 Pz1 | Zz0 Zz1 | lam2 lam3; the
+    // memory variables are read from half q and written to half 1 - q, and
+    // lambda^{t-1} overwrites lambda^{t+1} in place (own cell only)
     auto arr = [&](int a) { return P.state + ((size_t)a * P.units + ul) * cells; };
-    float* Lcur = arr(q);
-    float* Lnxt = arr(1 - q);
+    float* Lcur = arr(P.lam_cur < 2 ? P.lam_cur : P.lam_cur + 8);
+    float* Lnxt = arr(P.lam_prev < 2 ? P.lam_prev : P.lam_prev + 8);
     const float* PXr = arr(2 + q);
     float* PXw = arr(3 - q);
     const float* ZXr = arr(4 + q);
@@ -239,6 +254,22 @@ __global__ __launch_bounds__(NT) void k_adj_step(StepParams P) {
             acc += (-(v * v * g.dt2) * P.geo.src_w[s * 4 + c]) * Lcur[cell];
         }
         P.dwav[(size_t)unit * g.nt + t] = acc;
+    }
+    // own cells' lambda^t, lambda^{t+1}, snapshot and gradient, loaded before
+    // the staging so that their latency overlaps it
+    const int lx = tid % TX;
+    float pl[CPT], pn[CPT], ps[CPT], pg[CPT];
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+        const int x = xi0 + lx + 2, z = zi0 + tid / TX + RZ * k + 2;
+        pl[k] = pn[k] = ps[k] = pg[k] = 0.0f;
+        if (x < g.nx - 2 && z < g.nz - 2) {
+            const int i = z * g.nx + x;
+            pl[k] = Lcur[i];
+            pn[k] = Lnxt[i];
+            ps[k] = P.snaps[((size_t)(t - P.snap_t0) * P.units + ul) * g.inner + (size_t)(z - 2) * g.ni + (x - 2)];
+            pg[k] = grad[i];
+        }
     }
     // V on the tile + halo (vw(), wgsl:53-56); 0 on the ring (lambda is 0 there)
     for (int j = tid; j < HZ * HX; j += NT) {
@@ -284,9 +315,9 @@ __global__ __launch_bounds__(NT) void k_adj_step(StepParams P) {
             sUZ[j] = U1;
         }
     __syncthreads();
-    const int lx = tid % TX;
     const float f_t = P.wavelet[(size_t)unit * P.wav_stride + t];
     const float* res = P.resid + ((size_t)unit * g.nt + t) * g.nrec;
+#pragma unroll
     for (int k = 0; k < CPT; ++k) {
         const int lz = tid / TX + RZ * k;
         const int x = xi0 + lx + 2, z = zi0 + lz + 2;
@@ -314,10 +345,9 @@ __global__ __launch_bounds__(NT) void k_adj_step(StepParams P) {
         } else {
             w += adj_axis_int(sV, li, HX, v0, rh);
         }
-        const float lam = Lcur[i];
-        float next = w + 2.0f * lam - Lnxt[i];
-        const size_t qq = (size_t)(z - 2) * g.ni + (x - 2);
-        float gk = lam * P.snaps[((size_t)(t - P.snap_t0) * P.units + ul) * g.inner + qq];
+        const float lam = pl[k];
+        float next = w + 2.0f * lam - pn[k];
+        float gk = lam * ps[k];
         if (z >= P.geo.band_z0 && z < P.geo.band_z0 + P.geo.band_rows) {
             const int o = (z - P.geo.band_z0) * g.nx + x;
             for (int e = P.geo.band_off[o]; e < P.geo.band_off[o + 1]; ++e) next += P.geo.ent_w[e] * res[P.geo.ent_r[e]];
@@ -328,8 +358,340 @@ __global__ __launch_bounds__(NT) void k_adj_step(StepParams P) {
                 const float v = vel_at(vvis, g, x, z);
                 gk += lam * (-2.0f * v * g.dt2 * P.geo.src_w[s * 4 + c] * f_t);
             }
-        grad[i] += gk;
+        grad[i] = pg[k] + gk;
         Lnxt[i] = next;
+    }
+}
+
+
+// --------------------------------------------------------------------------
+// Two adjoint steps, t and t - 1, per launch (temporal blocking).  Step A
+// advances the tile and the 2-cell ring around it (the cells step B's
+// stencils read; the ring's corners are not needed) from lambda^t,
+// lambda^{t+1} and the C-PML memory variables, which are read from HBM once
+// per two steps; step B advances the tile from step A's values, kept in
+// registers and LDS.  Ring cells are recomputed by the neighbouring tiles
+// with the same operations, so the result is bit-identical to two single
+// steps.  The gradient is read and written once per two steps, its two terms
+// added in the single steps' order.  lambda^{t-1} and lambda^{t-2} go to two
+// buffers other than the two read, and the memory variables to the other
+// half, so no tile reads what another writes in the same launch.
+// --------------------------------------------------------------------------
+template <class TL>
+__global__ __launch_bounds__(NT) void k_adj_step2(StepParams P) {
+    WG_TILE_CONSTANTS
+    constexpr int AX = TX + 8, AZ = TZ + 8;  // tile + 4-cell halo
+    constexpr int BX = TX + 4, BZ = TZ + 4;  // tile + 2-cell halo
+    constexpr int RING = 4 * TX + 4 * TZ;    // 2-cell ring without its corners
+    constexpr int RPT = (RING + NT - 1) / NT;
+    // strips: step A x (rows tile+2, cols tile+4) and z (rows tile+4, cols
+    // tile+2), T then U1 each; step B reuses the space for x (rows tile, cols
+    // tile+2) and z (rows tile+2, cols tile)
+    constexpr int SAX = BZ * AX, SAZ = AZ * BX;
+    constexpr int A_XT = 0, A_XU = SAX, A_ZT = 2 * SAX, A_ZU = 2 * SAX + SAZ;
+    constexpr int SBX = TZ * BX, SBZ = BZ * TX;
+    constexpr int B_XT = 0, B_XU = SBX, B_ZT = 2 * SBX, B_ZU = 2 * SBX + SBZ;
+    static_assert(2 * SBX + 2 * SBZ <= 2 * SAX + 2 * SAZ, "step B strips must fit in step A's");
+    __shared__ float sV[AZ * AX];  // V^t on tile + 4 (pitch AX), then V^{t-1} on tile + 2 (pitch BX)
+    __shared__ float sS[2 * SAX + 2 * SAZ];
+    const Grid& g = P.g;
+    const int ul = blockIdx.z;
+    const int unit = P.unit0 + ul;
+    const int b = unit / g.shots, s = src_slot(P.geo, unit, unit - (unit / g.shots) * g.shots);
+    const size_t cells = (size_t)g.nx * g.nz;
+    const int t = P.t, h = P.half;
+    auto arr = [&](int a) { return P.state + ((size_t)a * P.units + ul) * cells; };
+    auto lam = [&](int k) { return arr(k < 2 ? k : k + 8); };
+    const float* Lcur = lam(P.lam_cur);    // lambda^t
+    const float* Lprev = lam(P.lam_prev);  // lambda^{t+1}
+    float* O0 = lam(P.lam_out0);           // lambda^{t-1}
+    float* O1 = lam(P.lam_out1);           // lambda^{t-2}
+    const float* PXr = arr(2 + h);
+    float* PXw = arr(3 - h);
+    const float* ZXr = arr(4 + h);
+    float* ZXw = arr(5 - h);
+    const float* PZr = arr(6 + h);
+    float* PZw = arr(7 - h);
+    const float* ZZr = arr(8 + h);
+    float* ZZw = arr(9 - h);
+    float* grad = P.grad + (size_t)unit * cells;
+    const float* vvis = P.vel + (size_t)b * g.nvx * g.nvz;
+    const float* pax = P.geo.prof;
+    const float* pbx = pax + g.nx;
+    const float* pdbx = pbx + g.nx;
+    const float* paz = P.geo.prof + 3 * g.nx;
+    const float* pbz = paz + g.nz;
+    const float* pdbz = pbz + g.nz;
+    const int tid = threadIdx.x;
+    const int xi0 = blockIdx.x * TX, zi0 = blockIdx.y * TZ;  // tile origin; cell (rx, rz) is padded (xi0+2+rx, zi0+2+rz)
+    const int ra = g.region_adj;
+    const float rh = g.rh;
+
+    if (!unit_active(P.geo, unit)) return;  // padding unit; its gradient rows are zeroed by the host
+
+    // cells of this thread: CPT of the tile (rows tid / TX + k RZ of column
+    // tid % TX) and up to RPT of the ring (top and bottom rows, then the left
+    // and right columns); their lambda^t and lambda^{t+1} (and for the tile
+    // cells the two snapshots and the gradient) are loaded before the staging
+    // so that their latency overlaps it
+    auto inner_cell = [&](int rx, int rz) {
+        const int x = xi0 + 2 + rx, z = zi0 + 2 + rz;
+        return x >= 2 && z >= 2 && x < g.nx - 2 && z < g.nz - 2;
+    };
+    const int lx = tid % TX;
+    float pl[CPT], pp[CPT], ps0[CPT], ps1[CPT], pg[CPT];
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+        const int lz = tid / TX + RZ * k;
+        pl[k] = pp[k] = ps0[k] = ps1[k] = pg[k] = 0.0f;
+        if (inner_cell(lx, lz)) {
+            const int x = xi0 + 2 + lx, z = zi0 + 2 + lz;
+            const int i = z * g.nx + x;
+            const size_t qq = (size_t)(z - 2) * g.ni + (x - 2);
+            pl[k] = Lcur[i];
+            pp[k] = Lprev[i];
+            ps0[k] = P.snaps[((size_t)(t - P.snap_t0) * P.units + ul) * g.inner + qq];
+            ps1[k] = P.snaps[((size_t)(t - 1 - P.snap_t0) * P.units + ul) * g.inner + qq];
+            pg[k] = grad[i];
+        }
+    }
+    int rrx[RPT], rrz[RPT];
+    float rl[RPT], rp[RPT];
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+        const int j = tid + NT * r;
+        int rx = 0, rz = -100;  // rz = -100: no ring cell for this thread
+        if (j < 2 * TX) {
+            rz = -2 + j / TX;
+            rx = j % TX;
+        } else if (j < 4 * TX) {
+            rz = TZ + (j - 2 * TX) / TX;
+            rx = (j - 2 * TX) % TX;
+        } else if (j < 4 * TX + 2 * TZ) {
+            rz = (j - 4 * TX) / 2;
+            rx = -2 + ((j - 4 * TX) & 1);
+        } else if (j < RING) {
+            rz = (j - 4 * TX - 2 * TZ) / 2;
+            rx = TX + ((j - 4 * TX - 2 * TZ) & 1);
+        }
+        rrx[r] = rx;
+        rrz[r] = rz;
+        rl[r] = rp[r] = 0.0f;
+        if (rz != -100 && inner_cell(rx, rz)) {
+            const int i = (zi0 + 2 + rz) * g.nx + (xi0 + 2 + rx);
+            rl[r] = Lcur[i];
+            rp[r] = Lprev[i];
+        }
+    }
+
+    // V^t on tile + 4 (vw(), wgsl:53-56); 0 on the ring of the grid and outside it
+    for (int j = tid; j < AZ * AX; j += NT) {
+        const int x = xi0 - 2 + (j % AX), z = zi0 - 2 + (j / AX);
+        float V = 0.0f;
+        if (x >= 0 && z >= 0 && x < g.nx && z < g.nz) {
+            const float v = vel_at(vvis, g, x, z);
+            V = v * v * g.dt2 * Lcur[z * g.nx + x];
+        }
+        sV[j] = V;
+    }
+    __syncthreads();
+    // step A strips: T = (1+b)V + bZ, U1 = db T + bP along each axis
+    const bool tax2 = xi0 < ra || xi0 + TX + 4 > g.nx - ra;
+    const bool taz2 = zi0 < ra || zi0 + TZ + 4 > g.nz - ra;
+    if (tax2)
+        for (int j = tid; j < SAX; j += NT) {
+            const int hx = j % AX, hz = j / AX;  // rx = hx - 4, rz = hz - 2
+            const int x = xi0 - 2 + hx, z = zi0 + hz;
+            float T = 0.0f, U1 = 0.0f;
+            if (x >= 0 && x < g.nx && z < g.nz) {
+                const int i = z * g.nx + x;
+                const float bb = pbx[x];
+                T = (1.0f + bb) * sV[(hz + 2) * AX + hx] + bb * ZXr[i];
+                U1 = pdbx[x] * T + bb * PXr[i];
+            }
+            sS[A_XT + j] = T;
+            sS[A_XU + j] = U1;
+        }
+    if (taz2)
+        for (int j = tid; j < SAZ; j += NT) {
+            const int hx = j % BX, hz = j / BX;  // rx = hx - 2, rz = hz - 4
+            const int x = xi0 + hx, z = zi0 - 2 + hz;
+            float T = 0.0f, U1 = 0.0f;
+            if (z >= 0 && x < g.nx && z < g.nz) {
+                const int i = z * g.nx + x;
+                const float bb = pbz[z];
+                T = (1.0f + bb) * sV[hz * AX + hx + 2] + bb * ZZr[i];
+                U1 = pdbz[z] * T + bb * PZr[i];
+            }
+            sS[A_ZT + j] = T;
+            sS[A_ZU + j] = U1;
+        }
+    __syncthreads();
+
+    // Step A at cell (rx, rz): lambda^{t-1}, V^{t-1} and the new memory
+    // variables (0 where the axis is outside the region, as the never-written
+    // HBM entries are); false outside the inner grid (all zero there).
+    const float* res_a = P.resid + ((size_t)unit * g.nt + t) * g.nrec;
+    auto step_a = [&](int rx, int rz, float l0, float lp, float& l1, float& V1, float& px, float& zx, float& pz,
+                      float& zz) -> bool {
+        l1 = V1 = px = zx = pz = zz = 0.0f;
+        const int x = xi0 + 2 + rx, z = zi0 + 2 + rz;
+        if (!inner_cell(rx, rz)) return false;
+        const int i = z * g.nx + x;
+        const int li = (rz + 4) * AX + (rx + 4);
+        const float v0 = sV[li];
+        float w;  // x term, then += z term (0 + x skipped: equal up to the sign of zero)
+        if (x < ra || x >= g.nx - ra) {
+            const float opb[5] = {1.0f + pbx[x - 2], 1.0f + pbx[x - 1], 1.0f + pbx[x], 1.0f + pbx[x + 1],
+                                  1.0f + pbx[x + 2]};
+            float Pv = PXr[i], Zv = ZXr[i];
+            const int o = (rz + 2) * AX + (rx + 4);
+            w = adj_axis_pml(sS + A_XT + o, sS + A_XU + o, 1, opb, pax[x], v0, rh, Pv, Zv);
+            px = Pv;
+            zx = Zv;
+        } else {
+            w = adj_axis_int(sV, li, 1, v0, rh);
+        }
+        if (z < ra || z >= g.nz - ra) {
+            const float opb[5] = {1.0f + pbz[z - 2], 1.0f + pbz[z - 1], 1.0f + pbz[z], 1.0f + pbz[z + 1],
+                                  1.0f + pbz[z + 2]};
+            float Pv = PZr[i], Zv = ZZr[i];
+            const int o = (rz + 4) * BX + (rx + 2);
+            w += adj_axis_pml(sS + A_ZT + o, sS + A_ZU + o, BX, opb, paz[z], v0, rh, Pv, Zv);
+            pz = Pv;
+            zz = Zv;
+        } else {
+            w += adj_axis_int(sV, li, AX, v0, rh);
+        }
+        float next = w + 2.0f * l0 - lp;
+        if (z >= P.geo.band_z0 && z < P.geo.band_z0 + P.geo.band_rows) {
+            const int o = (z - P.geo.band_z0) * g.nx + x;
+            for (int e = P.geo.band_off[o]; e < P.geo.band_off[o + 1]; ++e) next += P.geo.ent_w[e] * res_a[P.geo.ent_r[e]];
+        }
+        l1 = next;
+        const float v = vel_at(vvis, g, x, z);
+        V1 = v * v * g.dt2 * next;
+        return true;
+    };
+
+    float lt1[CPT], Vo[CPT], pxo[CPT], zxo[CPT], pzo[CPT], zzo[CPT], gk[CPT];
+    bool in[CPT];
+    const float f_a = P.wavelet[(size_t)unit * P.wav_stride + t];
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+        const int lz = tid / TX + RZ * k;
+        in[k] = step_a(lx, lz, pl[k], pp[k], lt1[k], Vo[k], pxo[k], zxo[k], pzo[k], zzo[k]);
+        gk[k] = 0.0f;
+        if (in[k]) {
+            const int x = xi0 + 2 + lx, z = zi0 + 2 + lz;
+            const int i = z * g.nx + x;
+            float gg = pl[k] * ps0[k];
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                if (P.geo.src_cell[s * 4 + c] == i) {
+                    const float v = vel_at(vvis, g, x, z);
+                    gg += pl[k] * (-2.0f * v * g.dt2 * P.geo.src_w[s * 4 + c] * f_a);
+                }
+            gk[k] = gg;
+        }
+    }
+    float rV[RPT], rpx[RPT], rzx[RPT], rpz[RPT], rzz[RPT];
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+        float l1;
+        rV[r] = rpx[r] = rzx[r] = rpz[r] = rzz[r] = 0.0f;
+        if (rrz[r] != -100) step_a(rrx[r], rrz[r], rl[r], rp[r], l1, rV[r], rpx[r], rzx[r], rpz[r], rzz[r]);
+    }
+    __syncthreads();  // step A's reads of sV and the strips are done
+
+    // V^{t-1} on tile + 2 and the step B strips, from step A's registers
+    const bool tax = xi0 + 2 < ra || xi0 + TX + 2 > g.nx - ra;
+    const bool taz = zi0 + 2 < ra || zi0 + TZ + 2 > g.nz - ra;
+    auto put = [&](int rx, int rz, float V1, float px, float zx, float pz, float zz) {
+        const int x = xi0 + 2 + rx, z = zi0 + 2 + rz;
+        const bool ing = x < g.nx && z < g.nz;
+        sV[(rz + 2) * BX + (rx + 2)] = V1;
+        if (tax && rz >= 0 && rz < TZ) {
+            float T = 0.0f, U1 = 0.0f;
+            if (ing) {
+                const float bb = pbx[x];
+                T = (1.0f + bb) * V1 + bb * zx;
+                U1 = pdbx[x] * T + bb * px;
+            }
+            sS[B_XT + rz * BX + (rx + 2)] = T;
+            sS[B_XU + rz * BX + (rx + 2)] = U1;
+        }
+        if (taz && rx >= 0 && rx < TX) {
+            float T = 0.0f, U1 = 0.0f;
+            if (ing) {
+                const float bb = pbz[z];
+                T = (1.0f + bb) * V1 + bb * zz;
+                U1 = pdbz[z] * T + bb * pz;
+            }
+            sS[B_ZT + (rz + 2) * TX + rx] = T;
+            sS[B_ZU + (rz + 2) * TX + rx] = U1;
+        }
+    };
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) put(lx, tid / TX + RZ * k, Vo[k], pxo[k], zxo[k], pzo[k], zzo[k]);
+#pragma unroll
+    for (int r = 0; r < RPT; ++r)
+        if (rrz[r] != -100) put(rrx[r], rrz[r], rV[r], rpx[r], rzx[r], rpz[r], rzz[r]);
+    __syncthreads();
+
+    // Step B (time t - 1) on the tile
+    const float* res_b = P.resid + ((size_t)unit * g.nt + (t - 1)) * g.nrec;
+    const float f_b = P.wavelet[(size_t)unit * P.wav_stride + (t - 1)];
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+        if (!in[k]) continue;
+        const int lz = tid / TX + RZ * k;
+        const int x = xi0 + 2 + lx, z = zi0 + 2 + lz;
+        const int i = z * g.nx + x;
+        const int li = (lz + 2) * BX + (lx + 2);
+        const float v0 = sV[li];
+        float w;
+        if (x < ra || x >= g.nx - ra) {
+            const float opb[5] = {1.0f + pbx[x - 2], 1.0f + pbx[x - 1], 1.0f + pbx[x], 1.0f + pbx[x + 1],
+                                  1.0f + pbx[x + 2]};
+            float Pv = pxo[k], Zv = zxo[k];
+            const int o = lz * BX + (lx + 2);
+            w = adj_axis_pml(sS + B_XT + o, sS + B_XU + o, 1, opb, pax[x], v0, rh, Pv, Zv);
+            PXw[i] = Pv;
+            ZXw[i] = Zv;
+        } else {
+            w = adj_axis_int(sV, li, 1, v0, rh);
+        }
+        if (z < ra || z >= g.nz - ra) {
+            const float opb[5] = {1.0f + pbz[z - 2], 1.0f + pbz[z - 1], 1.0f + pbz[z], 1.0f + pbz[z + 1],
+                                  1.0f + pbz[z + 2]};
+            float Pv = pzo[k], Zv = zzo[k];
+            const int o = (lz + 2) * TX + lx;
+            w += adj_axis_pml(sS + B_ZT + o, sS + B_ZU + o, TX, opb, paz[z], v0, rh, Pv, Zv);
+            PZw[i] = Pv;
+            ZZw[i] = Zv;
+        } else {
+            w += adj_axis_int(sV, li, BX, v0, rh);
+        }
+        const float l1 = lt1[k];
+        float next = w + 2.0f * l1 - pl[k];
+        float gb = l1 * ps1[k];
+        if (z >= P.geo.band_z0 && z < P.geo.band_z0 + P.geo.band_rows) {
+            const int o = (z - P.geo.band_z0) * g.nx + x;
+            for (int e = P.geo.band_off[o]; e < P.geo.band_off[o + 1]; ++e) next += P.geo.ent_w[e] * res_b[P.geo.ent_r[e]];
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+            if (P.geo.src_cell[s * 4 + c] == i) {
+                const float v = vel_at(vvis, g, x, z);
+                gb += l1 * (-2.0f * v * g.dt2 * P.geo.src_w[s * 4 + c] * f_b);
+            }
+        float gacc = pg[k];
+        gacc += gk[k];
+        gacc += gb;
+        grad[i] = gacc;
+        O0[i] = l1;
+        O1[i] = next;
     }
 }
 
@@ -354,6 +716,13 @@ hipError_t adj_step_t(const StepParams& p, int units, hipStream_t st) {
     return hipGetLastError();
 }
 
+template <class T>
+hipError_t adj_step2_t(const StepParams& p, int units, hipStream_t st) {
+    const dim3 grid((p.g.ni + T::TX - 1) / T::TX, (p.g.nzi + T::TZ - 1) / T::TZ, units);
+    hipLaunchKernelGGL(st::k_adj_step2<T>, grid, dim3(st::NT), 0, st, p);
+    return hipGetLastError();
+}
+
 using T0 = st::Tile<64, 16>;
 using T1 = st::Tile<128, 8>;
 using T2 = st::Tile<32, 32>;
@@ -368,6 +737,16 @@ hipError_t launch_fwd_step(const StepParams& p, int units, bool store, bool born
         case 2: return fwd_step_t<T2>(p, units, store, born, st);
         case 3: return fwd_step_t<T3>(p, units, store, born, st);
         default: return fwd_step_t<T0>(p, units, store, born, st);
+    }
+}
+
+hipError_t launch_adj_step2(const StepParams& p, int units, hipStream_t st) {
+    if (units < 1 || units > 65535 || p.dwav || p.t < 1) return hipErrorInvalidValue;
+    switch (p.tile) {
+        case 1: return adj_step2_t<T1>(p, units, st);
+        case 2: return adj_step2_t<T2>(p, units, st);
+        case 3: return adj_step2_t<T3>(p, units, st);
+        default: return adj_step2_t<T0>(p, units, st);
     }
 }
 

@@ -104,6 +104,10 @@ struct StepParams {
     float* grad;             // adjoint: [all units, nx * nz], accumulated
     float* dwav;             // adjoint: [all units, nt] or null
     int tile;                // tile shape: 0 64x16, 1 128x8, 2 32x32, 3 64x32
+    // adjoint state buffers: lambda buffers 0..3 (lambda^t, lambda^{t+1}, and
+    // for two-step launches the two written ones) and the half (0 / 1) the
+    // C-PML memory variables are read from (the other half is written)
+    int lam_cur, lam_prev, lam_out0, lam_out1, half;
 };
 constexpr int kStreamTiles = 4;
 
@@ -113,6 +117,8 @@ hipError_t launch_causal_fir(const float* x, long long nseries, int nt, int nrec
                              hipStream_t st);
 hipError_t launch_fwd_step(const StepParams& p, int units, bool store, bool born, hipStream_t st);
 hipError_t launch_adj_step(const StepParams& p, int units, hipStream_t st);
+// Two adjoint steps (t, t - 1) per launch; p.dwav must be null.
+hipError_t launch_adj_step2(const StepParams& p, int units, hipStream_t st);
 
 // Returns hipSuccess or the launch error.  lds_bytes_* report the dynamic
 // LDS a launch needs (0 when the grid does not fit one workgroup).

@@ -128,6 +128,15 @@ class PyWaveEngine:
         """Device-memory bound for snapshots and checkpoints."""
         _lib.check(self._lib.wg_set_snapshot_budget(self._h, int(nbytes)))
 
+    def set_time_block(self, steps: int) -> None:
+        """Adjoint time steps per HBM-streamed launch (1, the default, or 2);
+        results are identical either way."""
+        _lib.check(self._lib.wg_set_time_block(self._h, int(steps)))
+
+    def set_stream_tile(self, tile: int) -> None:
+        """Tile shape of the HBM-streamed kernels (0 .. wg_stream_tiles() - 1)."""
+        _lib.check(self._lib.wg_set_stream_tile(self._h, int(tile)))
+
     def dims(self) -> tuple[int, int, int]:
         nx, nz, pad = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
         _lib.check(self._lib.wg_engine_dims(self._h, ctypes.byref(nx), ctypes.byref(nz), ctypes.byref(pad)))
