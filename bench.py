@@ -101,6 +101,8 @@ def main() -> int:
     ap.add_argument("--no-compress", action="store_true")
     ap.add_argument("--stages", type=int, choices=(1, 2), default=2)
     ap.add_argument("--jumps", type=int, default=1)
+    ap.add_argument("--time-block", type=int, choices=(1, 2), default=1,
+                    help="large workload: adjoint steps per streamed launch")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only; this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -478,6 +480,7 @@ def bench_large(args) -> int:
     assert eng.streamed
     src = np.stack([np.linspace(0.0, (nvx - 1) * 10.0, S), np.full(S, 10.0)], 1).astype(np.float32)
     eng.set_sources(src)
+    eng.set_time_block(args.time_block)
     start, true = synthetic.large_batch(B, seed=3000, nvz=nvz, nvx=nvx)
     obs = ph.simulate_torch(torch.from_numpy(true / 1000.0).cuda())
     fields = torch.from_numpy(start / 1000.0).cuda().requires_grad_(True)
