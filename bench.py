@@ -103,6 +103,8 @@ def main() -> int:
     ap.add_argument("--jumps", type=int, default=1)
     ap.add_argument("--time-block", type=int, choices=(1, 2), default=1,
                     help="large workload: adjoint steps per streamed launch")
+    ap.add_argument("--time-split", type=int, default=4,
+                    help="default workload: time parts per unit of the resident work queue (0: chunk pipeline)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only; this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -120,12 +122,13 @@ def main() -> int:
 
     import torch
 
-    from wgnet_amd import synthetic
+    from wgnet_amd import _lib, synthetic
     from wgnet_amd.physics import WavePhysics
 
     B, S, nt, dt = args.models, 5, args.nt, 0.001
     ph = WavePhysics(shots=S, nt=nt, dt=dt)
     ph.engine.set_sources(SURFACE_SRC)
+    _lib.check(ph.engine._lib.wg_set_time_split(ph.engine.handle, args.time_split))
     start, true = synthetic.batch(B, seed=1000)
     start_km = torch.from_numpy(start / 1000.0).cuda()
     obs = ph.simulate_torch(torch.from_numpy(true / 1000.0).cuda())
@@ -142,8 +145,10 @@ def main() -> int:
     torch.cuda.synchronize()
     lib, h = ph.engine._lib, ph.engine.handle
     fms, fn, ams, an = ctypes.c_double(), ctypes.c_int64(), ctypes.c_double(), ctypes.c_int64()
+    qms, qn = ctypes.c_double(), ctypes.c_int64()
     lib.wg_timing_enable(h, 1)
     lib.wg_timing_read(h, ctypes.byref(fms), ctypes.byref(fn), ctypes.byref(ams), ctypes.byref(an))
+    lib.wg_timing_read_queue(h, ctypes.byref(qms), ctypes.byref(qn))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -151,6 +156,7 @@ def main() -> int:
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     lib.wg_timing_read(h, ctypes.byref(fms), ctypes.byref(fn), ctypes.byref(ams), ctypes.byref(an))
+    lib.wg_timing_read_queue(h, ctypes.byref(qms), ctypes.byref(qn))
     lib.wg_timing_enable(h, 0)
     assert bool(torch.isfinite(losses).all())
 
@@ -168,7 +174,15 @@ def main() -> int:
     adj_bytes = unit_bytes * units * args.steps / max(an.value, 1)
     f_avg = fms.value / max(fn.value, 1) / 1e3
     a_avg = ams.value / max(an.value, 1) / 1e3
-    if a_avg >= f_avg:
+    q_avg = qms.value / max(qn.value, 1) / 1e3
+    launch_ms = {"k_forward_fx": round(1e3 * f_avg, 3), "k_adjoint_fx": round(1e3 * a_avg, 3)}
+    launches = {"k_forward_fx": fn.value / args.steps, "k_adjoint_fx": an.value / args.steps}
+    if qn.value:
+        # work queue: one launch runs every unit's forward and adjoint
+        kern, kbytes, kavg = "k_queue_fx", 2 * unit_bytes * units * args.steps / qn.value, q_avg
+        launch_ms = {"k_queue_fx": round(1e3 * q_avg, 3)}
+        launches = {"k_queue_fx": qn.value / args.steps}
+    elif a_avg >= f_avg:
         kern, kbytes, kavg = "k_adjoint_fx", adj_bytes, a_avg
     else:
         kern, kbytes, kavg = "k_forward_fx", fwd_bytes, f_avg
@@ -202,8 +216,8 @@ def main() -> int:
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic_from_profiles(kern),
             "algorithmic_bytes_per_launch": int(kbytes),
-            "launch_ms": {"k_forward_fx": round(1e3 * f_avg, 3), "k_adjoint_fx": round(1e3 * a_avg, 3)},
-            "launches_per_step": {"k_forward_fx": fn.value / args.steps, "k_adjoint_fx": an.value / args.steps},
+            "launch_ms": launch_ms,
+            "launches_per_step": launches,
             "shot_solve_hbm_frac": round(ss_bytes * value / 1e9 / HBM_PEAK_GBS, 4),
         },
     }

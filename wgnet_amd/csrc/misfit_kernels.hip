@@ -155,6 +155,18 @@ hipError_t launch_zero(float* p, size_t n, hipStream_t st) {
     return hipGetLastError();
 }
 
+// Marks every loss NaN when a device-side wait gave up (err != 0).
+__global__ void k_flag_nan(const int* err, float* loss, int B) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < B && *err != 0) loss[b] = __builtin_nanf("");
+}
+
+hipError_t launch_flag_nan(const int* err, float* loss, int B, hipStream_t st) {
+    if (B <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_flag_nan, dim3((B + 63) / 64), dim3(64), 0, st, err, loss, B);
+    return hipGetLastError();
+}
+
 // Causal low-pass along time (python/wgnet/physics.py:48-57: conv1d of each
 // receiver trace with the taps h, left-padded by L - 1 zeros):
 //   y[n][t][r] = sum_{k = max(0, L-1-t)}^{L-1} h[k] x[n][t + k - (L - 1)][r].

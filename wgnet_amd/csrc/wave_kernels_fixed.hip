@@ -9,6 +9,8 @@
 // slot of a thread sits at a compile-time offset from the thread's base cell:
 // LDS and snapshot addresses are base + immediate, and the x-axis C-PML
 // coefficients are per-thread constants held in registers.
+#include <algorithm>
+
 #include "stencil.h"
 #include "wave_kernels.h"
 
@@ -142,6 +144,34 @@ struct FSt {
     float dv[BORN ? K : 1];
 };
 
+// Time-split runs hand their state to the next part through HBM: entry j of
+// thread tid at st[j * NT + tid].  SAVE writes the register, else reads it.
+template <bool SAVE>
+__device__ __forceinline__ void xfer(float& v, float* st, int& j, int tid) {
+    float* p = st + (size_t)(j++) * NT + tid;
+    if constexpr (SAVE) *p = v;
+    else v = *p;
+}
+
+// Forward: u^{t-1} and the C-PML memory variables of every slot.
+template <bool SAVE, int K, bool PX, bool PZ, bool STORE, bool BORN>
+__device__ __forceinline__ void f_xfer(FSt<K, PX, PZ, STORE, BORN>& S, float* st, int& j, int tid) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        xfer<SAVE>(S.uprev[k], st, j, tid);
+        if constexpr (PX) {
+            xfer<SAVE>(S.px[k], st, j, tid);
+            xfer<SAVE>(S.zx[k], st, j, tid);
+        }
+        if constexpr (PZ) {
+            xfer<SAVE>(S.pz[k], st, j, tid);
+            xfer<SAVE>(S.zz[k], st, j, tid);
+        }
+    }
+}
+template <class R>
+constexpr int f_regs() { return R::K * (1 + (R::BX ? 2 : 0) + (R::BZ ? 2 : 0)); }
+
 struct FCtxF {
     float *U, *AX, *AZ;
     const float* PR;
@@ -248,9 +278,22 @@ __device__ __forceinline__ void f_write(FSt<R::K, R::BX, R::BZ, STORE, BORN>& S,
     }
 }
 
+// Per-unit state of a time-split forward: the LDS field and strips [0, F_PR),
+// then f_regs of every rectangle per thread.
+template <class G>
+struct FState {
+    using RR = Rects<G, G::REG - 2>;
+    static constexpr int REGS = f_regs<typename RR::I>() + f_regs<typename RR::X>() + f_regs<typename RR::ZT>() +
+                                f_regs<typename RR::ZB>() + f_regs<typename RR::CT>() + f_regs<typename RR::CB>();
+    static constexpr size_t FLOATS = (size_t)G::F_PR + (size_t)REGS * NT;
+};
+
+// Steps [t0, t1) of the forward of unit P.unit0 + blk (snapshots at launch
+// position blk).  st_in (t0 > 0) restores the state a previous part saved;
+// st_out (t1 < nt) saves it for the next part.
 template <class G, bool STORE, bool BORN>
-__global__ __launch_bounds__(NT) void k_forward_fx(FwdParams P) {
-    extern __shared__ float lds[];
+__device__ __forceinline__ void forward_run(const FwdParams& P, float* lds, int blk, int t0, int t1,
+                                            const float* st_in, float* st_out) {
     using RR = Rects<G, G::REG - 2>;
     const Grid& g = P.g;
     FCtxF C;
@@ -264,7 +307,7 @@ __global__ __launch_bounds__(NT) void k_forward_fx(FwdParams P) {
     C.dt2 = g.dt2;
 
     const int tid = threadIdx.x;
-    const int unit = P.unit0 + blockIdx.x;
+    const int unit = P.unit0 + blk;
     const int b = unit / g.shots;
     const int s = src_slot(P.geo, unit, unit - b * g.shots);
     if (!unit_active(P.geo, unit)) {  // padding unit of an uneven batch
@@ -272,7 +315,7 @@ __global__ __launch_bounds__(NT) void k_forward_fx(FwdParams P) {
         return;
     }
 
-    for (int j = tid; j < G::F_PR; j += NT) lds[j] = 0.0f;
+    for (int j = tid; j < G::F_PR; j += NT) lds[j] = st_in ? st_in[j] : 0.0f;
     // profile tables: a, 1 + b, b, db for x then z (geometry prof: ax bx dbx az bz dbz)
     for (int j = tid; j < G::NX; j += NT) {
         const float bb = P.geo.prof[G::NX + j];
@@ -294,8 +337,8 @@ __global__ __launch_bounds__(NT) void k_forward_fx(FwdParams P) {
     const float* dvis = BORN ? P.dvel + (size_t)b * G::NVX * G::NVZ : nullptr;
     const float* wav = P.wavelet + (size_t)unit * P.wav_stride;
     float* traces = P.traces + (size_t)unit * g.nt * g.nrec;
-    float* snaps = STORE ? P.snaps + (size_t)blockIdx.x * g.nt * G::INNER : nullptr;
-    const float* bsnaps = BORN ? P.born_snaps + (size_t)blockIdx.x * g.nt * G::INNER : nullptr;
+    float* snaps = STORE ? P.snaps + (size_t)blk * g.nt * G::INNER : nullptr;
+    const float* bsnaps = BORN ? P.born_snaps + (size_t)blk * g.nt * G::INNER : nullptr;
 
     Place<typename RR::I> pI;
     Place<typename RR::X> pX;
@@ -316,6 +359,16 @@ __global__ __launch_bounds__(NT) void k_forward_fx(FwdParams P) {
     f_init<G>(sZB, pZB, vvis, dvis, g);
     f_init<G>(sCT, pCT, vvis, dvis, g);
     f_init<G>(sCB, pCB, vvis, dvis, g);
+    if (st_in) {
+        float* sr = const_cast<float*>(st_in) + G::F_PR;
+        int j = 0;
+        f_xfer<false>(sI, sr, j, tid);
+        f_xfer<false>(sX, sr, j, tid);
+        f_xfer<false>(sZT, sr, j, tid);
+        f_xfer<false>(sZB, sr, j, tid);
+        f_xfer<false>(sCT, sr, j, tid);
+        f_xfer<false>(sCB, sr, j, tid);
+    }
 
     // Point-source corners owned by this thread: added after the update by
     // the owner, -(v^2 dt^2) w f[t] (lib.rs:209-215); Born adds its derivative.
@@ -360,7 +413,7 @@ __global__ __launch_bounds__(NT) void k_forward_fx(FwdParams P) {
     // x-axis coefficients of this thread's band column (shared by X, CT, CB)
     const Pml pmx = pml_x<G, typename RR::X>(PR, pX.xi + 2);
 
-    for (int t = 0; t < g.nt; ++t) {
+    for (int t = t0; t < t1; ++t) {
         if (tid < g.nrec) {  // receivers record u^t before the update (lib.rs:146-149)
             float acc = 0.0f;
 #pragma unroll
@@ -388,6 +441,23 @@ __global__ __launch_bounds__(NT) void k_forward_fx(FwdParams P) {
             if (scell[c] >= 0) C.U[scell[c]] += scoef[c] * f_t;
         __syncthreads();
     }
+    if (st_out) {
+        for (int j = tid; j < G::F_PR; j += NT) st_out[j] = lds[j];
+        float* sr = st_out + G::F_PR;
+        int j = 0;
+        f_xfer<true>(sI, sr, j, tid);
+        f_xfer<true>(sX, sr, j, tid);
+        f_xfer<true>(sZT, sr, j, tid);
+        f_xfer<true>(sZB, sr, j, tid);
+        f_xfer<true>(sCT, sr, j, tid);
+        f_xfer<true>(sCB, sr, j, tid);
+    }
+}
+
+template <class G, bool STORE, bool BORN>
+__global__ __launch_bounds__(NT) void k_forward_fx(FwdParams P) {
+    extern __shared__ float lds[];
+    forward_run<G, STORE, BORN>(P, lds, blockIdx.x, 0, P.g.nt, nullptr, nullptr);
 }
 
 // ===========================================================================
@@ -399,6 +469,27 @@ struct ASt {
     float Px[PX ? K : 1], Zx[PX ? K : 1];
     float Pz[PZ ? K : 1], Zz[PZ ? K : 1];
 };
+
+// Adjoint: lambda^t, lambda^{t+1}, the gradient sums and the memory variables.
+template <bool SAVE, int K, bool PX, bool PZ>
+__device__ __forceinline__ void a_xfer(ASt<K, PX, PZ>& S, float* st, int& j, int tid) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        xfer<SAVE>(S.lam[k], st, j, tid);
+        xfer<SAVE>(S.lamp[k], st, j, tid);
+        xfer<SAVE>(S.gr[k], st, j, tid);
+        if constexpr (PX) {
+            xfer<SAVE>(S.Px[k], st, j, tid);
+            xfer<SAVE>(S.Zx[k], st, j, tid);
+        }
+        if constexpr (PZ) {
+            xfer<SAVE>(S.Pz[k], st, j, tid);
+            xfer<SAVE>(S.Zz[k], st, j, tid);
+        }
+    }
+}
+template <class R>
+constexpr int a_regs() { return R::K * (3 + (R::BX ? 2 : 0) + (R::BZ ? 2 : 0)); }
 
 struct ACtxF {
     float *V, *TX, *UX, *TZ, *UZ;
@@ -557,9 +648,22 @@ __device__ __forceinline__ void a_store(ASt<R::K, R::BX, R::BZ>& S, const Place<
     }
 }
 
+// Per-unit state of a time-split adjoint: a_regs of every rectangle per
+// thread (the LDS content is rebuilt from it at every step).
+template <class G>
+struct AState {
+    using RR = Rects<G, G::REGA - 2>;
+    static constexpr int REGS = a_regs<typename RR::I>() + a_regs<typename RR::X>() + a_regs<typename RR::ZT>() +
+                                a_regs<typename RR::ZB>() + a_regs<typename RR::CT>() + a_regs<typename RR::CB>();
+    static constexpr size_t FLOATS = (size_t)REGS * NT;
+};
+
+// Steps t_hi - 1 down to t_lo of the adjoint of unit P.unit0 + blk
+// (snapshots at launch position blk); st_in (t_hi < nt) restores, st_out
+// (t_lo > 0) saves the state; the gradient is written when t_lo = 0.
 template <class G, bool DWAV>
-__global__ __launch_bounds__(NT) void k_adjoint_fx(AdjParams P) {
-    extern __shared__ float lds[];
+__device__ __forceinline__ void adjoint_run(const AdjParams& P, float* lds, int blk, int t_hi, int t_lo,
+                                            const float* st_in, float* st_out) {
     using RR = Rects<G, G::REGA - 2>;
     const Grid& g = P.g;
     float* PR = lds + G::A_PR;
@@ -585,7 +689,7 @@ __global__ __launch_bounds__(NT) void k_adjoint_fx(AdjParams P) {
     C.rh = g.rh;
 
     const int tid = threadIdx.x;
-    const int unit = P.unit0 + blockIdx.x;
+    const int unit = P.unit0 + blk;
     const int b = unit / g.shots;
     const int s = src_slot(P.geo, unit, unit - b * g.shots);
     if (!unit_active(P.geo, unit)) {  // padding unit of an uneven batch
@@ -619,7 +723,7 @@ __global__ __launch_bounds__(NT) void k_adjoint_fx(AdjParams P) {
     const float* vvis = P.vel + (size_t)b * G::NVX * G::NVZ;
     const float* wav = P.wavelet + (size_t)unit * P.wav_stride;
     const float* resid = P.resid + (size_t)unit * g.nt * g.nrec;
-    const float* snaps = P.snaps + (size_t)blockIdx.x * g.nt * G::INNER;
+    const float* snaps = P.snaps + (size_t)blk * g.nt * G::INNER;
 
     if (tid < 4) {
         const int cell = P.geo.src_cell[s * 4 + tid];
@@ -673,6 +777,16 @@ __global__ __launch_bounds__(NT) void k_adjoint_fx(AdjParams P) {
     a_init<G>(sZB, pZB, vvis, g);
     a_init<G>(sCT, pCT, vvis, g);
     a_init<G>(sCB, pCB, vvis, g);
+    if (st_in) {
+        float* sr = const_cast<float*>(st_in);
+        int j = 0;
+        a_xfer<false>(sI, sr, j, tid);
+        a_xfer<false>(sX, sr, j, tid);
+        a_xfer<false>(sZT, sr, j, tid);
+        a_xfer<false>(sZB, sr, j, tid);
+        a_xfer<false>(sCT, sr, j, tid);
+        a_xfer<false>(sCB, sr, j, tid);
+    }
     __syncthreads();
     ColX cx;
     {
@@ -691,7 +805,7 @@ __global__ __launch_bounds__(NT) void k_adjoint_fx(AdjParams P) {
     rs[3] = pZB.slot_in_rows(C.bz0, C.bz1);
     rs[4] = pCT.slot_in_rows(C.bz0, C.bz1);
     rs[5] = pCB.slot_in_rows(C.bz0, C.bz1);
-    for (int t = g.nt - 1; t >= 0; --t) {
+    for (int t = t_hi - 1; t >= t_lo; --t) {
         for (int rr = tid; rr < g.nrec; rr += NT) RES[rr] = resid[(size_t)t * g.nrec + rr];
         a_phase1<G>(sI, pI, C, cx, so[0]);
         a_phase1<G>(sX, pX, C, cx, so[1]);
@@ -716,6 +830,16 @@ __global__ __launch_bounds__(NT) void k_adjoint_fx(AdjParams P) {
         a_phase2<G>(sCB, pCB, C, cx, so[5], rs[5], snap_t, f_t);
         __syncthreads();
     }
+    if (st_out) {
+        int j = 0;
+        a_xfer<true>(sI, st_out, j, tid);
+        a_xfer<true>(sX, st_out, j, tid);
+        a_xfer<true>(sZT, st_out, j, tid);
+        a_xfer<true>(sZB, st_out, j, tid);
+        a_xfer<true>(sCT, st_out, j, tid);
+        a_xfer<true>(sCB, st_out, j, tid);
+    }
+    if (t_lo > 0) return;
     float* gout = P.grad + (size_t)unit * G::NX * G::NZ;
     for (int j = tid; j < G::NX * G::NZ; j += NT) {
         const int z = j / G::NX, x = j - z * G::NX;
@@ -727,6 +851,86 @@ __global__ __launch_bounds__(NT) void k_adjoint_fx(AdjParams P) {
     a_store<G>(sZB, pZB, gout);
     a_store<G>(sCT, pCT, gout);
     a_store<G>(sCB, pCB, gout);
+}
+
+template <class G, bool DWAV>
+__global__ __launch_bounds__(NT) void k_adjoint_fx(AdjParams P) {
+    extern __shared__ float lds[];
+    adjoint_run<G, DWAV>(P, lds, blockIdx.x, P.g.nt, 0, nullptr, nullptr);
+}
+
+// ===========================================================================
+// Time-split work queue: one launch runs the forward (with snapshots) and
+// the adjoint of every unit.  Each unit's forward and adjoint are cut into
+// `parts` time segments; task k = p * units + u is part p of unit u (parts
+// 0 .. parts-1 forward in time order, then the adjoint segments in reverse).
+// Workgroups (one per compute unit) take tasks in that order from a counter;
+// a task waits for its unit's previous part, whose state it restores from
+// HBM.  Every unit still runs the single-launch arithmetic step for step, so
+// results are bit-identical, and the cut units pack the compute units evenly
+// where whole units (320 on 256 compute units, say) leave a quarter idle.
+// A task only waits for a task taken before it by a running workgroup, so
+// the queue cannot deadlock; the wait is bounded all the same (err is set
+// and the results are flagged invalid).
+// ===========================================================================
+constexpr int kSpinLimit = 1 << 24;  // ~ seconds of s_sleep polling; tasks take milliseconds
+
+__device__ __forceinline__ void wait_done(const int* flag, int* err) {
+    for (int n = 0; __hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0; ++n) {
+        if (n > kSpinLimit) {
+            __hip_atomic_fetch_or(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return;
+        }
+        __builtin_amdgcn_s_sleep(8);
+    }
+}
+
+template <class G>
+__global__ __launch_bounds__(NT) void k_queue_fx(QueueParams Q) {
+    extern __shared__ float lds[];
+    __shared__ int task_s;
+    const int tid = threadIdx.x;
+    const int parts = Q.parts, units = Q.units;
+    const int ntask = 2 * parts * units;
+    const int nt = Q.f.g.nt, nrec = Q.f.g.nrec;
+    for (;;) {
+        __syncthreads();  // the previous task is done with LDS and task_s
+        if (tid == 0) task_s = __hip_atomic_fetch_add(Q.next, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        const int task = task_s;
+        if (task >= ntask) break;
+        const int p = task / units, u = task - p * units;
+        int* done = Q.done + (size_t)u * 2 * parts;
+        if (p > 0) {
+            if (tid == 0) wait_done(done + p - 1, Q.err);
+            __syncthreads();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // the previous part's HBM writes
+        }
+        float* fst = Q.fstate + (size_t)u * FState<G>::FLOATS;
+        float* ast = Q.astate + (size_t)u * AState<G>::FLOATS;
+        if (p < parts) {
+            const int t0 = (int)((long long)nt * p / parts), t1 = (int)((long long)nt * (p + 1) / parts);
+            forward_run<G, true, false>(Q.f, lds, u, t0, t1, p > 0 ? fst : nullptr, p < parts - 1 ? fst : nullptr);
+        } else {
+            const int j = p - parts, sg = parts - 1 - j;
+            const int t_lo = (int)((long long)nt * sg / parts), t_hi = (int)((long long)nt * (sg + 1) / parts);
+            if (j == 0) {
+                // this unit's scaled residual (p - d) scale, as RESIDUAL_WGSL
+                // forms it (wave_wgpu.rs:28-53); the loss tree runs after the queue
+                const float sc = Q.scale[u / Q.f.g.shots];
+                const size_t base = (size_t)u * nt * nrec;
+                for (int i = tid; i < nt * nrec; i += NT) {
+                    const float d = Q.f.traces[base + i] - Q.obs[base + i];
+                    Q.resid[base + i] = d * sc;
+                }
+                __syncthreads();
+            }
+            adjoint_run<G, false>(Q.a, lds, u, t_hi, t_lo, j > 0 ? ast : nullptr, j < parts - 1 ? ast : nullptr);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // this part's HBM writes, before its flag
+        __syncthreads();
+        if (tid == 0) __hip_atomic_store(done + p, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
 }
 
 using G70 = Geo<70, 70, 20>;
@@ -768,6 +972,26 @@ bool launch_adjoint_fx(const AdjParams& p, int units, hipStream_t st, hipError_t
     hipError_t e = hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e == hipSuccess) {
         hipLaunchKernelGGL(fn, dim3(units), dim3(fx::NT), lds, st, p);
+        e = hipGetLastError();
+    }
+    *err = e;
+    return true;
+}
+
+bool queue_state_fx(const Grid& g, size_t* fwd_floats, size_t* adj_floats) {
+    if (!is_g70(g)) return false;
+    *fwd_floats = fx::FState<fx::G70>::FLOATS;
+    *adj_floats = fx::AState<fx::G70>::FLOATS;
+    return true;
+}
+
+bool launch_queue_fx(const QueueParams& q, int workgroups, hipStream_t st, hipError_t* err) {
+    if (!is_g70(q.f.g)) return false;
+    const size_t lds = std::max(forward_lds_bytes_fx(q.f.g), adjoint_lds_bytes_fx(q.a.g, q.a.geo));
+    hipError_t e = hipFuncSetAttribute((const void*)fx::k_queue_fx<fx::G70>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds);
+    if (e == hipSuccess && workgroups > 0) {
+        hipLaunchKernelGGL(fx::k_queue_fx<fx::G70>, dim3(workgroups), dim3(fx::NT), lds, st, q);
         e = hipGetLastError();
     }
     *err = e;
