@@ -145,10 +145,8 @@ def main() -> int:
     torch.cuda.synchronize()
     lib, h = ph.engine._lib, ph.engine.handle
     fms, fn, ams, an = ctypes.c_double(), ctypes.c_int64(), ctypes.c_double(), ctypes.c_int64()
-    qms, qn = ctypes.c_double(), ctypes.c_int64()
     lib.wg_timing_enable(h, 1)
     lib.wg_timing_read(h, ctypes.byref(fms), ctypes.byref(fn), ctypes.byref(ams), ctypes.byref(an))
-    lib.wg_timing_read_queue(h, ctypes.byref(qms), ctypes.byref(qn))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -156,7 +154,6 @@ def main() -> int:
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     lib.wg_timing_read(h, ctypes.byref(fms), ctypes.byref(fn), ctypes.byref(ams), ctypes.byref(an))
-    lib.wg_timing_read_queue(h, ctypes.byref(qms), ctypes.byref(qn))
     lib.wg_timing_enable(h, 0)
     assert bool(torch.isfinite(losses).all())
 
@@ -174,18 +171,15 @@ def main() -> int:
     adj_bytes = unit_bytes * units * args.steps / max(an.value, 1)
     f_avg = fms.value / max(fn.value, 1) / 1e3
     a_avg = ams.value / max(an.value, 1) / 1e3
-    q_avg = qms.value / max(qn.value, 1) / 1e3
-    launch_ms = {"k_forward_fx": round(1e3 * f_avg, 3), "k_adjoint_fx": round(1e3 * a_avg, 3)}
-    launches = {"k_forward_fx": fn.value / args.steps, "k_adjoint_fx": an.value / args.steps}
-    if qn.value:
-        # work queue: one launch runs every unit's forward and adjoint
-        kern, kbytes, kavg = "k_queue_fx", 2 * unit_bytes * units * args.steps / qn.value, q_avg
-        launch_ms = {"k_queue_fx": round(1e3 * q_avg, 3)}
-        launches = {"k_queue_fx": qn.value / args.steps}
-    elif a_avg >= f_avg:
-        kern, kbytes, kavg = "k_adjoint_fx", adj_bytes, a_avg
+    # the resident work queues (one forward and one adjoint launch per step)
+    # or the chunk launches of the whole-unit kernels
+    fk, ak = ("k_queue_fwd_fx", "k_queue_adj_fx") if args.time_split > 0 else ("k_forward_fx", "k_adjoint_fx")
+    launch_ms = {fk: round(1e3 * f_avg, 3), ak: round(1e3 * a_avg, 3)}
+    launches = {fk: fn.value / args.steps, ak: an.value / args.steps}
+    if a_avg >= f_avg:
+        kern, kbytes, kavg = ak, adj_bytes, a_avg
     else:
-        kern, kbytes, kavg = "k_forward_fx", fwd_bytes, f_avg
+        kern, kbytes, kavg = fk, fwd_bytes, f_avg
     achieved = kbytes / kavg / 1e9
     # Per shot-solve algorithmic figures (bytes 8 inner nt + 12 R nt).
     ss_bytes = 8 * inner * nt + 12 * R * nt

@@ -199,18 +199,16 @@ int32_t wg_set_stream_tile(wg_engine *e, int32_t tile);
  * wavelet-gradient adjoint (wg_adjoint_apply_dev with dwav) always runs
  * single steps. */
 int32_t wg_set_time_block(wg_engine *e, int32_t steps);
-/* Resident work queue (no reference counterpart; grids of the compiled
- * 70 x 70 kernels): the loss and gradient of a whole batch run as one launch
- * in which every (model, shot) unit's forward and adjoint are cut into
- * `parts` time segments (default 4), taken from a device-side queue by one
- * workgroup per compute unit; a segment restores its unit's state from HBM.
- * Results are identical to whole-unit launches; 0 turns the queue off (the
- * chunk pipeline runs instead).  The queue also stays off when the snapshots
- * of the whole batch exceed the snapshot budget. */
+/* Resident work queues (no reference counterpart; grids of the compiled
+ * 70 x 70 kernels): the loss and gradient of a whole batch run as one
+ * forward launch and one adjoint launch, in which every (model, shot) unit's
+ * forward or adjoint is cut into `parts` time segments (default 4) taken from
+ * a device-side queue by one workgroup per compute unit; a segment restores
+ * its unit's state from HBM.  Results are identical to whole-unit launches;
+ * 0 turns the queues off (the chunk pipeline runs instead).  The queues also
+ * stay off when the snapshots of the whole batch exceed the snapshot budget.
+ * Timing (below) counts the forward and adjoint queue launches. */
 int32_t wg_set_time_split(wg_engine *e, int32_t parts);
-/* Summed duration (ms) and count of work-queue launches since the last read
- * (timing enabled as below), then clears them. */
-int32_t wg_timing_read_queue(wg_engine *e, double *ms, int64_t *launches);
 
 /* Kernel timing (no reference counterpart; the reference records no
  * timestamps, wave_wgpu.rs:829).  When enabled, HIP events are recorded on the

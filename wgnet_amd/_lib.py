@@ -74,7 +74,6 @@ SIGNATURES = {
     "wg_set_overlap": (_i32, [_vp, _i32]),
     "wg_set_time_block": (_i32, [_vp, _i32]),
     "wg_set_time_split": (_i32, [_vp, _i32]),
-    "wg_timing_read_queue": (_i32, [_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]),
     "wg_causal_filter_dev": (_i32, [_vp, ctypes.c_int64, _i32, _i32, _vp, _i32, _vp, _vp]),
     "wg_set_stream_tile": (_i32, [_vp, _i32]),
     "wg_timing_enable": (_i32, [_vp, _i32]),

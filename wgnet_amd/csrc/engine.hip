@@ -109,8 +109,8 @@ struct wg_engine {
     int cus = 256;                 // compute units of the device
     bool overlap = true;           // overlap chunk forwards with the previous chunk's adjoint
     int time_block = 1;            // adjoint steps per streamed launch (1 or 2)
-    int qparts = 4;                // time parts per unit of the resident work queue (0: chunk pipeline)
-    DevBuf qstate, qflags;         // work queue: per-unit part state; counter, error flag, done flags
+    int qparts = 4;                // time parts per unit of the resident work queues (0: chunk pipeline)
+    DevBuf qstate, qflags;         // work queues: per-unit part state; counters, error flag, done flags
     hipStream_t last = nullptr;    // stream of the last enqueued call
     bool last_valid = false;
     size_t snap_budget = 0;
@@ -126,7 +126,7 @@ struct wg_engine {
     std::mutex mu;
     // kernel timing: (start, stop, launches) records of forward / adjoint launches
     bool timing = false;
-    std::vector<TimedSpan> ev_fwd, ev_adj, ev_queue;
+    std::vector<TimedSpan> ev_fwd, ev_adj;
 
     wg::Geometry geo() const {
         wg::Geometry G;
@@ -537,23 +537,26 @@ int stream_simulate(wg_engine* e, int b0, int nb, const float* vel, const float*
     return WG_OK;
 }
 
-// Loss and gradient of all B models as one launch of the resident work queue
-// (compiled-grid kernels): every unit's forward with snapshots, residual and
-// adjoint, cut into e->qparts time parts.  Sets *used = false, doing nothing,
-// when the queue does not apply (another grid, disabled, or the snapshots
-// and part states of all units over the snapshot budget).
+// Loss and gradient of all B models on the resident work queues (compiled-
+// grid kernels): every unit's forward with snapshots, then the residual and
+// loss partials, then every unit's adjoint, the forward and the adjoint each
+// cut into e->qparts time parts.  Sets *used = false, doing nothing, when the
+// queues do not apply (another grid, disabled, or the snapshots and part
+// states of all units over the snapshot budget).
 int queue_loss_grad(wg_engine* e, int B, const float* vel, const float* obs, float* traces, const float* scale,
                     hipStream_t st, bool* used) {
     *used = false;
+    const wg::Grid& g = e->g;
     const int S = e->cfg.shots, U = B * S, parts = e->qparts;
     size_t ff = 0, af = 0;
-    if (parts <= 0 || U <= 0 || !wg::queue_state_fx(e->g, &ff, &af)) return WG_OK;
+    if (parts <= 0 || U <= 0 || !wg::queue_state_fx(g, &ff, &af)) return WG_OK;
     const size_t per_unit = snap_bytes_per_unit(e) + (ff + af) * sizeof(float);
     if (per_unit * (size_t)U > e->snap_budget) return WG_OK;
+    const long long per = (long long)S * g.nt * g.nrec;
     int rc;
     if ((rc = ensure(e, e->snaps, snap_bytes_per_unit(e) * U))) return rc;
     if ((rc = ensure(e, e->qstate, (ff + af) * sizeof(float) * U))) return rc;
-    const size_t nflag = 2 + (size_t)2 * parts * U;
+    const size_t nflag = 3 + (size_t)2 * parts * U;
     if ((rc = ensure(e, e->qflags, nflag * sizeof(int)))) return rc;
     LAUNCH(wg::launch_zero((float*)e->qflags.p, nflag, st));  // int 0 = float +0 bits
     wg::QueueParams q{};
@@ -561,23 +564,29 @@ int queue_loss_grad(wg_engine* e, int B, const float* vel, const float* obs, flo
     q.f.snaps = (float*)e->snaps.p;
     q.a = adj_params(e, vel, (const float*)e->resid.p, (float*)e->gunit.p);
     q.a.snaps = (const float*)e->snaps.p;
-    q.obs = obs;
-    q.scale = scale;
-    q.resid = (float*)e->resid.p;
     q.fstate = (float*)e->qstate.p;
     q.astate = q.fstate + ff * U;
     int* fl = (int*)e->qflags.p;
-    q.next = fl;
-    q.err = fl + 1;
-    q.done = fl + 2;
+    q.next_f = fl;
+    q.next_a = fl + 1;
+    q.err = fl + 2;
+    q.done_f = fl + 3;
+    q.done_a = q.done_f + (size_t)parts * U;
     q.units = U;
     q.parts = parts;
+    const int wgs = std::min(e->cus, parts * U);
     hipError_t err = hipSuccess;
     {
-        Timed tm(e, &e->ev_queue, st);
-        wg::launch_queue_fx(q, std::min(e->cus, 2 * parts * U), st, &err);
+        Timed tm(e, &e->ev_fwd, st);
+        wg::launch_queue_fx(q, false, wgs, st, &err);
     }
-    if (err != hipSuccess) return fail(WG_E_DEVICE, std::string("work queue launch: ") + hipGetErrorString(err));
+    if (err != hipSuccess) return fail(WG_E_DEVICE, std::string("forward queue launch: ") + hipGetErrorString(err));
+    LAUNCH(wg::launch_residual(traces, obs, scale, per, B, (float*)e->resid.p, (float*)e->partials.p, st));
+    {
+        Timed tm(e, &e->ev_adj, st);
+        wg::launch_queue_fx(q, true, wgs, st, &err);
+    }
+    if (err != hipSuccess) return fail(WG_E_DEVICE, std::string("adjoint queue launch: ") + hipGetErrorString(err));
     *used = true;
     return WG_OK;
 }
@@ -628,10 +637,8 @@ int loss_grad_core(wg_engine* e, int B, const float* vel, const float* obs, floa
     bool queued = false;
     if ((rc = queue_loss_grad(e, B, vel, obs, traces, scale, st, &queued))) return rc;
     if (queued) {
-        // loss partials (the residual is rewritten with the values the queue used)
-        LAUNCH(wg::launch_residual(traces, obs, scale, per, B, (float*)e->resid.p, (float*)e->partials.p, st));
         LAUNCH(wg::launch_loss_sum((const float*)e->partials.p, chunks, B, energy, loss, st));
-        LAUNCH(wg::launch_flag_nan((const int*)e->qflags.p + 1, loss, B, st));
+        LAUNCH(wg::launch_flag_nan((const int*)e->qflags.p + 2, loss, B, st));
         LAUNCH(wg::launch_fold(g, (const float*)e->gunit.p, vel, energy, B, grad, st));
         return WG_OK;
     }
@@ -844,7 +851,7 @@ int32_t wg_engine_create(const wg_config* cfg, wg_engine** out) {
 void wg_engine_destroy(wg_engine* e) {
     if (!e) return;
     if (e->last_valid) (void)hipStreamSynchronize(e->last);
-    for (auto* l : {&e->ev_fwd, &e->ev_adj, &e->ev_queue})
+    for (auto* l : {&e->ev_fwd, &e->ev_adj})
         for (auto& pr : *l) {
             (void)hipEventDestroy(pr.a);
             (void)hipEventDestroy(pr.b);
@@ -1209,27 +1216,6 @@ int32_t wg_set_time_split(wg_engine* e, int32_t parts) {
     if (!e || parts < 0 || parts > 16) return fail(WG_E_INVALID, "parts must be in [0, 16]");
     Guard G(e);
     e->qparts = parts;
-    return WG_OK;
-}
-
-int32_t wg_timing_read_queue(wg_engine* e, double* ms, int64_t* launches) {
-    if (!e) return fail(WG_E_INVALID, "null engine");
-    Guard G(e);
-    HIPCK(hipSetDevice(e->device));
-    double sum = 0.0;
-    int64_t n = 0;
-    for (auto& pr : e->ev_queue) {
-        HIPCK(hipEventSynchronize(pr.b));
-        float t = 0.0f;
-        HIPCK(hipEventElapsedTime(&t, pr.a, pr.b));
-        sum += t;
-        n += pr.n;
-        (void)hipEventDestroy(pr.a);
-        (void)hipEventDestroy(pr.b);
-    }
-    e->ev_queue.clear();
-    if (ms) *ms = sum;
-    if (launches) *launches = n;
     return WG_OK;
 }
 

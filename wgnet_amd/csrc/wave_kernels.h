@@ -137,27 +137,22 @@ bool launch_forward_fx(const FwdParams& p, int units, bool store, bool born, hip
                        hipError_t* err);
 bool launch_adjoint_fx(const AdjParams& p, int units, hipStream_t st, hipError_t* err);
 
-// Time-split work queue of the compiled-grid kernels: the forward (storing
-// snapshots, f.unit0 = a.unit0 = 0) and the adjoint of `units` units, each cut
-// into `parts` time segments, in one launch of `workgroups` workgroups.  The
-// adjoint's residual (traces - obs) * scale is formed in the queue, into
-// resid.  next, err and done (2 * parts * units ints) start at zero.  Both
-// return false when the grid is not the compiled one.
+// Time-split work queues of the compiled-grid kernels: the forward (storing
+// snapshots, f.unit0 = a.unit0 = 0) or the adjoint of `units` units, each cut
+// into `parts` time segments, in one launch of `workgroups` workgroups.
+// next_f, next_a, err and the done flags (parts * units ints each) start at
+// zero.  They return false when the grid is not the compiled one.
 struct QueueParams {
     FwdParams f;
     AdjParams a;
-    const float* obs;    // [units, nt, nrec]
-    const float* scale;  // [models]
-    float* resid;        // [units, nt, nrec]
-    float* fstate;       // per unit: queue_state_fx's forward floats
-    float* astate;       // per unit: its adjoint floats
-    int* next;
-    int* err;
-    int* done;
+    float* fstate;  // per unit: queue_state_fx's forward floats
+    float* astate;  // per unit: its adjoint floats
+    int *next_f, *next_a, *err;
+    int *done_f, *done_a;
     int units, parts;
 };
 bool queue_state_fx(const Grid& g, size_t* fwd_floats, size_t* adj_floats);
-bool launch_queue_fx(const QueueParams& q, int workgroups, hipStream_t st, hipError_t* err);
+bool launch_queue_fx(const QueueParams& q, bool adjoint, int workgroups, hipStream_t st, hipError_t* err);
 hipError_t launch_flag_nan(const int* err, float* loss, int B, hipStream_t st);
 size_t forward_lds_bytes_fx(const Grid& g);
 size_t adjoint_lds_bytes_fx(const Grid& g, const Geometry& geo);

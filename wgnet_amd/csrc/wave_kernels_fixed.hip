@@ -860,18 +860,20 @@ __global__ __launch_bounds__(NT) void k_adjoint_fx(AdjParams P) {
 }
 
 // ===========================================================================
-// Time-split work queue: one launch runs the forward (with snapshots) and
-// the adjoint of every unit.  Each unit's forward and adjoint are cut into
-// `parts` time segments; task k = p * units + u is part p of unit u (parts
-// 0 .. parts-1 forward in time order, then the adjoint segments in reverse).
-// Workgroups (one per compute unit) take tasks in that order from a counter;
-// a task waits for its unit's previous part, whose state it restores from
-// HBM.  Every unit still runs the single-launch arithmetic step for step, so
-// results are bit-identical, and the cut units pack the compute units evenly
-// where whole units (320 on 256 compute units, say) leave a quarter idle.
-// A task only waits for a task taken before it by a running workgroup, so
-// the queue cannot deadlock; the wait is bounded all the same (err is set
-// and the results are flagged invalid).
+// Time-split work queues: each unit's forward (with snapshots) and adjoint
+// are cut into `parts` time segments, and one launch runs all forward
+// segments, a second all adjoint segments.  Task k = p * units + u is
+// segment p of unit u (forward in time order; adjoint segments from the last
+// in time to the first).  Workgroups (one per compute unit) take tasks in
+// that order from a counter; a task waits for its unit's previous segment,
+// whose state it restores from HBM.  Every unit still runs the single-launch
+// arithmetic step for step, so results are bit-identical, and the cut units
+// pack the compute units evenly where whole units (320 on 256 compute units,
+// say) leave a quarter of them idle.  A task only waits for a task taken
+// before it by a running workgroup, so a queue cannot deadlock; the wait is
+// bounded all the same (err is set and the results are flagged invalid).
+// The two bodies are separate kernels: in one kernel their registers would
+// be allocated together and the forward's time loop would spill.
 // ===========================================================================
 constexpr int kSpinLimit = 1 << 24;  // ~ seconds of s_sleep polling; tasks take milliseconds
 
@@ -885,51 +887,57 @@ __device__ __forceinline__ void wait_done(const int* flag, int* err) {
     }
 }
 
+// Takes the next task of a queue and waits for its predecessor; returns
+// false when the queue is empty.  All threads of the workgroup call it.
+__device__ __forceinline__ bool next_task(int* next, const int* done, int ntask, int units, int* err, int* ts,
+                                          int& p, int& u) {
+    __syncthreads();  // the previous task is done with LDS and *ts
+    if (threadIdx.x == 0) *ts = __hip_atomic_fetch_add(next, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    const int task = *ts;
+    if (task >= ntask) return false;
+    p = task / units;
+    u = task - p * units;
+    if (p > 0) {
+        if (threadIdx.x == 0) wait_done(done + (size_t)(p - 1) * units + u, err);
+        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // the previous segment's HBM writes
+    }
+    return true;
+}
+
+__device__ __forceinline__ void task_done(int* done, int units, int p, int u) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // this segment's HBM writes, before its flag
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_store(done + (size_t)p * units + u, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 template <class G>
-__global__ __launch_bounds__(NT) void k_queue_fx(QueueParams Q) {
+__global__ __launch_bounds__(NT) void k_queue_fwd_fx(QueueParams Q) {
     extern __shared__ float lds[];
     __shared__ int task_s;
-    const int tid = threadIdx.x;
-    const int parts = Q.parts, units = Q.units;
-    const int ntask = 2 * parts * units;
-    const int nt = Q.f.g.nt, nrec = Q.f.g.nrec;
-    for (;;) {
-        __syncthreads();  // the previous task is done with LDS and task_s
-        if (tid == 0) task_s = __hip_atomic_fetch_add(Q.next, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __syncthreads();
-        const int task = task_s;
-        if (task >= ntask) break;
-        const int p = task / units, u = task - p * units;
-        int* done = Q.done + (size_t)u * 2 * parts;
-        if (p > 0) {
-            if (tid == 0) wait_done(done + p - 1, Q.err);
-            __syncthreads();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // the previous part's HBM writes
-        }
-        float* fst = Q.fstate + (size_t)u * FState<G>::FLOATS;
-        float* ast = Q.astate + (size_t)u * AState<G>::FLOATS;
-        if (p < parts) {
-            const int t0 = (int)((long long)nt * p / parts), t1 = (int)((long long)nt * (p + 1) / parts);
-            forward_run<G, true, false>(Q.f, lds, u, t0, t1, p > 0 ? fst : nullptr, p < parts - 1 ? fst : nullptr);
-        } else {
-            const int j = p - parts, sg = parts - 1 - j;
-            const int t_lo = (int)((long long)nt * sg / parts), t_hi = (int)((long long)nt * (sg + 1) / parts);
-            if (j == 0) {
-                // this unit's scaled residual (p - d) scale, as RESIDUAL_WGSL
-                // forms it (wave_wgpu.rs:28-53); the loss tree runs after the queue
-                const float sc = Q.scale[u / Q.f.g.shots];
-                const size_t base = (size_t)u * nt * nrec;
-                for (int i = tid; i < nt * nrec; i += NT) {
-                    const float d = Q.f.traces[base + i] - Q.obs[base + i];
-                    Q.resid[base + i] = d * sc;
-                }
-                __syncthreads();
-            }
-            adjoint_run<G, false>(Q.a, lds, u, t_hi, t_lo, j > 0 ? ast : nullptr, j < parts - 1 ? ast : nullptr);
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // this part's HBM writes, before its flag
-        __syncthreads();
-        if (tid == 0) __hip_atomic_store(done + p, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int parts = Q.parts, units = Q.units, nt = Q.f.g.nt;
+    int p, u;
+    while (next_task(Q.next_f, Q.done_f, parts * units, units, Q.err, &task_s, p, u)) {
+        float* st = Q.fstate + (size_t)u * FState<G>::FLOATS;
+        const int t0 = (int)((long long)nt * p / parts), t1 = (int)((long long)nt * (p + 1) / parts);
+        forward_run<G, true, false>(Q.f, lds, u, t0, t1, p > 0 ? st : nullptr, p < parts - 1 ? st : nullptr);
+        task_done(Q.done_f, units, p, u);
+    }
+}
+
+template <class G>
+__global__ __launch_bounds__(NT) void k_queue_adj_fx(QueueParams Q) {
+    extern __shared__ float lds[];
+    __shared__ int task_s;
+    const int parts = Q.parts, units = Q.units, nt = Q.a.g.nt;
+    int j, u;
+    while (next_task(Q.next_a, Q.done_a, parts * units, units, Q.err, &task_s, j, u)) {
+        float* st = Q.astate + (size_t)u * AState<G>::FLOATS;
+        const int sg = parts - 1 - j;
+        const int t_lo = (int)((long long)nt * sg / parts), t_hi = (int)((long long)nt * (sg + 1) / parts);
+        adjoint_run<G, false>(Q.a, lds, u, t_hi, t_lo, j > 0 ? st : nullptr, j < parts - 1 ? st : nullptr);
+        task_done(Q.done_a, units, j, u);
     }
 }
 
@@ -985,13 +993,13 @@ bool queue_state_fx(const Grid& g, size_t* fwd_floats, size_t* adj_floats) {
     return true;
 }
 
-bool launch_queue_fx(const QueueParams& q, int workgroups, hipStream_t st, hipError_t* err) {
+bool launch_queue_fx(const QueueParams& q, bool adjoint, int workgroups, hipStream_t st, hipError_t* err) {
     if (!is_g70(q.f.g)) return false;
-    const size_t lds = std::max(forward_lds_bytes_fx(q.f.g), adjoint_lds_bytes_fx(q.a.g, q.a.geo));
-    hipError_t e = hipFuncSetAttribute((const void*)fx::k_queue_fx<fx::G70>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)lds);
+    void (*fn)(QueueParams) = adjoint ? fx::k_queue_adj_fx<fx::G70> : fx::k_queue_fwd_fx<fx::G70>;
+    const size_t lds = adjoint ? adjoint_lds_bytes_fx(q.a.g, q.a.geo) : forward_lds_bytes_fx(q.f.g);
+    hipError_t e = hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e == hipSuccess && workgroups > 0) {
-        hipLaunchKernelGGL(fx::k_queue_fx<fx::G70>, dim3(workgroups), dim3(fx::NT), lds, st, q);
+        hipLaunchKernelGGL(fn, dim3(workgroups), dim3(fx::NT), lds, st, q);
         e = hipGetLastError();
     }
     *err = e;
