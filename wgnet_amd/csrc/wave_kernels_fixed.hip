@@ -899,17 +899,20 @@ __device__ __forceinline__ bool next_task(int* next, const int* done, int ntask,
     p = task / units;
     u = task - p * units;
     if (p > 0) {
+        // the acquiring load invalidates this compute unit's L1 and this
+        // XCD's L2 before the barrier lets the other waves read
         if (threadIdx.x == 0) wait_done(done + (size_t)(p - 1) * units + u, err);
         __syncthreads();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // the previous segment's HBM writes
     }
     return true;
 }
 
 __device__ __forceinline__ void task_done(int* done, int units, int p, int u) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // this segment's HBM writes, before its flag
+    // every wave waits until its stores are acknowledged by the L2, then one
+    // agent-scope release writes the L2 back before the flag is set
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    if (threadIdx.x == 0) __hip_atomic_store(done + (size_t)p * units + u, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x == 0) __hip_atomic_store(done + (size_t)p * units + u, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 template <class G>
