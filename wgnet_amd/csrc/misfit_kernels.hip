@@ -21,7 +21,16 @@ __global__ __launch_bounds__(256) void k_energy(const float* obs, long long per,
     __shared__ double part[256];
     const float* o = obs + (size_t)blockIdx.x * per;
     double acc = 0.0;
-    for (long long i = threadIdx.x; i < per; i += 256) acc += (double)o[i] * (double)o[i];
+    long long i = threadIdx.x;
+    // eight loads in flight, added in the same order as one at a time
+    for (; i + 7 * 256 < per; i += 8 * 256) {
+        float v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = o[i + u * 256];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc += (double)v[u] * (double)v[u];
+    }
+    for (; i < per; i += 256) acc += (double)o[i] * (double)o[i];
     part[threadIdx.x] = acc;
     __syncthreads();
     for (int st = 128; st > 0; st >>= 1) {
@@ -71,7 +80,16 @@ __global__ void k_loss_sum(const float* partials, long long chunks, int B, const
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= B) return;
     double acc = 0.0;
-    for (long long c = 0; c < chunks; ++c) acc += (double)partials[(size_t)b * chunks + c];
+    const float* pb = partials + (size_t)b * chunks;
+    long long c = 0;
+    for (; c + 8 <= chunks; c += 8) {  // eight loads in flight, added in order
+        float v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = pb[c + u];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc += (double)v[u];
+    }
+    for (; c < chunks; ++c) acc += (double)pb[c];
     loss[b] = energy[b] > 0.0 ? (float)acc : __builtin_nanf("");
 }
 
