@@ -33,6 +33,7 @@ import numpy as np
 
 METRIC = "fwd+adjoint shot-solves/s at 70x70x1000 FP32 vs CPU ref; refine solves-to-misfit"
 HBM_PEAK_GBS = 8000.0
+VALU_CLOCK_HZ = 2.4e9  # MI355X peak engine clock
 FP32_PEAK_TFLOPS = 157.3
 SURFACE_SRC = np.array([[0, 10], [170, 10], [340, 10], [520, 10], [690, 10]], np.float32)
 
@@ -70,6 +71,16 @@ def cpu_baseline_large(start, src, nvx, nvz, nt, dt, threads, units):
     O.batch_fwd_adj_f32(g, s, vel, w, obs, threads)
     el = time.perf_counter() - t0
     return units / el, el
+
+
+def valu_from_profiles(kernel: str):
+    """VALU instructions per launch of `kernel` (rocprofv3 SQ_INSTS_VALU) from
+    profiles/valu_<kernel>.json, or None."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", f"valu_{kernel}.json")
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f).get("sq_insts_valu_per_launch")
+    return None
 
 
 def traffic_from_profiles(kernel: str):
@@ -215,6 +226,11 @@ def main() -> int:
             "shot_solve_hbm_frac": round(ss_bytes * value / 1e9 / HBM_PEAK_GBS, 4),
         },
     }
+    valu = valu_from_profiles(kern)
+    if valu:
+        # the kernel's real bound: VALU issue (a wave64 instruction takes 4 cycles
+        # on a 16-lane SIMD; 256 CUs x 4 SIMDs at 2.4 GHz), not HBM
+        out["roofline"]["valu_issue_frac"] = round(valu * 4 / (1024 * VALU_CLOCK_HZ * kavg), 4)
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
         units_cpu = 64 * threads  # about 10-15 s of CPU work
