@@ -15,6 +15,10 @@ snapshots, misfit, HIP adjoint) plus its backward: 320 shot-solves.
 0.8 ms, adjoint with a checkpoint every 100 steps and segment recompute; it
 also reports the FP64 dot-product test of the wavelet operator at that size.
 
+`--workload uneven` runs 512 uneven solves as one HIP graph, `--workload refine`
+the learned preconditioner inside projected L-BFGS, and `--workload wavecache`
+the training-data cache writer (2560 models, nt = 2001, decimated float16 file).
+
 Prints one JSON line.  `roofline` is the dominant kernel's algorithmic HBM bytes
 per launch over its mean launch time (HIP events on the launch stream);
 `cpu_baseline` times the float32 CPU solver on the host cores on a bounded
@@ -100,7 +104,11 @@ def main() -> int:
     ap.add_argument("--nt", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--workload", choices=("batch70", "large", "copy", "refine", "uneven"), default="batch70")
+    ap.add_argument("--workload", choices=("batch70", "large", "copy", "refine", "uneven", "wavecache"),
+                    default="batch70")
+    ap.add_argument("--cache-models", type=int, default=2560)
+    ap.add_argument("--cache-batch", type=int, default=256)
+    ap.add_argument("--cache-loop-models", type=int, default=64)
     ap.add_argument("--solves", type=int, default=512)
     ap.add_argument("--train-pairs", type=int, default=20000)
     ap.add_argument("--eval-models", type=int, default=256)
@@ -128,6 +136,8 @@ def main() -> int:
         return bench_refine(args)
     if args.workload == "uneven":
         return bench_uneven(args)
+    if args.workload == "wavecache":
+        return bench_wavecache(args)
 
     import ctypes
 
@@ -463,6 +473,141 @@ def bench_uneven(args) -> int:
         "inner_gcell_steps_per_s": round(2 * cell_steps * args.steps / el / 1e9, 2),
         "eager_ms": round(1e3 * el_eager, 2),
     }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def bench_wavecache(args) -> int:
+    """The training-data cache writer: `cache-models` synthetic 70 x 70 models
+    x 5 shots, 2001 steps of 0.5 ms, decimated to 251 samples, float16 file
+    with progress and RMS bookkeeping, in batches of `cache-batch` models.
+    A step is the whole cache (fresh directory each step); the reference's
+    call pattern (one engine.simulate per model) is timed beside it on the
+    same engine."""
+    import ctypes
+    import shutil
+    import tempfile
+
+    import torch
+
+    from wgnet_amd import synthetic, wavecache
+
+    n, S, nt, dt = args.cache_models, 5, 2001, 0.0005
+    phys = wavecache.make_physics()
+    _, true = synthetic.batch(n, seed=4000)
+    fields = {"train": (true / 1000.0).astype(np.float32)}
+    root = tempfile.mkdtemp(prefix="wavecache_", dir=os.environ.get("TMPDIR", "/tmp"))
+    lib, h = phys.engine._lib, phys.engine.handle
+    fms, fn, ams, an = ctypes.c_double(), ctypes.c_int64(), ctypes.c_double(), ctypes.c_int64()
+
+    def step(k):
+        d = os.path.join(root, f"s{k}")
+        wavecache.ensure_wave_cache(fields, d, ("train",), phys, batch=args.cache_batch, log=lambda *_: None)
+        return d
+
+    try:
+        for k in range(args.warmup):
+            shutil.rmtree(step(-1 - k))
+        torch.cuda.synchronize()
+        lib.wg_timing_enable(h, 1)
+        lib.wg_timing_read(h, ctypes.byref(fms), ctypes.byref(fn), ctypes.byref(ams), ctypes.byref(an))
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            d = step(k)
+            print(f"wavecache: step {k} done", file=sys.stderr, flush=True)
+            if k + 1 < args.steps:
+                shutil.rmtree(d)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        lib.wg_timing_read(h, ctypes.byref(fms), ctypes.byref(fn), ctypes.byref(ams), ctypes.byref(an))
+        lib.wg_timing_enable(h, 0)
+        with open(os.path.join(d, "waves_train.done.json")) as f:
+            rms = json.load(f)["rms"]
+        assert np.isfinite(rms) and rms > 0.0
+        # the reference's call pattern: one host-buffer simulate per model
+        m_loop = min(args.cache_loop_models, n)
+        t1 = time.perf_counter()
+        for i in range(m_loop):
+            phys.engine.simulate(np.ascontiguousarray(fields["train"][i].reshape(-1) * 1000.0,
+                                                      dtype=np.float32))[:, ::8, :]
+        el_loop = time.perf_counter() - t1
+    finally:
+        shutil.rmtree(root, ignore_errors=True)
+
+    value = n * args.steps / el
+    nx, nz, _ = phys.engine.dims()
+    inner, R = (nx - 4) * (nz - 4), phys.engine.nrec
+    f_avg = fms.value / max(fn.value, 1) / 1e3
+    units_per_launch = n * S * args.steps / max(fn.value, 1)
+    # Algorithmic HBM bytes of the non-storing forward: the gathers written
+    # (4 B per receiver-step of each unit) and each model's velocity read once
+    # (4 B per cell; its shots share it).
+    f_bytes = int(units_per_launch * 4 * R * nt + units_per_launch / S * 4 * phys.engine.nvx * phys.engine.nvz)
+    achieved = f_bytes / f_avg / 1e9
+    kern = "k_forward_fx_nostore"
+    out = {
+        "metric": "wave-cache models/s (70x70, 5 shots, 2001 steps of 0.5 ms, decimated float16 cache + RMS)",
+        "value": round(value, 2),
+        "unit": "models/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * el / args.steps, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (flat/curved/faulted 3-layer models)",
+        "config": {"workload": f"{n} models x {S} shots, 70x70 grid (114x114 padded), nt={nt}, dt=0.5 ms, "
+                               f"batches of {args.cache_batch} models, waves_train.f16.bin + progress/done json",
+                   "models": n, "shots": S, "nt": nt, "batch": args.cache_batch},
+        "roofline": {
+            "bound": "hbm",
+            "kernel": kern,
+            "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5),
+            "traffic": traffic_from_profiles(kern),
+            "algorithmic_bytes_per_launch": f_bytes,
+            "launch_ms": {kern: round(1e3 * f_avg, 3)},
+            "launches_per_step": fn.value / args.steps,
+        },
+        "per_model_loop": {"models": m_loop, "models_per_s": round(m_loop / el_loop, 2) if m_loop else None,
+                           "note": "engine.simulate of one model at a time (data.py:160-163), same engine"},
+        "gpu_forward_share": round(fms.value / 1e3 / el, 3),
+    }
+    valu = valu_from_profiles(kern)
+    if valu:
+        out["roofline"]["valu_issue_frac"] = round(valu * 4 / (1024 * VALU_CLOCK_HZ * f_avg), 4)
+    if not args.no_cpu:
+        from oracle import oracle as O
+
+        threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
+        m_cpu = 24 * threads  # whole models, about 10-15 s of CPU work
+        g = O.Geometry(nt=nt, dt=dt)
+        w = O.ricker_f32(nt, dt, 15.0)
+        src = np.tile(phys.engine.sources(), (m_cpu, 1))
+        vel = np.ascontiguousarray(np.repeat(true[:m_cpu].reshape(m_cpu, -1), S, 0))
+        O.batch_forward_f32(g, src[:threads], vel[:threads], w, threads)  # warm-up
+        t2 = time.perf_counter()
+        traces = O.batch_forward_f32(g, src, vel, w, threads).reshape(m_cpu, S, nt, R)[:, :, ::8, :]
+        sq = 0.0
+        for i in range(m_cpu):
+            sq += float(np.square(traces[i], dtype=np.float64).sum())
+            traces[i].astype(np.float16)
+        cel = time.perf_counter() - t2
+        rate = m_cpu / cel
+        out["cpu_baseline"] = {
+            "value": round(rate, 3),
+            "unit": "models/s",
+            "cores": threads,
+            "kind": "port",
+            "sample": f"{m_cpu} models x {S} shots simulated at nt={nt} ({m_cpu * S} forward solves), decimated, "
+                      f"float16 cast and square-summed, {threads} threads, {cel:.1f} s wall; "
+                      "float32 C restatement of the reference solver",
+        }
+        out["speedup_vs_cpu"] = round(value / rate, 1)
     print(json.dumps(out), flush=True)
     return 0
 
