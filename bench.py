@@ -104,11 +104,13 @@ def main() -> int:
     ap.add_argument("--nt", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--workload", choices=("batch70", "large", "copy", "refine", "uneven", "wavecache"),
+    ap.add_argument("--workload", choices=("batch70", "large", "copy", "refine", "uneven", "wavecache", "physstage"),
                     default="batch70")
     ap.add_argument("--cache-models", type=int, default=2560)
     ap.add_argument("--cache-batch", type=int, default=256)
     ap.add_argument("--cache-loop-models", type=int, default=64)
+    ap.add_argument("--phys-batch", type=int, default=8)
+    ap.add_argument("--ref-steps", type=int, default=2)
     ap.add_argument("--solves", type=int, default=512)
     ap.add_argument("--train-pairs", type=int, default=20000)
     ap.add_argument("--eval-models", type=int, default=256)
@@ -138,6 +140,8 @@ def main() -> int:
         return bench_uneven(args)
     if args.workload == "wavecache":
         return bench_wavecache(args)
+    if args.workload == "physstage":
+        return bench_physstage(args)
 
     import ctypes
 
@@ -608,6 +612,156 @@ def bench_wavecache(args) -> int:
                       "float32 C restatement of the reference solver",
         }
         out["speedup_vs_cpu"] = round(value / rate, 1)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def bench_physstage(args) -> int:
+    """The wave-equation term of the phys fine-tuning stage at the reference's
+    settings (5 shots, 2001 steps of 0.5 ms, phys_batch models, phys_ratio 0.5,
+    bands 6/9/12/full over the timed steps).  A step: re-simulate the observed
+    data of phys_batch true models, misfit and adjoint gradient of the
+    predicted fields, gradient-norm balancing and the backward through a
+    stand-in for the network's output (field = start + 0.2 tanh(w), L1
+    supervised loss).  The reference's per-model host loop on the same engine
+    is timed beside it."""
+    import ctypes
+
+    import torch
+
+    from wgnet_amd import _lib, synthetic
+    from wgnet_amd.physics import WavePhysics
+    from wgnet_amd.physstage import PhysStage, band_at
+
+    nb, S, nt, B = args.phys_batch, 5, 2001, max(32, args.phys_batch)
+    dev = torch.device("cuda")
+    total = args.warmup + args.steps
+    stage = PhysStage(steps=total, bands="6,9,12,full", phys_ratio=0.5, phys_batch=nb)
+    eng = stage.physics.engine
+    _lib.check(eng._lib.wg_set_time_split(eng.handle, args.time_split))
+    start, true = synthetic.batch(B, seed=5000)
+    base = torch.from_numpy(start / 1000.0).to(dev)
+    true_km = torch.from_numpy((true / 1000.0).astype(np.float32)).to(dev)
+    w = torch.zeros(B, 70, 70, device=dev, requires_grad=True)
+    opt = torch.optim.SGD([w], lr=1.0)
+
+    def step(k):
+        field = base + 0.2 * torch.tanh(w)
+        loss = (field - true_km).abs().mean()
+        loss, parts = stage(loss, field, true_km, k)
+        opt.zero_grad(set_to_none=True)
+        loss.backward()
+        opt.step()
+        return loss, parts
+
+    for k in range(1, args.warmup + 1):
+        step(k)
+    torch.cuda.synchronize()
+    lib, h = eng._lib, eng.handle
+    fms, fn, ams, an = ctypes.c_double(), ctypes.c_int64(), ctypes.c_double(), ctypes.c_int64()
+    lib.wg_timing_enable(h, 1)
+    lib.wg_timing_read(h, ctypes.byref(fms), ctypes.byref(fn), ctypes.byref(ams), ctypes.byref(an))
+    t0 = time.perf_counter()
+    bands = []
+    for k in range(args.warmup + 1, total + 1):
+        loss, parts = step(k)
+        bands.append(parts["band_hz"])
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    lib.wg_timing_read(h, ctypes.byref(fms), ctypes.byref(fn), ctypes.byref(ams), ctypes.byref(an))
+    lib.wg_timing_enable(h, 0)
+    assert bool(torch.isfinite(loss))
+
+    # the reference's pattern: per-model host simulate and evaluate_adjoint
+    ref = WavePhysics()
+    t1 = time.perf_counter()
+    for k in range(args.ref_steps):
+        ref.set_band(band_at(k + 1, args.ref_steps, stage.bands))
+        field = (base + 0.2 * torch.tanh(w)).detach()[:nb].cpu().numpy()
+        tn = true_km[:nb].cpu().numpy()
+        obs = [ref.engine.simulate(np.ascontiguousarray(tn[b].reshape(-1) * 1000.0, dtype=np.float32))
+               for b in range(nb)]
+        for b in range(nb):
+            ref.engine.evaluate_adjoint(np.ascontiguousarray(field[b].reshape(-1) * 1000.0, dtype=np.float32),
+                                        np.ascontiguousarray(obs[b].reshape(-1)))
+    el_ref = (time.perf_counter() - t1) / max(1, args.ref_steps)
+
+    value = nb * args.steps / el
+    nx, nz, _ = eng.dims()
+    inner, R = (nx - 4) * (nz - 4), eng.nrec
+    units = nb * S
+    unit_bytes = 4 * inner * nt + 4 * R * nt
+    # forward launches: the observed simulation (no snapshots) and the storing
+    # forward; the adjoint launches read the snapshots and the residual
+    adj_bytes = unit_bytes * units * args.steps / max(an.value, 1)
+    a_avg = ams.value / max(an.value, 1) / 1e3
+    f_avg = fms.value / max(fn.value, 1) / 1e3
+    # the work queues run only when there are more units than compute units
+    cus = torch.cuda.get_device_properties(dev).multi_processor_count
+    ak = "k_queue_adj_fx" if args.time_split > 0 and units > cus else "k_adjoint_fx"
+    achieved = adj_bytes / a_avg / 1e9
+    out = {
+        "metric": "phys-stage wave term models/s (5 shots, 2001 steps of 0.5 ms, observed re-simulated, balanced "
+                  "adjoint gradient)",
+        "value": round(value, 2),
+        "unit": "models/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * el / args.steps, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (flat/curved/faulted 3-layer models, Gaussian-anomaly truth); stand-in network output",
+        "config": {"workload": f"phys_batch {nb} models x {S} shots per training step, 70x70 (114x114 padded), "
+                               f"nt={nt}, dt=0.5 ms, phys_ratio 0.5, bands 6/9/12/full",
+                   "phys_batch": nb, "shots": S, "nt": nt, "bands_timed": sorted(set(bands)),
+                   "time_split": args.time_split},
+        "roofline": {
+            "bound": "hbm",
+            "kernel": ak,
+            "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5),
+            "traffic": None,
+            "algorithmic_bytes_per_launch": int(adj_bytes),
+            "launch_ms": {"k_forward_fx (observed and storing, side by side), mean": round(1e3 * f_avg, 3),
+                          ak: round(1e3 * a_avg, 3)},
+            "launches_per_step": {"forward": fn.value / args.steps, "adjoint": an.value / args.steps},
+        },
+        "per_model_loop": {"ms_per_step": round(1e3 * el_ref, 2), "steps": args.ref_steps,
+                           "note": "physics.py:59-76's per-model engine.simulate and evaluate_adjoint through host "
+                                   "arrays, same engine"},
+        "speedup_vs_per_model_loop": round(el_ref / (el / args.steps), 2),
+    }
+    if not args.no_cpu:
+        from oracle import oracle as O
+
+        threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
+        g = O.Geometry(nt=nt, dt=0.0005)
+        wav = eng.wavelet()
+        src = np.tile(eng.sources(), (nb, 1))
+        tv = np.ascontiguousarray(np.repeat(true[:nb].reshape(nb, -1), S, 0))
+        cv = np.ascontiguousarray(np.repeat(start[:nb].reshape(nb, -1), S, 0))
+        reps = 0
+        t2 = time.perf_counter()
+        while reps < 20 and (reps == 0 or time.perf_counter() - t2 < 10.0):  # about 10 s of CPU work
+            obs = O.batch_forward_f32(g, src, tv, wav, threads)
+            O.batch_fwd_adj_f32(g, src, cv, wav, obs, threads)
+            reps += 1
+        cel = time.perf_counter() - t2
+        out["cpu_baseline"] = {
+            "value": round(nb * reps / cel, 3),
+            "unit": "models/s",
+            "cores": threads,
+            "kind": "port",
+            "sample": f"{reps} steps of {nb} models x {S} shots, observed forward plus forward+adjoint at nt={nt} "
+                      f"({2 * nb * S} solves a step), {threads} threads, {cel:.1f} s wall; float32 C restatement of "
+                      "the reference solver",
+        }
+        out["speedup_vs_cpu"] = round(value / (nb * reps / cel), 1)
     print(json.dumps(out), flush=True)
     return 0
 

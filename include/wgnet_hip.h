@@ -107,6 +107,19 @@ int32_t wg_loss_grad_dev(wg_engine *e, int32_t B, const float *vel_dev,
                          float *grad_dev, float *traces_dev,
                          double *energy_dev, void *stream);
 
+/* The phys stage's two calls in one (train.py:262-263: physics.simulate of
+ * the true models, physics.py:59-64, then physics.loss_and_grad of the
+ * predicted ones, physics.py:66-76): observed gathers of true_dev
+ * [B, nvz, nvx] m/s into obs_dev [B, shots, nt, nrec], then wg_loss_grad_dev
+ * of vel_dev against them.  When the batch is one resident round of units
+ * the observed forward runs beside the storing forward; results equal
+ * wg_simulate_dev followed by wg_loss_grad_dev bit for bit. */
+int32_t wg_loss_grad_true_dev(wg_engine *e, int32_t B, const float *vel_dev,
+                              const float *true_dev, float *obs_dev,
+                              float *loss_dev, float *grad_dev,
+                              float *traces_dev, double *energy_dev,
+                              void *stream);
+
 /* wg_loss_grad_dev with per-(model, shot) wavelets wav_dev [B, shots, nt]
  * in place of the engine wavelet, so that models of one batch can sit in
  * different passbands (the reference changes the engine wavelet per band,
@@ -206,7 +219,8 @@ int32_t wg_set_time_block(wg_engine *e, int32_t steps);
  * a device-side queue by one workgroup per compute unit; a segment restores
  * its unit's state from HBM.  Results are identical to whole-unit launches;
  * 0 turns the queues off (the chunk pipeline runs instead).  The queues also
- * stay off when the snapshots of the whole batch exceed the snapshot budget.
+ * stay off when a batch has no more units than the device has compute units
+ * (nothing to balance) or its snapshots exceed the snapshot budget.
  * Timing (below) counts the forward and adjoint queue launches. */
 int32_t wg_set_time_split(wg_engine *e, int32_t parts);
 

@@ -541,15 +541,17 @@ int stream_simulate(wg_engine* e, int b0, int nb, const float* vel, const float*
 // grid kernels): every unit's forward with snapshots, then the residual and
 // loss partials, then every unit's adjoint, the forward and the adjoint each
 // cut into e->qparts time parts.  Sets *used = false, doing nothing, when the
-// queues do not apply (another grid, disabled, or the snapshots and part
-// states of all units over the snapshot budget).
+// queues do not apply (another grid, disabled, no more units than CUs, or the
+// snapshots and part states of all units over the snapshot budget).
 int queue_loss_grad(wg_engine* e, int B, const float* vel, const float* obs, float* traces, const float* scale,
                     hipStream_t st, bool* used) {
     *used = false;
     const wg::Grid& g = e->g;
     const int S = e->cfg.shots, U = B * S, parts = e->qparts;
     size_t ff = 0, af = 0;
-    if (parts <= 0 || U <= 0 || !wg::queue_state_fx(g, &ff, &af)) return WG_OK;
+    // With at most one unit per CU there is nothing to balance: every unit's
+    // parts would run one after another, so the whole-unit kernels are faster.
+    if (parts <= 0 || U <= e->cus || !wg::queue_state_fx(g, &ff, &af)) return WG_OK;
     const size_t per_unit = snap_bytes_per_unit(e) + (ff + af) * sizeof(float);
     if (per_unit * (size_t)U > e->snap_budget) return WG_OK;
     const long long per = (long long)S * g.nt * g.nrec;
@@ -1010,6 +1012,62 @@ static int32_t loss_grad_entry(wg_engine* e, int32_t B, const float* vel_dev, co
 int32_t wg_loss_grad_dev(wg_engine* e, int32_t B, const float* vel_dev, const float* obs_dev, float* loss_dev,
                          float* grad_dev, float* traces_dev, double* energy_dev, void* stream) {
     return loss_grad_entry(e, B, vel_dev, nullptr, obs_dev, loss_dev, grad_dev, traces_dev, energy_dev, stream);
+}
+
+int32_t wg_loss_grad_true_dev(wg_engine* e, int32_t B, const float* vel_dev, const float* true_dev, float* obs_dev,
+                              float* loss_dev, float* grad_dev, float* traces_dev, double* energy_dev, void* stream) {
+    if (!e || !vel_dev || !true_dev || !obs_dev || !loss_dev || !grad_dev || B < 0)
+        return fail(WG_E_INVALID, "invalid argument");
+    Guard G(e);
+    HIPCK(hipSetDevice(e->device));
+    hipStream_t st = (hipStream_t)stream;
+    int rc;
+    if ((rc = fence(e, st))) return rc;
+    if (B == 0) return WG_OK;
+    if ((rc = ensure(e, e->scale, (size_t)B * sizeof(float)))) return rc;
+    double* energy = energy_dev;
+    if (!energy) {
+        if ((rc = ensure(e, e->energy, (size_t)B * sizeof(double)))) return rc;
+        energy = (double*)e->energy.p;
+    }
+    const wg::Grid& g = e->g;
+    const int S = e->cfg.shots;
+    // One resident round of units, snapshots within budget: the observed
+    // forward (no snapshots) runs on the side stream beside the storing
+    // forward, and the residual waits for both.  Per-unit arithmetic is that
+    // of wg_simulate_dev followed by wg_loss_grad_dev.
+    const bool beside = !e->streamed && B * S <= e->cus && models_per_chunk(e, B) >= B;
+    if (!beside) {
+        if ((rc = run_chunk_forward(e, 0, B, true_dev, obs_dev, false, st))) return rc;
+        return loss_grad_core(e, B, vel_dev, obs_dev, loss_dev, grad_dev, traces_dev, energy, (float*)e->scale.p,
+                              true, st);
+    }
+    const long long per = (long long)S * g.nt * g.nrec;
+    const long long chunks = (per + 255) / 256;
+    if ((rc = ensure(e, e->resid, (size_t)B * per * sizeof(float)))) return rc;
+    if ((rc = ensure(e, e->partials, (size_t)B * chunks * sizeof(float)))) return rc;
+    if ((rc = ensure(e, e->gunit, (size_t)B * S * g.nx * g.nz * sizeof(float)))) return rc;
+    if ((rc = ensure(e, e->snaps, snap_bytes_per_unit(e) * S * B))) return rc;
+    float* traces = traces_dev;
+    if (!traces) {
+        if ((rc = ensure(e, e->traces, (size_t)B * per * sizeof(float)))) return rc;
+        traces = (float*)e->traces.p;
+    }
+    if ((rc = ensure_pipeline(e))) return rc;
+    HIPCK(hipEventRecord(e->ev_pipe[0], st));
+    HIPCK(hipStreamWaitEvent(e->side, e->ev_pipe[0], 0));
+    if ((rc = run_chunk_forward(e, 0, B, true_dev, obs_dev, false, e->side))) return rc;
+    HIPCK(hipEventRecord(e->ev_pipe[1], e->side));
+    if ((rc = run_chunk_forward(e, 0, B, vel_dev, traces, true, st))) return rc;
+    HIPCK(hipStreamWaitEvent(st, e->ev_pipe[1], 0));
+    float* scale = (float*)e->scale.p;
+    LAUNCH(wg::launch_energy(obs_dev, per, B, energy, scale, st));
+    LAUNCH(wg::launch_residual(traces, obs_dev, scale, per, B, (float*)e->resid.p, (float*)e->partials.p, st));
+    if ((rc = run_chunk_adjoint(e, 0, B, vel_dev, (const float*)e->resid.p, (float*)e->gunit.p, nullptr, st)))
+        return rc;
+    LAUNCH(wg::launch_loss_sum((const float*)e->partials.p, chunks, B, energy, loss_dev, st));
+    LAUNCH(wg::launch_fold(g, (const float*)e->gunit.p, vel_dev, energy, B, grad_dev, st));
+    return WG_OK;
 }
 
 int32_t wg_loss_grad_wavelets_dev(wg_engine* e, int32_t B, const float* vel_dev, const float* wav_dev,

@@ -222,6 +222,28 @@ class WavePhysics:
             raise ValueError("Observed traces have zero energy")
         return losses, grads * 1000.0  # d/d(km/s) = 1000 d/d(m/s)
 
+    def loss_and_grad_true_torch(self, fields_km: torch.Tensor, true_km: torch.Tensor
+                                 ) -> tuple[torch.Tensor, torch.Tensor, torch.Tensor]:
+        """simulate_torch(true_km) then loss_and_grad_torch(fields_km, observed)
+        in one engine call (train.py:262-263); returns (losses, grads w.r.t.
+        km/s, observed [B, shots, nt, nrec]).  The observed forward runs beside
+        the predicted one when the batch fits one round of the device."""
+        vel = (_device_f32(fields_km, self.device) * 1000.0).contiguous()
+        tvel = (_device_f32(true_km, self.device) * 1000.0).contiguous()
+        B = vel.shape[0]
+        if tuple(tvel.shape) != tuple(vel.shape):
+            raise ValueError("true and predicted fields must have the same shape")
+        obs = torch.empty((B, self.shots, self.nt, self.engine.nrec), dtype=torch.float32, device=self.device)
+        losses = torch.empty(B, dtype=torch.float32, device=self.device)
+        grads = torch.empty_like(vel)
+        energy = torch.empty(B, dtype=torch.float64, device=self.device)
+        lib, h = self.engine._lib, self.engine.handle
+        _lib.check(lib.wg_loss_grad_true_dev(h, B, _dev_ptr(vel), _dev_ptr(tvel), _dev_ptr(obs), _dev_ptr(losses),
+                                             _dev_ptr(grads), None, _dev_ptr(energy), _stream()))
+        if bool((energy <= 0).any()):
+            raise ValueError("Observed traces have zero energy")
+        return losses, grads * 1000.0, obs
+
     def loss_and_grad(self, fields_km, observed):
         """physics.py:66-76: numpy in -> (losses [B] f32, grads [B, 70, 70] f32)."""
         if isinstance(fields_km, torch.Tensor):
