@@ -730,7 +730,7 @@ def bench_physstage(args) -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
-            "traffic": None,
+            "traffic": traffic_from_profiles("k_adjoint_fx_physstage") if ak == "k_adjoint_fx" and nb == 8 else None,
             "algorithmic_bytes_per_launch": int(adj_bytes),
             "launch_ms": {"k_forward_fx (observed and storing, side by side), mean": round(1e3 * f_avg, 3),
                           ak: round(1e3 * a_avg, 3)},
