@@ -805,8 +805,19 @@ __device__ __forceinline__ void adjoint_run(const AdjParams& P, float* lds, int 
     rs[3] = pZB.slot_in_rows(C.bz0, C.bz1);
     rs[4] = pCT.slot_in_rows(C.bz0, C.bz1);
     rs[5] = pCB.slot_in_rows(C.bz0, C.bz1);
+    // The residual row and the wavelet sample of each step are loaded one
+    // step ahead (at most NT receivers: one per thread), so their latency
+    // overlaps a whole step instead of stalling its start.
+    const bool has_rec = tid < g.nrec;
+    float res_next = has_rec ? resid[(size_t)(t_hi - 1) * g.nrec + tid] : 0.0f;
+    float f_next = wav[t_hi - 1];
     for (int t = t_hi - 1; t >= t_lo; --t) {
-        for (int rr = tid; rr < g.nrec; rr += NT) RES[rr] = resid[(size_t)t * g.nrec + rr];
+        if (has_rec) RES[tid] = res_next;
+        const float f_t = f_next;
+        if (t > t_lo) {
+            if (has_rec) res_next = resid[(size_t)(t - 1) * g.nrec + tid];
+            f_next = wav[t - 1];
+        }
         a_phase1<G>(sI, pI, C, cx, so[0]);
         a_phase1<G>(sX, pX, C, cx, so[1]);
         a_phase1<G>(sZT, pZT, C, cx, so[2]);
@@ -820,7 +831,6 @@ __device__ __forceinline__ void adjoint_run(const AdjParams& P, float* lds, int 
             for (int c = 0; c < 4; ++c) acc += SRC[4 + c] * SRC[c];
             P.dwav[(size_t)unit * g.nt + t] = acc;
         }
-        const float f_t = wav[t];
         const float* snap_t = snaps + (size_t)t * G::INNER;
         a_phase2<G>(sI, pI, C, cx, so[0], rs[0], snap_t, f_t);
         a_phase2<G>(sX, pX, C, cx, so[1], rs[1], snap_t, f_t);
