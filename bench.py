@@ -784,7 +784,7 @@ def bench_fwi(args) -> int:
     import torch
 
     from wgnet_amd import synthetic
-    from wgnet_amd.fwi_eval import fwi, fwi_batch, macro_start
+    from wgnet_amd.fwi_eval import _default_workers, fwi, fwi_batch, macro_start
     from wgnet_amd.physics import WavePhysics
 
     M, S, nt, bands = args.fwi_models, 5, 2001, [5.0, 8.0, 12.0, None]
@@ -837,7 +837,8 @@ def bench_fwi(args) -> int:
         "data": "synthetic (flat/curved/faulted 3-layer models, Gaussian-anomaly truth)",
         "config": {"workload": f"{P} FWI problems ({M} models x linear-1D and macro starts), budget "
                                f"{args.fwi_budget} solves over 4 bands, lockstep batches",
-                   "problems": P, "budget": args.fwi_budget, "bands": "5,8,12,full", "smooth_grad": 1.5},
+                   "problems": P, "budget": args.fwi_budget, "bands": "5,8,12,full", "smooth_grad": 1.5,
+                   "host_workers": _default_workers(P)},
         "solves": {"total": int(sum(solves)), "per_problem_mean": round(float(np.mean(solves)), 1),
                    "batched_calls": calls, "solves_per_s": round(sum(solves) / el, 1)},
         "mae_mps": {"linear1d": [round(float(mae0[:M].mean()), 1), round(float(mae1[:M].mean()), 1)],

@@ -73,11 +73,13 @@ def _problems(P, seed=0):
 
 
 @pytest.mark.timeout(120)
-def test_lockstep_batch_equals_one_problem_at_a_time():
+@pytest.mark.parametrize("workers", [0, 2, None])
+def test_lockstep_batch_equals_one_problem_at_a_time(workers):
+    """In-process threads (0), two worker processes, and the default."""
     bands, budget = [5.0, 12.0, None], 18
     starts, observed = _problems(4)
     phys = _StandIn()
-    finals, hists, calls = fwi_eval.fwi_batch(starts, observed, phys, bands, budget)
+    finals, hists, calls = fwi_eval.fwi_batch(starts, observed, phys, bands, budget, workers=workers)
     solves = []
     for p in range(4):
         f1, h1 = fwi_eval.fwi(starts[p], observed[p], _StandIn(), bands, budget)
@@ -92,7 +94,8 @@ def test_lockstep_batch_equals_one_problem_at_a_time():
 
 
 @pytest.mark.timeout(120)
-def test_problems_finishing_early_leave_the_batch():
+@pytest.mark.parametrize("workers", [0, 2])
+def test_problems_finishing_early_leave_the_batch(workers):
     """Problems whose start is already the answer stop after a few solves;
     the others keep running in smaller batches."""
     bands, budget = [None], 30
@@ -101,7 +104,7 @@ def test_problems_finishing_early_leave_the_batch():
     truth = (2.0 + 2.0 * rng.random((3, N, N))).astype(np.float32)  # as _problems(3, seed=1) draws it
     starts[1] = truth[1]  # converged at once
     phys = _StandIn()
-    finals, hists, calls = fwi_eval.fwi_batch(starts, observed, phys, bands, budget)
+    finals, hists, calls = fwi_eval.fwi_batch(starts, observed, phys, bands, budget, workers=workers)
     s = [h[0]["solves"] for h in hists]
     assert s[1] < min(s[0], s[2])
     assert calls == max(s)
@@ -111,10 +114,11 @@ def test_problems_finishing_early_leave_the_batch():
 
 
 @pytest.mark.timeout(60)
-def test_failed_batched_call_reaches_every_problem():
+@pytest.mark.parametrize("workers", [0, 2])
+def test_failed_batched_call_reaches_every_problem(workers):
     starts, observed = _problems(3)
     with pytest.raises(RuntimeError):
-        fwi_eval.fwi_batch(starts, observed, _StandIn(fail_after=2), [None], 10)
+        fwi_eval.fwi_batch(starts, observed, _StandIn(fail_after=2), [None], 10, workers=workers)
 
 
 def test_fwi_restores_full_band():
@@ -141,3 +145,12 @@ def test_macro_start_is_mirror_padded_gaussian(sigma):
     assert np.abs(m - ref).max() <= 1e-5
     assert np.abs(fwi_eval.macro_start(np.full((1, N, N), 2.5, np.float32), sigma) - 2.5).max() <= 1e-6
     assert np.array_equal(fwi_eval.macro_start(v, 0.0), v)
+
+
+@pytest.mark.timeout(60)
+def test_worker_failure_is_reported():
+    """An exception inside a worker (here: an empty band list) comes back as
+    a RuntimeError carrying the worker's traceback, and no worker is left."""
+    starts, observed = _problems(2)
+    with pytest.raises(RuntimeError, match="ZeroDivisionError"):
+        fwi_eval.fwi_batch(starts, observed, _StandIn(), [], 6, workers=2)

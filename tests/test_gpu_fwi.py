@@ -3,7 +3,8 @@
 gradient smoothing 1.5 cells; bounds [1.4, 4.6] km/s), with a small budget:
 - `fwi` equals the reference's loop restated over the engine's one-model
   host calls (physics.py:66-76's evaluate_adjoint loop), bit for bit;
-- `fwi_batch` gives each problem exactly what `fwi` gives it alone;
+- `fwi_batch` gives each problem exactly what `fwi` gives it alone, with the
+  optimiser side in worker processes and in threads of this process;
 - the first objective value and gradient (start model, 5 Hz) equal the CPU
   oracle's on the same band-limited data;
 - FWI lowers the full-band misfit."""
@@ -93,6 +94,8 @@ def test_fwi_equals_reference_loop_and_batch_equals_fwi():
     for p in (1, 2):
         f_p, h_p = fwi(starts[p], observed[p], phys, BANDS, budget)
         assert finals[p].tobytes() == f_p.tobytes() and hists[p] == h_p
+    f0, h0, _ = fwi_batch(starts, observed, phys, BANDS, budget, workers=0)  # threads of this process
+    assert all(f0[p].tobytes() == finals[p].tobytes() and h0[p] == hists[p] for p in range(3))
     for p in range(3):
         assert misfit_full(finals[p], observed[p], phys) < misfit_full(starts[p], observed[p], phys)
         assert finals[p].min() >= 1.4 and finals[p].max() <= 4.6

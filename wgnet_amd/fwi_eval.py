@@ -12,49 +12,28 @@ the physics' causal low-pass.
 lockstep: each problem's L-BFGS-B runs unchanged in its own thread, and
 whenever every running problem waits for an objective value, their models are
 evaluated in one batched engine call, each with its own passband wavelet
-(problems move through the bands at their own pace).  Per-model results of
-the batched call equal one-model calls bit for bit, so each problem follows
-exactly the trajectory that `fwi` gives it alone.
+(problems move through the bands at their own pace).  The optimiser side is
+host work that the interpreter lock serialises, so by default the problems
+are shared among worker processes (`_fwi_host`, numpy and scipy only), and
+this process, which owns the GPU, evaluates every round's requests from all
+of them in one call.  Per-model results of the batched call equal one-model
+calls bit for bit, so each problem follows exactly the trajectory that `fwi`
+gives it alone.
 """
 
 from __future__ import annotations
 
 import math
-import threading
+import os
+import subprocess
+import sys
 
 import numpy as np
 import torch
-from scipy.ndimage import gaussian_filter
-from scipy.optimize import minimize
 
-from .physics import N
+from ._fwi_host import N, V_BOUNDS_KM, lbfgsb_bands, recv, run_lockstep, send
 
-V_BOUNDS_KM = (1.4, 4.6)  # fwi_eval.py:29
-
-
-def _lbfgsb_bands(start_km, observed_full, bands, budget, smooth_grad, band_observed, evaluate):
-    """The band loop of fwi_eval.py:37-57.  band_observed(band) -> a handle of
-    the band-limited observed data; evaluate(field_f32 [N, N], band, handle)
-    -> (loss, grad [N, N] float32)."""
-    x = start_km.astype(np.float64).reshape(-1)
-    per_band = max(2, budget // len(bands))
-    history = []
-    for band in bands:
-        observed = band_observed(band)
-        calls = 0
-
-        def objective(vec: np.ndarray):
-            nonlocal calls
-            calls += 1
-            loss, grad = evaluate(vec.reshape(N, N).astype(np.float32), band, observed)
-            g = gaussian_filter(grad, smooth_grad, mode="nearest") if smooth_grad > 0 else grad
-            return float(loss), g.reshape(-1).astype(np.float64)
-
-        res = minimize(objective, x, jac=True, method="L-BFGS-B", bounds=[V_BOUNDS_KM] * (N * N),
-                       options={"maxfun": per_band, "maxiter": per_band, "maxcor": 10})
-        x = res.x
-        history.append({"band_hz": band, "misfit": float(res.fun), "solves": calls})
-    return x.reshape(N, N).astype(np.float32), history
+MAX_WORKERS = 16
 
 
 def fwi(start_km: np.ndarray, observed_full: np.ndarray, physics, bands: list, budget: int,
@@ -70,7 +49,7 @@ def fwi(start_km: np.ndarray, observed_full: np.ndarray, physics, bands: list, b
         losses, grads = physics.loss_and_grad(field[None], observed[None])
         return losses[0], grads[0]
 
-    out = _lbfgsb_bands(start_km, observed_full, bands, budget, smooth_grad, band_observed, evaluate)
+    out = lbfgsb_bands(start_km, bands, budget, smooth_grad, band_observed, evaluate)
     physics.set_band(None)
     return out
 
@@ -82,19 +61,17 @@ def misfit_full(model_km: np.ndarray, observed_full: np.ndarray, physics) -> flo
     return float(losses[0])
 
 
-class _Lockstep:
-    """Collects one objective request per running problem and answers them
-    with one batched loss-and-gradient call."""
+class _Evaluator:
+    """One batched loss-and-gradient call for the requests of a round, each
+    model in its own passband against its own band-limited observed data
+    (physics.bandlimit's CPU filter, as `fwi` applies it, kept on the device
+    while the problem stays in that band)."""
 
-    def __init__(self, physics, problems: int):
-        self.physics = physics
-        self.cond = threading.Condition()
-        self.active = problems
-        self.pending: dict[int, tuple] = {}
-        self.results: dict[int, tuple] = {}
+    def __init__(self, physics, full: np.ndarray):
+        self.physics, self.full = physics, full
         self.wavelets: dict = {}
+        self.observed: dict = {}  # pid -> (band, device gathers)
         self.calls = 0
-        self.error: BaseException | None = None
 
     def wavelet(self, band):
         if band not in self.wavelets:
@@ -102,81 +79,104 @@ class _Lockstep:
             self.wavelets[band] = torch.from_numpy(np.ascontiguousarray(w, np.float32)).to(self.physics.device)
         return self.wavelets[band]
 
-    def _run_batch(self) -> None:
-        ids = sorted(self.pending)
-        try:
-            fields = torch.from_numpy(np.stack([self.pending[i][0] for i in ids])).to(self.physics.device)
-            obs = torch.stack([self.pending[i][2] for i in ids])
-            wav = torch.stack([self.wavelet(self.pending[i][1]) for i in ids])
-            losses, grads = self.physics.loss_and_grad_torch(fields, obs, wavelets=wav)
-            losses, grads = losses.cpu().numpy(), grads.cpu().numpy()
-            for k, i in enumerate(ids):
-                self.results[i] = (losses[k], grads[k])
-        except BaseException as exc:  # every waiting problem sees the failure
-            self.error = exc
+    def band_observed(self, pid: int, band):
+        cur = self.observed.get(pid)
+        if cur is None or cur[0] != band:
+            d = np.ascontiguousarray(self.physics.bandlimit_to(self.full[pid], band), np.float32)
+            cur = (band, torch.from_numpy(d).to(self.physics.device))
+            self.observed[pid] = cur
+        return cur[1]
+
+    def __call__(self, requests: list) -> dict:
+        """requests: [(pid, band, field [70, 70] float32)] -> {pid: (loss, grad)}."""
+        dev = self.physics.device
+        fields = torch.from_numpy(np.stack([f for _, _, f in requests])).to(dev)
+        obs = torch.stack([self.band_observed(pid, band) for pid, band, _ in requests])
+        wav = torch.stack([self.wavelet(band) for _, band, _ in requests])
+        losses, grads = self.physics.loss_and_grad_torch(fields, obs, wavelets=wav)
+        losses, grads = losses.cpu().numpy(), grads.cpu().numpy()
         self.calls += 1
-        self.pending.clear()
-        self.cond.notify_all()
+        return {pid: (losses[k], grads[k]) for k, (pid, _, _) in enumerate(requests)}
 
-    def submit(self, pid: int, field: np.ndarray, band, observed: torch.Tensor):
-        with self.cond:
-            if self.error is not None:
-                raise RuntimeError("a batched evaluation failed") from self.error
-            self.pending[pid] = (field, band, observed)
-            if len(self.pending) == self.active:
-                self._run_batch()
-            while pid not in self.results and self.error is None:
-                self.cond.wait()
-            if pid not in self.results:
-                raise RuntimeError("a batched evaluation failed") from self.error
-            return self.results.pop(pid)
 
-    def finish(self) -> None:
-        with self.cond:
-            self.active -= 1
-            if self.pending and len(self.pending) == self.active:
-                self._run_batch()
+def _default_workers(P: int) -> int:
+    try:
+        cpus = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cpus = os.cpu_count() or 1
+    return max(1, min(P, MAX_WORKERS, cpus))
+
+
+def _run_workers(starts, bands, budget, smooth_grad, ev: _Evaluator, W: int):
+    """Problems shared among W worker processes; one batched call per round
+    for the requests of all of them."""
+    P = starts.shape[0]
+    shares = [s for s in np.array_split(np.arange(P), W) if len(s)]
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    env = dict(os.environ)
+    env["PYTHONPATH"] = root + (os.pathsep + env["PYTHONPATH"] if env.get("PYTHONPATH") else "")
+    for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS"):
+        env[k] = "1"
+    procs = [subprocess.Popen([sys.executable, "-m", "wgnet_amd._fwi_host"], stdin=subprocess.PIPE,
+                              stdout=subprocess.PIPE, env=env, cwd=root) for _ in shares]
+    results: dict = {}
+    try:
+        for pr, share in zip(procs, shares):
+            send(pr.stdin, {"pids": [int(p) for p in share], "starts": [starts[p] for p in share],
+                            "bands": list(bands), "budget": int(budget), "smooth_grad": float(smooth_grad)})
+        live = list(range(len(procs)))
+        while live:
+            waiting, requests = [], []
+            for w in live:
+                kind, body = recv(procs[w].stdout)
+                if kind == "eval":
+                    waiting.append(w)
+                    requests.append(body)
+                elif kind == "done":
+                    results.update(body)
+                else:
+                    raise RuntimeError(f"FWI worker failed:\n{body}")
+            live = waiting
+            if requests:
+                answers = ev([r for req in requests for r in req])
+                for w, req in zip(waiting, requests):
+                    send(procs[w].stdin, {pid: answers[pid] for pid, _, _ in req})
+    finally:
+        for pr in procs:
+            try:
+                pr.stdin.close()
+            except OSError:
+                pass
+        for pr in procs:
+            try:
+                pr.wait(timeout=10)
+            except subprocess.TimeoutExpired:
+                pr.kill()
+                pr.wait()
+    return results
 
 
 def fwi_batch(starts_km: np.ndarray, observed_full: np.ndarray, physics, bands: list, budget: int,
-              smooth_grad: float = 1.5) -> tuple[list[np.ndarray], list[list[dict]], int]:
+              smooth_grad: float = 1.5, workers: int | None = None) -> tuple[list[np.ndarray], list[list[dict]], int]:
     """`fwi` for P problems at once: starts_km [P, 70, 70] km/s, observed_full
-    [P, shots, nt, 70] full-band gathers.  Returns the P final models, their
-    histories (each equal to what `fwi` returns for that problem alone) and
-    the number of batched engine calls made."""
+    [P, shots, nt, 70] full-band gathers.  `workers` worker processes run the
+    optimiser side (default: one per host core, at most 16 and at most P;
+    0 runs every problem as a thread of this process).  Returns the P final
+    models, their histories (each equal to what `fwi` returns for that
+    problem alone) and the number of batched engine calls made."""
     starts = np.asarray(starts_km, np.float32)
     full = np.asarray(observed_full, np.float32)
     P = starts.shape[0]
     if full.shape[0] != P:
         raise ValueError("one observed gather set per start")
-    hub = _Lockstep(physics, P)
-    finals: list = [None] * P
-    histories: list = [None] * P
-    errors: list = [None] * P
-
-    def run(p: int) -> None:
-        def band_observed(band):
-            # physics.bandlimit's CPU filter, as `fwi` applies it, then on the device
-            return torch.from_numpy(np.ascontiguousarray(physics.bandlimit_to(full[p], band),
-                                                         np.float32)).to(physics.device)
-
-        try:
-            finals[p], histories[p] = _lbfgsb_bands(starts[p], full[p], bands, budget, smooth_grad, band_observed,
-                                                    lambda f, band, obs: hub.submit(p, f, band, obs))
-        except BaseException as exc:
-            errors[p] = exc
-        finally:
-            hub.finish()
-
-    threads = [threading.Thread(target=run, args=(p,), daemon=True) for p in range(P)]
-    for t in threads:
-        t.start()
-    for t in threads:
-        t.join()
-    for e in errors:
-        if e is not None:
-            raise e
-    return finals, histories, hub.calls
+    ev = _Evaluator(physics, full)
+    W = _default_workers(P) if workers is None else int(workers)
+    if W <= 0:
+        res, _ = run_lockstep(list(range(P)), list(starts), bands, budget, smooth_grad, ev.band_observed,
+                              lambda pending: ev([(pid, band, f) for pid, (f, band, _) in sorted(pending.items())]))
+    else:
+        res = _run_workers(starts, bands, budget, smooth_grad, ev, W)
+    return [res[p][0] for p in range(P)], [res[p][1] for p in range(P)], ev.calls
 
 
 def macro_start(v_km: np.ndarray, sigma_cells: float) -> np.ndarray:
