@@ -96,63 +96,106 @@ __global__ __launch_bounds__(NT) void k_fwd_step(StepParams P) {
             tr[r] = acc;
         }
     }
-    // own cells' u^{t-1} and velocity, loaded before the staging so that
-    // their latency overlaps it
-    const int lx = tid % TX;
-    float pu[CPT], pv[CPT];
-#pragma unroll
-    for (int k = 0; k < CPT; ++k) {
-        const int x = xi0 + lx + 2, z = zi0 + tid / TX + RZ * k + 2;
-        pu[k] = pv[k] = 0.0f;
-        if (x < g.nx - 2 && z < g.nz - 2) {
-            pu[k] = Unxt[z * g.nx + x];
-            pv[k] = vis_raw(vvis, g, x, z);
-        }
-    }
-    // Stage u (tile + 2-cell halo), a*psi_x (x halo) and a*psi_z (z halo).
-    for (int j = tid; j < HZ * HX; j += NT) {
-        const int x = xi0 + (j % HX), z = zi0 + (j / HX);  // padded coords: halo starts at inner - 2
-        sU[j] = (x < g.nx && z < g.nz) ? Ucur[z * g.nx + x] : 0.0f;
-    }
     // does the tile hold C-PML cells along x / z (padded coords [xi0+2, xi0+TX+2))?
     const bool tpx = xi0 + 2 < g.region || xi0 + TX + 2 > g.nx - g.region;
     const bool tpz = zi0 + 2 < g.region || zi0 + TZ + 2 > g.nz - g.region;
-    if (tpx)
-        for (int j = tid; j < TZ * HX; j += NT) {
+    // Everything a thread reads from HBM for its own cells is loaded before
+    // the staging so that its latency overlaps it: u^{t-1}, the velocity,
+    // the C-PML memory variables and profile coefficients, the unit's
+    // source corners.
+    const int lx = tid % TX;
+    const int xo = xi0 + lx + 2;  // the thread's column (padded)
+    const bool pxo = tpx && (xo < g.region || xo >= g.nx - g.region);
+    Pml mx{0, 0, 0, 0};
+    if (pxo && xo < g.nx - 2) {
+        const float bb = pbx[xo];
+        mx = Pml{pax[xo], 1.0f + bb, bb, pdbx[xo]};
+    }
+    int scell[4];
+    float sw[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        scell[c] = P.geo.src_cell[s * 4 + c];
+        sw[c] = P.geo.src_w[s * 4 + c];
+    }
+    float pu[CPT], pv[CPT], ppx[CPT], pzx[CPT], ppz[CPT], pzz[CPT];
+    Pml pmz[CPT];
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+        const int z = zi0 + tid / TX + RZ * k + 2;
+        pu[k] = pv[k] = ppx[k] = pzx[k] = ppz[k] = pzz[k] = 0.0f;
+        pmz[k] = Pml{0, 0, 0, 0};
+        if (xo < g.nx - 2 && z < g.nz - 2) {
+            const int i = z * g.nx + xo;
+            pu[k] = Unxt[i];
+            pv[k] = vis_raw(vvis, g, xo, z);
+            if (pxo) {
+                ppx[k] = PXc[i];
+                pzx[k] = ZX[i];
+            }
+            if (tpz && (z < g.region || z >= g.nz - g.region)) {
+                const float bb = pbz[z];
+                pmz[k] = Pml{paz[z], 1.0f + bb, bb, pdbz[z]};
+                ppz[k] = PZc[i];
+                pzz[k] = ZZ[i];
+            }
+        }
+    }
+    // Stage u (tile + 2-cell halo), a*psi_x (x halo) and a*psi_z (z halo),
+    // each thread's loads issued together before its LDS writes.
+    {
+        constexpr int NS = (HZ * HX + NT - 1) / NT;
+        float tu[NS];
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            const int j = tid + k * NT;
+            const int x = xi0 + (j % HX), z = zi0 + (j / HX);  // padded coords: halo starts at inner - 2
+            tu[k] = (j < HZ * HX && x < g.nx && z < g.nz) ? Ucur[z * g.nx + x] : 0.0f;
+        }
+#pragma unroll
+        for (int k = 0; k < NS; ++k)
+            if (tid + k * NT < HZ * HX) sU[tid + k * NT] = tu[k];
+    }
+    if (tpx) {
+        constexpr int NS = (TZ * HX + NT - 1) / NT;
+        float ta[NS];
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            const int j = tid + k * NT;
             const int x = xi0 + (j % HX), z = zi0 + 2 + (j / HX);
-            sAX[j] = (x < g.nx && z < g.nz) ? pax[x] * PXc[z * g.nx + x] : 0.0f;
+            ta[k] = (j < TZ * HX && x < g.nx && z < g.nz) ? pax[x] * PXc[z * g.nx + x] : 0.0f;
         }
-    if (tpz)
-        for (int j = tid; j < HZ * TX; j += NT) {
+#pragma unroll
+        for (int k = 0; k < NS; ++k)
+            if (tid + k * NT < TZ * HX) sAX[tid + k * NT] = ta[k];
+    }
+    if (tpz) {
+        constexpr int NS = (HZ * TX + NT - 1) / NT;
+        float ta[NS];
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            const int j = tid + k * NT;
             const int x = xi0 + 2 + (j % TX), z = zi0 + (j / TX);
-            sAZ[j] = (x < g.nx && z < g.nz) ? paz[z] * PZc[z * g.nx + x] : 0.0f;
+            ta[k] = (j < HZ * TX && x < g.nx && z < g.nz) ? paz[z] * PZc[z * g.nx + x] : 0.0f;
         }
+#pragma unroll
+        for (int k = 0; k < NS; ++k)
+            if (tid + k * NT < HZ * TX) sAZ[tid + k * NT] = ta[k];
+    }
     __syncthreads();
     const float f_t = P.wavelet[(size_t)unit * P.wav_stride + t];
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
         const int lz = tid / TX + RZ * k;
-        const int x = xi0 + lx + 2, z = zi0 + lz + 2;
+        const int x = xo, z = zi0 + lz + 2;
         if (x >= g.nx - 2 || z >= g.nz - 2) continue;
         const int i = z * g.nx + x;
         const int li = (lz + 2) * HX + (lx + 2);
         const float u = sU[li];
         const bool px = x < g.region || x >= g.nx - g.region;
         const bool pz = z < g.region || z >= g.nz - g.region;
-        Pml mx{0, 0, 0, 0}, mz{0, 0, 0, 0};
-        float psx = 0.0f, zex = 0.0f, psz = 0.0f, zez = 0.0f;
-        if (px) {
-            const float bb = pbx[x];
-            mx = Pml{pax[x], 1.0f + bb, bb, pdbx[x]};
-            psx = PXc[i];
-            zex = ZX[i];
-        }
-        if (pz) {
-            const float bb = pbz[z];
-            mz = Pml{paz[z], 1.0f + bb, bb, pdbz[z]};
-            psz = PZc[i];
-            zez = ZZ[i];
-        }
+        const Pml mz = pmz[k];
+        float psx = ppx[k], zex = pzx[k], psz = ppz[k], zez = pzz[k];
         const float* AXc = sAX + lz * HX + (lx + 2);
         const float* AZc = sAZ + (lz + 2) * TX + lx;
         float w;
@@ -177,8 +220,8 @@ __global__ __launch_bounds__(NT) void k_fwd_step(StepParams P) {
         // point source after the update (lib.rs:209-215); Born: its derivative
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-            const float wc = P.geo.src_w[s * 4 + c];
-            if (P.geo.src_cell[s * 4 + c] == i && wc != 0.0f) {
+            const float wc = sw[c];
+            if (scell[c] == i && wc != 0.0f) {
                 if (BORN) {
                     const float m = (vr > g.vmin && vr < g.vmax) ? 1.0f : 0.0f;
                     const float dv = vis_raw(P.dvel + (size_t)b * g.nvx * g.nvz, g, x, z) * m;
@@ -255,31 +298,59 @@ __global__ __launch_bounds__(NT) void k_adj_step(StepParams P) {
         }
         P.dwav[(size_t)unit * g.nt + t] = acc;
     }
-    // own cells' lambda^t, lambda^{t+1}, snapshot and gradient, loaded before
-    // the staging so that their latency overlaps it
+    // Everything a thread reads from HBM for its own cells is loaded before
+    // the staging so that its latency overlaps it: lambda^t, lambda^{t+1},
+    // the snapshot, the gradient, the C-PML memory variables and the unit's
+    // source corners.
     const int lx = tid % TX;
-    float pl[CPT], pn[CPT], ps[CPT], pg[CPT];
+    const int xo = xi0 + lx + 2;  // the thread's column (padded)
+    const bool pxo = xo < g.region_adj || xo >= g.nx - g.region_adj;
+    int scell[4];
+    float sw[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        scell[c] = P.geo.src_cell[s * 4 + c];
+        sw[c] = P.geo.src_w[s * 4 + c];
+    }
+    float pl[CPT], pn[CPT], ps[CPT], pg[CPT], ppx[CPT], pzx[CPT], ppz[CPT], pzz[CPT];
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
-        const int x = xi0 + lx + 2, z = zi0 + tid / TX + RZ * k + 2;
-        pl[k] = pn[k] = ps[k] = pg[k] = 0.0f;
-        if (x < g.nx - 2 && z < g.nz - 2) {
-            const int i = z * g.nx + x;
+        const int z = zi0 + tid / TX + RZ * k + 2;
+        pl[k] = pn[k] = ps[k] = pg[k] = ppx[k] = pzx[k] = ppz[k] = pzz[k] = 0.0f;
+        if (xo < g.nx - 2 && z < g.nz - 2) {
+            const int i = z * g.nx + xo;
             pl[k] = Lcur[i];
             pn[k] = Lnxt[i];
-            ps[k] = P.snaps[((size_t)(t - P.snap_t0) * P.units + ul) * g.inner + (size_t)(z - 2) * g.ni + (x - 2)];
+            ps[k] = P.snaps[((size_t)(t - P.snap_t0) * P.units + ul) * g.inner + (size_t)(z - 2) * g.ni + (xo - 2)];
             pg[k] = grad[i];
+            if (pxo) {
+                ppx[k] = PXr[i];
+                pzx[k] = ZXr[i];
+            }
+            if (z < g.region_adj || z >= g.nz - g.region_adj) {
+                ppz[k] = PZr[i];
+                pzz[k] = ZZr[i];
+            }
         }
     }
-    // V on the tile + halo (vw(), wgsl:53-56); 0 on the ring (lambda is 0 there)
-    for (int j = tid; j < HZ * HX; j += NT) {
-        const int x = xi0 + (j % HX), z = zi0 + (j / HX);
-        float V = 0.0f;
-        if (x < g.nx && z < g.nz) {
-            const float v = vel_at(vvis, g, x, z);
-            V = v * v * g.dt2 * Lcur[z * g.nx + x];
+    // V on the tile + halo (vw(), wgsl:53-56); 0 on the ring (lambda is 0
+    // there); each thread's loads issued together before its LDS writes
+    {
+        constexpr int NS = (HZ * HX + NT - 1) / NT;
+        float tl[NS], tv[NS];
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            const int j = tid + k * NT;
+            const int x = xi0 + (j % HX), z = zi0 + (j / HX);
+            tl[k] = tv[k] = 0.0f;
+            if (j < HZ * HX && x < g.nx && z < g.nz) {
+                tv[k] = vel_at(vvis, g, x, z);
+                tl[k] = Lcur[z * g.nx + x];
+            }
         }
-        sV[j] = V;
+#pragma unroll
+        for (int k = 0; k < NS; ++k)
+            if (tid + k * NT < HZ * HX) sV[tid + k * NT] = tv[k] * tv[k] * g.dt2 * tl[k];
     }
     __syncthreads();
     // T = (1+b)V + bZ, U1 = db T + bP of the x-neighbours (x halo) and z-neighbours
@@ -320,16 +391,16 @@ __global__ __launch_bounds__(NT) void k_adj_step(StepParams P) {
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
         const int lz = tid / TX + RZ * k;
-        const int x = xi0 + lx + 2, z = zi0 + lz + 2;
+        const int x = xo, z = zi0 + lz + 2;
         if (x >= g.nx - 2 || z >= g.nz - 2) continue;
         const int i = z * g.nx + x;
         const int li = (lz + 2) * HX + (lx + 2);
         const float v0 = sV[li];
         const float rh = g.rh;
         float w;  // x term, then += z term (0 + x skipped: equal up to the sign of zero)
-        if (x < g.region_adj || x >= g.nx - g.region_adj) {
+        if (pxo) {
             const float opb[5] = {1.0f + pbx[x - 2], 1.0f + pbx[x - 1], 1.0f + pbx[x], 1.0f + pbx[x + 1], 1.0f + pbx[x + 2]};
-            float Pv = PXr[i], Zv = ZXr[i];
+            float Pv = ppx[k], Zv = pzx[k];
             w = adj_axis_pml(sTX + lz * HX + (lx + 2), sUX + lz * HX + (lx + 2), 1, opb, pax[x], v0, rh, Pv, Zv);
             PXw[i] = Pv;
             ZXw[i] = Zv;
@@ -338,7 +409,7 @@ __global__ __launch_bounds__(NT) void k_adj_step(StepParams P) {
         }
         if (z < g.region_adj || z >= g.nz - g.region_adj) {
             const float opb[5] = {1.0f + pbz[z - 2], 1.0f + pbz[z - 1], 1.0f + pbz[z], 1.0f + pbz[z + 1], 1.0f + pbz[z + 2]};
-            float Pv = PZr[i], Zv = ZZr[i];
+            float Pv = ppz[k], Zv = pzz[k];
             w += adj_axis_pml(sTZ + (lz + 2) * TX + lx, sUZ + (lz + 2) * TX + lx, TX, opb, paz[z], v0, rh, Pv, Zv);
             PZw[i] = Pv;
             ZZw[i] = Zv;
@@ -354,9 +425,9 @@ __global__ __launch_bounds__(NT) void k_adj_step(StepParams P) {
         }
 #pragma unroll
         for (int c = 0; c < 4; ++c)
-            if (P.geo.src_cell[s * 4 + c] == i) {
+            if (scell[c] == i) {
                 const float v = vel_at(vvis, g, x, z);
-                gk += lam * (-2.0f * v * g.dt2 * P.geo.src_w[s * 4 + c] * f_t);
+                gk += lam * (-2.0f * v * g.dt2 * sw[c] * f_t);
             }
         grad[i] = pg[k] + gk;
         Lnxt[i] = next;
