@@ -1019,7 +1019,7 @@ def bench_large(args) -> int:
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
         nt_cpu = 500
-        units_cpu = 64 * threads  # about 10-15 s of CPU work
+        units_cpu = 16 * threads  # about 25 s of CPU work
         rate, cel = cpu_baseline_large(start, src, nvx, nvz, nt_cpu, dt, threads, units_cpu)
         out["cpu_baseline"] = {
             "value": round(rate * nt_cpu / nt, 5),
