@@ -510,6 +510,13 @@ __global__ __launch_bounds__(NT) void k_adj_step2(StepParams P) {
         return x >= 2 && z >= 2 && x < g.nx - 2 && z < g.nz - 2;
     };
     const int lx = tid % TX;
+    int scell[4];
+    float sw[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        scell[c] = P.geo.src_cell[s * 4 + c];
+        sw[c] = P.geo.src_w[s * 4 + c];
+    }
     float pl[CPT], pp[CPT], ps0[CPT], ps1[CPT], pg[CPT];
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
@@ -555,48 +562,96 @@ __global__ __launch_bounds__(NT) void k_adj_step2(StepParams P) {
         }
     }
 
-    // V^t on tile + 4 (vw(), wgsl:53-56); 0 on the ring of the grid and outside it
-    for (int j = tid; j < AZ * AX; j += NT) {
-        const int x = xi0 - 2 + (j % AX), z = zi0 - 2 + (j / AX);
-        float V = 0.0f;
-        if (x >= 0 && z >= 0 && x < g.nx && z < g.nz) {
-            const float v = vel_at(vvis, g, x, z);
-            V = v * v * g.dt2 * Lcur[z * g.nx + x];
+    // V^t on tile + 4 (vw(), wgsl:53-56); 0 on the ring of the grid and
+    // outside it; each thread's loads issued together before its LDS writes
+    {
+        constexpr int NS = (AZ * AX + NT - 1) / NT;
+        float tl[NS], tv[NS];
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            const int j = tid + k * NT;
+            const int x = xi0 - 2 + (j % AX), z = zi0 - 2 + (j / AX);
+            tl[k] = tv[k] = 0.0f;
+            if (j < AZ * AX && x >= 0 && z >= 0 && x < g.nx && z < g.nz) {
+                tv[k] = vel_at(vvis, g, x, z);
+                tl[k] = Lcur[z * g.nx + x];
+            }
         }
-        sV[j] = V;
+#pragma unroll
+        for (int k = 0; k < NS; ++k)
+            if (tid + k * NT < AZ * AX) sV[tid + k * NT] = tv[k] * tv[k] * g.dt2 * tl[k];
     }
     __syncthreads();
     // step A strips: T = (1+b)V + bZ, U1 = db T + bP along each axis
     const bool tax2 = xi0 < ra || xi0 + TX + 4 > g.nx - ra;
     const bool taz2 = zi0 < ra || zi0 + TZ + 4 > g.nz - ra;
-    if (tax2)
-        for (int j = tid; j < SAX; j += NT) {
+    // (each thread's HBM loads issued together before it computes and writes)
+    if (tax2) {
+        constexpr int NS = (SAX + NT - 1) / NT;
+        float tb[NS], td[NS], tz[NS], tp[NS];
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            const int j = tid + k * NT;
+            const int hx = j % AX, hz = j / AX;
+            const int x = xi0 - 2 + hx, z = zi0 + hz;
+            tb[k] = td[k] = tz[k] = tp[k] = 0.0f;
+            if (j < SAX && x >= 0 && x < g.nx && z < g.nz) {
+                const int i = z * g.nx + x;
+                tb[k] = pbx[x];
+                td[k] = pdbx[x];
+                tz[k] = ZXr[i];
+                tp[k] = PXr[i];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            const int j = tid + k * NT;
+            if (j >= SAX) continue;
             const int hx = j % AX, hz = j / AX;  // rx = hx - 4, rz = hz - 2
             const int x = xi0 - 2 + hx, z = zi0 + hz;
             float T = 0.0f, U1 = 0.0f;
             if (x >= 0 && x < g.nx && z < g.nz) {
-                const int i = z * g.nx + x;
-                const float bb = pbx[x];
-                T = (1.0f + bb) * sV[(hz + 2) * AX + hx] + bb * ZXr[i];
-                U1 = pdbx[x] * T + bb * PXr[i];
+                const float bb = tb[k];
+                T = (1.0f + bb) * sV[(hz + 2) * AX + hx] + bb * tz[k];
+                U1 = td[k] * T + bb * tp[k];
             }
             sS[A_XT + j] = T;
             sS[A_XU + j] = U1;
         }
-    if (taz2)
-        for (int j = tid; j < SAZ; j += NT) {
+    }
+    if (taz2) {
+        constexpr int NS = (SAZ + NT - 1) / NT;
+        float tb[NS], td[NS], tz[NS], tp[NS];
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            const int j = tid + k * NT;
+            const int hx = j % BX, hz = j / BX;
+            const int x = xi0 + hx, z = zi0 - 2 + hz;
+            tb[k] = td[k] = tz[k] = tp[k] = 0.0f;
+            if (j < SAZ && z >= 0 && x < g.nx && z < g.nz) {
+                const int i = z * g.nx + x;
+                tb[k] = pbz[z];
+                td[k] = pdbz[z];
+                tz[k] = ZZr[i];
+                tp[k] = PZr[i];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            const int j = tid + k * NT;
+            if (j >= SAZ) continue;
             const int hx = j % BX, hz = j / BX;  // rx = hx - 2, rz = hz - 4
             const int x = xi0 + hx, z = zi0 - 2 + hz;
             float T = 0.0f, U1 = 0.0f;
             if (z >= 0 && x < g.nx && z < g.nz) {
-                const int i = z * g.nx + x;
-                const float bb = pbz[z];
-                T = (1.0f + bb) * sV[hz * AX + hx + 2] + bb * ZZr[i];
-                U1 = pdbz[z] * T + bb * PZr[i];
+                const float bb = tb[k];
+                T = (1.0f + bb) * sV[hz * AX + hx + 2] + bb * tz[k];
+                U1 = td[k] * T + bb * tp[k];
             }
             sS[A_ZT + j] = T;
             sS[A_ZU + j] = U1;
         }
+    }
     __syncthreads();
 
     // Step A at cell (rx, rz): lambda^{t-1}, V^{t-1} and the new memory
@@ -659,9 +714,9 @@ __global__ __launch_bounds__(NT) void k_adj_step2(StepParams P) {
             float gg = pl[k] * ps0[k];
 #pragma unroll
             for (int c = 0; c < 4; ++c)
-                if (P.geo.src_cell[s * 4 + c] == i) {
+                if (scell[c] == i) {
                     const float v = vel_at(vvis, g, x, z);
-                    gg += pl[k] * (-2.0f * v * g.dt2 * P.geo.src_w[s * 4 + c] * f_a);
+                    gg += pl[k] * (-2.0f * v * g.dt2 * sw[c] * f_a);
                 }
             gk[k] = gg;
         }
@@ -753,9 +808,9 @@ __global__ __launch_bounds__(NT) void k_adj_step2(StepParams P) {
         }
 #pragma unroll
         for (int c = 0; c < 4; ++c)
-            if (P.geo.src_cell[s * 4 + c] == i) {
+            if (scell[c] == i) {
                 const float v = vel_at(vvis, g, x, z);
-                gb += l1 * (-2.0f * v * g.dt2 * P.geo.src_w[s * 4 + c] * f_b);
+                gb += l1 * (-2.0f * v * g.dt2 * sw[c] * f_b);
             }
         float gacc = pg[k];
         gacc += gk[k];
