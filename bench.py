@@ -125,6 +125,8 @@ def main() -> int:
     ap.add_argument("--no-compress", action="store_true")
     ap.add_argument("--stages", type=int, choices=(1, 2), default=2)
     ap.add_argument("--jumps", type=int, default=1)
+    ap.add_argument("--tile", type=int, default=0, help="streamed tile shape (large workload): 0 64x16, "
+                    "1 128x8, 2 32x32, 3 64x32")
     ap.add_argument("--time-block", type=int, choices=(1, 2), default=1,
                     help="large workload: adjoint steps per streamed launch")
     ap.add_argument("--time-split", type=int, default=4,
@@ -927,6 +929,7 @@ def bench_large(args) -> int:
     src = np.stack([np.linspace(0.0, (nvx - 1) * 10.0, S), np.full(S, 10.0)], 1).astype(np.float32)
     eng.set_sources(src)
     eng.set_time_block(args.time_block)
+    eng.set_stream_tile(args.tile)
     start, true = synthetic.large_batch(B, seed=3000, nvz=nvz, nvx=nvx)
     obs = ph.simulate_torch(torch.from_numpy(true / 1000.0).cuda())
     fields = torch.from_numpy(start / 1000.0).cuda().requires_grad_(True)
