@@ -31,6 +31,34 @@ struct Tile {
     static_assert(NT % TX == 0 && TZ % RZ == 0, "tile must tile the workgroup");
 };
 
+// The tile a workgroup computes.  The dispatcher hands workgroups to the
+// XCDs round-robin by linear id, so neighbouring tiles, which read each
+// other's halos, would sit behind different L2 caches.  With P.xcd_order the
+// linear ids are renumbered so that each XCD gets one contiguous run of
+// tiles in (x, z, unit) order: the halos a tile reads were mostly fetched
+// into its own XCD's L2 by its neighbours.  Tiles of one launch are
+// independent, so the order changes no result.  At 1000 x 300 (374 tiles a
+// unit) this cuts the steps' HBM reads by a third and their time by 2-4 %; on
+// grids of a few dozen tiles a unit (100 x 100, 141 x 141) the round-robin
+// order is the faster one, so the launchers turn it on from kXcdMinTiles
+// tiles a unit.
+constexpr unsigned kXcds = 8;  // MI355X
+constexpr unsigned kXcdMinTiles = 128;
+
+struct TileId {
+    int x, z, unit;
+};
+
+__device__ __forceinline__ TileId tile_id(const StepParams& P) {
+    if (!P.xcd_order) return TileId{(int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z};
+    const unsigned gx = gridDim.x, gy = gridDim.y;
+    const unsigned n = gx * gy * gridDim.z;
+    const unsigned lin = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    const unsigned k = lin % kXcds, j = lin / kXcds, q = n / kXcds, r = n % kXcds;
+    const unsigned t = k * q + (k < r ? k : r) + j;  // a bijection of [0, n)
+    return TileId{(int)(t % gx), (int)((t / gx) % gy), (int)(t / (gx * gy))};
+}
+
 #define WG_TILE_CONSTANTS                                              \
     constexpr int TX = TL::TX, TZ = TL::TZ, RZ = TL::RZ, CPT = TL::CPT; \
     constexpr int HX = TL::HX, HZ = TL::HZ;
@@ -58,7 +86,8 @@ __global__ __launch_bounds__(NT) void k_fwd_step(StepParams P) {
     __shared__ float sAX[TZ * HX];
     __shared__ float sAZ[HZ * TX];
     const Grid& g = P.g;
-    const int ul = blockIdx.z;             // unit within the chunk
+    const TileId tile = tile_id(P);
+    const int ul = tile.unit;              // unit within the chunk
     const int unit = P.unit0 + ul;
     const int b = unit / g.shots, s = src_slot(P.geo, unit, unit - (unit / g.shots) * g.shots);
     const size_t cells = (size_t)g.nx * g.nz;
@@ -79,15 +108,15 @@ __global__ __launch_bounds__(NT) void k_fwd_step(StepParams P) {
     const float* pbz = paz + g.nz;
     const float* pdbz = pbz + g.nz;
     const int tid = threadIdx.x;
-    const int xi0 = blockIdx.x * TX, zi0 = blockIdx.y * TZ;  // tile origin, inner coords
+    const int xi0 = tile.x * TX, zi0 = tile.z * TZ;  // tile origin, inner coords
 
     if (!unit_active(P.geo, unit)) {  // padding unit of an uneven batch
-        if (P.traces && blockIdx.x == 0 && blockIdx.y == 0)
+        if (P.traces && tile.x == 0 && tile.z == 0)
             for (int r = tid; r < g.nrec; r += NT) P.traces[((size_t)unit * g.nt + t) * g.nrec + r] = 0.0f;
         return;
     }
     // Receivers record u^t before the update (lib.rs:146-149): one tile per unit.
-    if (P.traces && blockIdx.x == 0 && blockIdx.y == 0) {
+    if (P.traces && tile.x == 0 && tile.z == 0) {
         float* tr = P.traces + ((size_t)unit * g.nt + t) * g.nrec;
         for (int r = tid; r < g.nrec; r += NT) {
             float acc = 0.0f;
@@ -253,7 +282,8 @@ __global__ __launch_bounds__(NT) void k_adj_step(StepParams P) {
     __shared__ float sTX[TZ * HX], sUX[TZ * HX];
     __shared__ float sTZ[HZ * TX], sUZ[HZ * TX];
     const Grid& g = P.g;
-    const int ul = blockIdx.z;
+    const TileId tile = tile_id(P);
+    const int ul = tile.unit;
     const int unit = P.unit0 + ul;
     const int b = unit / g.shots, s = src_slot(P.geo, unit, unit - (unit / g.shots) * g.shots);
     const size_t cells = (size_t)g.nx * g.nz;
@@ -281,13 +311,13 @@ __global__ __launch_bounds__(NT) void k_adj_step(StepParams P) {
     const float* pbz = paz + g.nz;
     const float* pdbz = pbz + g.nz;
     const int tid = threadIdx.x;
-    const int xi0 = blockIdx.x * TX, zi0 = blockIdx.y * TZ;
+    const int xi0 = tile.x * TX, zi0 = tile.z * TZ;
 
     if (!unit_active(P.geo, unit)) {  // padding unit; its gradient rows are zeroed by the host
-        if (DWAV && blockIdx.x == 0 && blockIdx.y == 0 && tid == 0) P.dwav[(size_t)unit * g.nt + t] = 0.0f;
+        if (DWAV && tile.x == 0 && tile.z == 0 && tid == 0) P.dwav[(size_t)unit * g.nt + t] = 0.0f;
         return;
     }
-    if (DWAV && blockIdx.x == 0 && blockIdx.y == 0 && tid == 0) {
+    if (DWAV && tile.x == 0 && tile.z == 0 && tid == 0) {
         float acc = 0.0f;
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
@@ -466,7 +496,8 @@ __global__ __launch_bounds__(NT) void k_adj_step2(StepParams P) {
     __shared__ float sV[AZ * AX];  // V^t on tile + 4 (pitch AX), then V^{t-1} on tile + 2 (pitch BX)
     __shared__ float sS[2 * SAX + 2 * SAZ];
     const Grid& g = P.g;
-    const int ul = blockIdx.z;
+    const TileId tile = tile_id(P);
+    const int ul = tile.unit;
     const int unit = P.unit0 + ul;
     const int b = unit / g.shots, s = src_slot(P.geo, unit, unit - (unit / g.shots) * g.shots);
     const size_t cells = (size_t)g.nx * g.nz;
@@ -494,7 +525,7 @@ __global__ __launch_bounds__(NT) void k_adj_step2(StepParams P) {
     const float* pbz = paz + g.nz;
     const float* pdbz = pbz + g.nz;
     const int tid = threadIdx.x;
-    const int xi0 = blockIdx.x * TX, zi0 = blockIdx.y * TZ;  // tile origin; cell (rx, rz) is padded (xi0+2+rx, zi0+2+rz)
+    const int xi0 = tile.x * TX, zi0 = tile.z * TZ;  // tile origin; cell (rx, rz) is padded (xi0+2+rx, zi0+2+rz)
     const int ra = g.region_adj;
     const float rh = g.rh;
 
@@ -825,12 +856,18 @@ __global__ __launch_bounds__(NT) void k_adj_step2(StepParams P) {
 
 namespace {
 
+StepParams ordered(const StepParams& p, const dim3& grid) {
+    StepParams q = p;
+    q.xcd_order = grid.x * grid.y >= st::kXcdMinTiles ? 1 : 0;
+    return q;
+}
+
 template <class T>
 hipError_t fwd_step_t(const StepParams& p, int units, bool store, bool born, hipStream_t st) {
     const dim3 grid((p.g.ni + T::TX - 1) / T::TX, (p.g.nzi + T::TZ - 1) / T::TZ, units);
     void (*fn)(StepParams) =
         born ? st::k_fwd_step<T, false, true> : (store ? st::k_fwd_step<T, true, false> : st::k_fwd_step<T, false, false>);
-    hipLaunchKernelGGL(fn, grid, dim3(st::NT), 0, st, p);
+    hipLaunchKernelGGL(fn, grid, dim3(st::NT), 0, st, ordered(p, grid));
     return hipGetLastError();
 }
 
@@ -838,14 +875,14 @@ template <class T>
 hipError_t adj_step_t(const StepParams& p, int units, hipStream_t st) {
     const dim3 grid((p.g.ni + T::TX - 1) / T::TX, (p.g.nzi + T::TZ - 1) / T::TZ, units);
     void (*fn)(StepParams) = p.dwav ? st::k_adj_step<T, true> : st::k_adj_step<T, false>;
-    hipLaunchKernelGGL(fn, grid, dim3(st::NT), 0, st, p);
+    hipLaunchKernelGGL(fn, grid, dim3(st::NT), 0, st, ordered(p, grid));
     return hipGetLastError();
 }
 
 template <class T>
 hipError_t adj_step2_t(const StepParams& p, int units, hipStream_t st) {
     const dim3 grid((p.g.ni + T::TX - 1) / T::TX, (p.g.nzi + T::TZ - 1) / T::TZ, units);
-    hipLaunchKernelGGL(st::k_adj_step2<T>, grid, dim3(st::NT), 0, st, p);
+    hipLaunchKernelGGL(st::k_adj_step2<T>, grid, dim3(st::NT), 0, st, ordered(p, grid));
     return hipGetLastError();
 }
 

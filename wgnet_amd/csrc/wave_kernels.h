@@ -108,6 +108,7 @@ struct StepParams {
     // for two-step launches the two written ones) and the half (0 / 1) the
     // C-PML memory variables are read from (the other half is written)
     int lam_cur, lam_prev, lam_out0, lam_out1, half;
+    int xcd_order;           // set by the launchers: tiles renumbered per XCD (large grids)
 };
 constexpr int kStreamTiles = 4;
 
