@@ -148,12 +148,10 @@ __global__ __launch_bounds__(NT) void k_fwd_step(StepParams P) {
         sw[c] = P.geo.src_w[s * 4 + c];
     }
     float pu[CPT], pv[CPT], ppx[CPT], pzx[CPT], ppz[CPT], pzz[CPT];
-    Pml pmz[CPT];
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
         const int z = zi0 + tid / TX + RZ * k + 2;
         pu[k] = pv[k] = ppx[k] = pzx[k] = ppz[k] = pzz[k] = 0.0f;
-        pmz[k] = Pml{0, 0, 0, 0};
         if (xo < g.nx - 2 && z < g.nz - 2) {
             const int i = z * g.nx + xo;
             pu[k] = Unxt[i];
@@ -163,8 +161,6 @@ __global__ __launch_bounds__(NT) void k_fwd_step(StepParams P) {
                 pzx[k] = ZX[i];
             }
             if (tpz && (z < g.region || z >= g.nz - g.region)) {
-                const float bb = pbz[z];
-                pmz[k] = Pml{paz[z], 1.0f + bb, bb, pdbz[z]};
                 ppz[k] = PZc[i];
                 pzz[k] = ZZ[i];
             }
@@ -223,7 +219,11 @@ __global__ __launch_bounds__(NT) void k_fwd_step(StepParams P) {
         const float u = sU[li];
         const bool px = x < g.region || x >= g.nx - g.region;
         const bool pz = z < g.region || z >= g.nz - g.region;
-        const Pml mz = pmz[k];
+        Pml mz{0, 0, 0, 0};
+        if (pz) {
+            const float bb = pbz[z];
+            mz = Pml{paz[z], 1.0f + bb, bb, pdbz[z]};
+        }
         float psx = ppx[k], zex = pzx[k], psz = ppz[k], zez = pzz[k];
         const float* AXc = sAX + lz * HX + (lx + 2);
         const float* AZc = sAZ + (lz + 2) * TX + lx;
