@@ -138,3 +138,37 @@ def test_time_blocked_adjoint_is_bit_identical(nvx, nvz, nt, budget):
     for key, (loss, grad) in results.items():
         assert loss == ref_loss, key
         assert np.array_equal(grad, ref_grad), key
+
+
+def test_streamed_xcd_tile_order_matches_oracle(orc):
+    """A 600 x 200 grid (644 x 244 padded): 10 x 15 tiles of 64 x 16 a unit,
+    enough for the launchers' XCD-aware tile order (from 128 tiles a unit).
+    Gathers and gradient equal the oracle; the two-step adjoint equals the
+    single-step one bit for bit."""
+    from wgnet_amd.wave_gauss_rs import PyWaveEngine
+
+    nt, nvx, nvz = 200, 600, 200
+    E = PyWaveEngine(2, nt, 0.0008, 15.0, 20, 1000.0, 5000.0, nvx=nvx, nvz=nvz)
+    assert E.streamed
+    nx, nz, _ = E.dims()
+    assert ((nx - 4 + 63) // 64) * ((nz - 4 + 15) // 16) >= 128
+    E.set_sources(np.array([[1000.0, 10.0], [4321.0, 10.0]], np.float32))
+    true = layered(nvz, nvx, 5)
+    cur = true.copy()
+    cur[3:15, 80:520] += 120.0  # shallow enough for the 160 ms recorded
+    obs = E.simulate(true.reshape(-1))
+    for s, (x, z) in enumerate(E.sources()):
+        g = orc.Geometry(nvx=nvx, nvz=nvz, nt=nt, dt=0.0008, src_x=float(x), src_z=float(z))
+        err = relerr(obs[s], orc.forward_f32(g, true, E.wavelet()))
+        print(f"600x200 shot {s}: gather rel-L2 {err:.3e}", flush=True)
+        assert err <= 1e-5
+    loss, grad = E.evaluate_adjoint(cur.reshape(-1), obs.reshape(-1))
+    rl, rg = orc.evaluate_adjoint_f32(orc.Geometry(nvx=nvx, nvz=nvz, nt=nt, dt=0.0008), E.sources(), cur,
+                                      E.wavelet(), obs)
+    print(f"600x200: loss {loss:.9e} vs {rl:.9e}; grad rel-L2 {relerr(grad, rg):.3e}", flush=True)
+    assert rl > 0.0 and np.linalg.norm(rg) > 0.0
+    assert abs(loss - rl) <= 1e-6 * rl
+    assert relerr(grad, rg) <= 1e-6
+    E.set_time_block(2)
+    loss2, grad2 = E.evaluate_adjoint(cur.reshape(-1), obs.reshape(-1))
+    assert loss2 == loss and np.array_equal(grad2, grad)
