@@ -125,6 +125,8 @@ def main() -> int:
     ap.add_argument("--no-compress", action="store_true")
     ap.add_argument("--stages", type=int, choices=(1, 2), default=2)
     ap.add_argument("--jumps", type=int, default=1)
+    ap.add_argument("--lanes", type=int, default=0, help="uneven workload: streams the groups run on "
+                    "(0: one per group)")
     ap.add_argument("--tile", type=int, default=0, help="streamed tile shape (large workload): 0 64x16, "
                     "1 128x8, 2 32x32, 3 64x32")
     ap.add_argument("--time-block", type=int, choices=(1, 2), default=1,
@@ -431,7 +433,7 @@ def bench_uneven(args) -> int:
 
     solves = random_solves(args.solves, seed=2000)
     t0 = time.perf_counter()
-    ub = UnevenBatch(solves, dt=0.001)
+    ub = UnevenBatch(solves, dt=0.001, lanes=args.lanes or None)
     tiles = ub.autotune(repeats=1)
     start, true = [], []
     for i, s in enumerate(solves):
@@ -479,6 +481,7 @@ def bench_uneven(args) -> int:
         "dtype": "f32",
         "data": "synthetic (flat/curved/faulted families per grid, Gaussian-anomaly observed data)",
         "config": {"workload": f"{len(solves)} solves, one HIP graph replay per step", "solves": len(solves),
+                   "lanes": len(ub.lanes),
                    "shot_solves": shot_solves, "groups": groups, "tiles": {f"{k[0]}x{k[1]}x{k[2]}": v
                                                                             for k, v in tiles.items()}},
         "inner_gcell_steps_per_s": round(2 * cell_steps * args.steps / el / 1e9, 2),

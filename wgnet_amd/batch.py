@@ -12,9 +12,9 @@ group runs as one launch of the resident kernels (one workgroup per unit, all
 nt steps, LDS-resident wavefield, snapshots to HBM) when its grid fits a
 workgroup, else as the HBM-streamed per-step kernels with the tile shape and
 adjoint steps per launch (one, or two with temporal blocking) that the
-autotuner picked for that grid.  Groups are issued longest first, and the
-whole batch (every group's forward, misfit and adjoint) is captured once as a
-HIP graph and replayed.
+autotuner picked for that grid.  Groups run side by side on a few streams
+(longest first onto the least loaded), and the whole batch (every group's
+forward, misfit and adjoint) is captured once as a HIP graph and replayed.
 
 Each solve applies the causal low-pass of its passband (physics.py:18-57) to
 both its wavelet and its observed data.
@@ -61,6 +61,7 @@ class _Group:
     ids: list
     engine: PyWaveEngine = None
     cost: float = 0.0
+    lane: int = 0
     buf: dict = field(default_factory=dict)
 
 
@@ -68,7 +69,7 @@ class UnevenBatch:
     """Fixed batch geometry; models and observed data are set per call."""
 
     def __init__(self, solves: list[Solve], dt: float = 0.001, peak_hz: float = 15.0, h: float = 10.0,
-                 max_shots: int = 5, memory_fraction: float = 0.6):
+                 max_shots: int = 5, memory_fraction: float = 0.6, lanes: int | None = None):
         self.solves = list(solves)
         self.dt, self.peak_hz, self.h, self.max_shots = float(dt), float(peak_hz), float(h), int(max_shots)
         self._lib = _lib.load()
@@ -92,6 +93,17 @@ class UnevenBatch:
         for g, nd in zip(self.groups, need):
             self._setup_group(g, max(int(budget * nd / max(total_need, 1.0)), 1 << 26))
         self.groups.sort(key=lambda g: -g.cost)  # longest first
+        # Groups are independent (an engine and buffers each), so they run on
+        # `lanes` streams side by side (default: one per group, at most 16):
+        # the small grids' per-step launches are latency-bound and leave the
+        # device half idle one after another.  Longest first onto the least
+        # loaded lane.
+        n_lanes = min(len(self.groups), 16) if lanes is None else min(int(lanes), len(self.groups))
+        self.lanes = [torch.cuda.Stream() for _ in range(max(1, n_lanes))]
+        load = [0.0] * len(self.lanes)
+        for g in self.groups:
+            g.lane = min(range(len(load)), key=load.__getitem__)
+            load[g.lane] += g.cost
 
     # -- set-up --------------------------------------------------------------
     def _setup_group(self, g: _Group, snap_budget: int) -> None:
@@ -226,8 +238,23 @@ class UnevenBatch:
             _p(b["obs"]), _p(b["loss"]), _p(b["grad"]), _p(b["traces"]), _p(b["energy"]), _stream()))
 
     def _run(self) -> None:
-        for g in self.groups:
-            self._run_group(g)
+        """Every group on its lane, forked from and joined back into the
+        current stream (also inside a graph capture)."""
+        cur = torch.cuda.current_stream()
+        fork = torch.cuda.Event()
+        fork.record(cur)
+        joins = []
+        for k, lane in enumerate(self.lanes):
+            lane.wait_event(fork)
+            with torch.cuda.stream(lane):
+                for g in self.groups:
+                    if g.lane == k:
+                        self._run_group(g)
+            e = torch.cuda.Event()
+            e.record(lane)
+            joins.append(e)
+        for e in joins:
+            cur.wait_event(e)
 
     def run_eager(self) -> None:
         """Loss and gradient of every solve, launched directly."""
