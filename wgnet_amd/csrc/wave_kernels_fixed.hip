@@ -532,14 +532,16 @@ __device__ __forceinline__ void a_init(ASt<R::K, R::BX, R::BZ>& S, const Place<R
 }
 
 // Phase 1: publish V, and T / U1 on the strips (T = V just outside the region).
-template <class G, class R>
+// With DWAV the step's lambda at the source corners goes to LDS for the
+// wavelet gradient; the plain adjoint needs it nowhere.
+template <class G, class R, bool DWAV>
 __device__ __forceinline__ void a_phase1(ASt<R::K, R::BX, R::BZ>& S, const Place<R>& pl, const ACtxF& C,
                                          const ColX& cx, const SrcOwnF& so) {
     if (!pl.active) return;
     constexpr int SW2 = 2 * G::SWA;
     const int xi = opq(pl.xi), zi0 = opq(pl.zi0);
     const float* PR = opq_s(C.PR);
-    const int sk = opq(so.slot);
+    const int sk = DWAV ? opq(so.slot) : -1;
     const int x = xi + 2;
     const int z0 = zi0 + 2;
     const int i0 = m24(z0, G::NX) + x;
@@ -573,7 +575,9 @@ __device__ __forceinline__ void a_phase1(ASt<R::K, R::BX, R::BZ>& S, const Place
             if (z < G::SWA) C.TZ[m24(z, G::NI) + xi] = Vc;
             else if (z >= G::NZ - G::SWA) C.TZ[m24(z - G::NZ + SW2, G::NI) + xi] = Vc;
         }
-        if (k == sk) C.SRC[so.corner] = S.lam[k];
+        if constexpr (DWAV) {
+            if (k == sk) C.SRC[so.corner] = S.lam[k];
+        }
     }
 }
 
@@ -818,12 +822,12 @@ __device__ __forceinline__ void adjoint_run(const AdjParams& P, float* lds, int 
             if (has_rec) res_next = resid[(size_t)(t - 1) * g.nrec + tid];
             f_next = wav[t - 1];
         }
-        a_phase1<G>(sI, pI, C, cx, so[0]);
-        a_phase1<G>(sX, pX, C, cx, so[1]);
-        a_phase1<G>(sZT, pZT, C, cx, so[2]);
-        a_phase1<G>(sZB, pZB, C, cx, so[3]);
-        a_phase1<G>(sCT, pCT, C, cx, so[4]);
-        a_phase1<G>(sCB, pCB, C, cx, so[5]);
+        a_phase1<G, typename RR::I, DWAV>(sI, pI, C, cx, so[0]);
+        a_phase1<G, typename RR::X, DWAV>(sX, pX, C, cx, so[1]);
+        a_phase1<G, typename RR::ZT, DWAV>(sZT, pZT, C, cx, so[2]);
+        a_phase1<G, typename RR::ZB, DWAV>(sZB, pZB, C, cx, so[3]);
+        a_phase1<G, typename RR::CT, DWAV>(sCT, pCT, C, cx, so[4]);
+        a_phase1<G, typename RR::CB, DWAV>(sCB, pCB, C, cx, so[5]);
         __syncthreads();
         if (DWAV && tid == 0) {
             float acc = 0.0f;
