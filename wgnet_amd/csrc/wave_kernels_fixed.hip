@@ -103,6 +103,10 @@ struct Place {
         last_ok = active && (row0 + (R::K - 1) * R::RP < R::H);
     }
     __device__ bool valid(int k) const { return k < R::K - 1 ? active : last_ok; }
+    // xi and zi0 recomputed from the thread index, so that they need not stay
+    // live in registers across the time loop
+    __device__ static int xi_at(int tid) { return R::xi_of(tid % R::W); }
+    __device__ static int zi0_at(int tid) { return R::ZOFF + tid / R::W; }
     // The slot whose padded row lies in [z_lo, z_hi), or -1 (rows of one
     // thread are RP > 1 apart, so a band of at most RP rows holds one).
     __device__ int slot_in_rows(int z_lo, int z_hi) const {
@@ -539,7 +543,8 @@ __device__ __forceinline__ void a_phase1(ASt<R::K, R::BX, R::BZ>& S, const Place
                                          const ColX& cx, const SrcOwnF& so) {
     if (!pl.active) return;
     constexpr int SW2 = 2 * G::SWA;
-    const int xi = opq(pl.xi), zi0 = opq(pl.zi0);
+    const int tq = opq((int)threadIdx.x);
+    const int xi = Place<R>::xi_at(tq), zi0 = Place<R>::zi0_at(tq);
     const float* PR = opq_s(C.PR);
     const int sk = DWAV ? opq(so.slot) : -1;
     const int x = xi + 2;
@@ -587,7 +592,8 @@ __device__ __forceinline__ void a_phase2(ASt<R::K, R::BX, R::BZ>& S, const Place
                                          float f_t) {
     if (!pl.active) return;
     constexpr int SW2 = 2 * G::SWA;
-    const int xi = opq(pl.xi), zi0 = opq(pl.zi0);
+    const int tq = opq((int)threadIdx.x);
+    const int xi = Place<R>::xi_at(tq), zi0 = Place<R>::zi0_at(tq);
     const float* PR = opq_s(C.PR);
     const int sk = opq(so.slot);
     const int rk = opq(rslot);
