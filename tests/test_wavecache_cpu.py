@@ -122,3 +122,10 @@ def test_rms_scale_simulates_missing_train_split(tmp_path):
     rms = wavecache.rms_scale(tmp_path, fields, phys)
     assert rms == max(_reference_loop(phys, fields["train"])[1], 1e-6)
     assert wavecache.wave_path(tmp_path, "train").exists()
+
+
+def test_rejects_empty_batches(tmp_path):
+    import pytest
+
+    with pytest.raises(ValueError):
+        wavecache.ensure_wave_cache({"train": _fields(2, 4)}, tmp_path, ("train",), _FakePhysics(), batch=0)

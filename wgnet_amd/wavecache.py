@@ -53,6 +53,8 @@ def ensure_wave_cache(fields_km: dict, cache_dir: Path | str, splits: tuple[str,
     km/s; returns split -> read-only memmap [models, 5, 251, 70].  `physics`
     is a WavePhysics (or anything with simulate_torch and shots/nt); `batch`
     models go to the GPU per call."""
+    if batch < 1 or progress_every < 1:
+        raise ValueError("batch and progress_every must be at least 1")
     cache_dir = Path(cache_dir)
     cache_dir.mkdir(parents=True, exist_ok=True)
     if physics is None:
