@@ -577,8 +577,13 @@ __device__ __forceinline__ void a_phase1(ASt<R::K, R::BX, R::BZ>& S, const Place
             C.TZ[tz] = T;
             C.UZ[tz] = U1;
         } else {
-            if (z < G::SWA) C.TZ[m24(z, G::NI) + xi] = Vc;
-            else if (z >= G::NZ - G::SWA) C.TZ[m24(z - G::NZ + SW2, G::NI) + xi] = Vc;
+            // Slot k's rows are [ZOFF + 2 + k RP, ZOFF + 1 + (k + 1) RP]: only
+            // the slots whose range meets a z strip test for it (folded at
+            // compile time after unrolling).
+            constexpr int zlo = R::ZOFF + 2;
+            if (zlo + k * R::RP < G::SWA && z < G::SWA) C.TZ[m24(z, G::NI) + xi] = Vc;
+            else if (zlo + (k + 1) * R::RP - 1 >= G::NZ - G::SWA && z >= G::NZ - G::SWA)
+                C.TZ[m24(z - G::NZ + SW2, G::NI) + xi] = Vc;
         }
         if constexpr (DWAV) {
             if (k == sk) C.SRC[so.corner] = S.lam[k];
