@@ -93,6 +93,32 @@ __global__ void k_loss_sum(const float* partials, long long chunks, int B, const
     loss[b] = energy[b] > 0.0 ? (float)acc : __builtin_nanf("");
 }
 
+// k_loss_sum for up to kLossLdsMax partials per model: one workgroup per
+// model copies its partials to LDS, then one thread adds them in chunk order
+// (k_loss_sum's sum bit for bit), reading LDS instead of waiting on HBM.
+constexpr size_t kLossLdsMax = 64 * 1024;
+
+__global__ __launch_bounds__(256) void k_loss_sum_lds(const float* partials, long long chunks,
+                                                      const double* energy, float* loss) {
+    extern __shared__ float part_s[];
+    const int b = blockIdx.x;
+    const float* pb = partials + (size_t)b * chunks;
+    for (long long c = threadIdx.x; c < chunks; c += 256) part_s[c] = pb[c];
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double acc = 0.0;
+    long long c = 0;
+    for (; c + 8 <= chunks; c += 8) {
+        float v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = part_s[c + u];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc += (double)v[u];
+    }
+    for (; c < chunks; ++c) acc += (double)part_s[c];
+    loss[b] = energy[b] > 0.0 ? (float)acc : __builtin_nanf("");
+}
+
 // wave_wgpu.rs:974-981 then fold_node_gradient (:993-1005): one thread per
 // visible node; the padded cells folding onto it are visited in row-major
 // order and summed in f64, as the reference's sequential fold does.
@@ -136,6 +162,68 @@ __global__ void k_fold(Grid g, const float* grad_unit, const float* vel, const d
     out[gid] = energy && !(energy[b] > 0.0) ? __builtin_nanf("") : (float)acc;
 }
 
+// k_fold for grids whose padded cells fit in LDS: one workgroup per model.
+// The shot sums of every padded cell (from 0.0f, shots in order) go to LDS
+// first, then each thread folds whole visible cells from LDS in k_fold's
+// (z, x) order, so the result is k_fold's bit for bit.  k_fold gives each
+// border cell (up to (PAD + 1)^2 padded cells times the shots) to one thread
+// reading HBM; here the long folds read LDS and the HBM reads are spread
+// over the workgroup.
+constexpr int kFoldThreads = 1024;
+constexpr size_t kFoldLdsMax = 160 * 1024;
+
+__global__ __launch_bounds__(kFoldThreads) void k_fold_lds(Grid g, const float* grad_unit, const float* vel,
+                                                           const double* energy, float* out) {
+    extern __shared__ float ssum[];
+    const int b = blockIdx.x;
+    const int cells = g.nx * g.nz;
+    const float* gb = grad_unit + (size_t)b * g.shots * cells;
+    constexpr int U = 4;
+    for (int c0 = threadIdx.x; c0 < cells; c0 += U * kFoldThreads) {
+        float sum_s[U] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll 5
+        for (int s = 0; s < g.shots; ++s) {
+            const float* r = gb + (size_t)s * cells;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int c = c0 + u * kFoldThreads;
+                if (c < cells) sum_s[u] += r[c];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int c = c0 + u * kFoldThreads;
+            if (c < cells) ssum[c] = sum_s[u];
+        }
+    }
+    __syncthreads();
+    const int nvis = g.nvx * g.nvz;
+    const bool bad = energy && !(energy[b] > 0.0);
+    for (int vi = threadIdx.x; vi < nvis; vi += kFoldThreads) {
+        const int vz = vi / g.nvx, vx = vi - vz * g.nvx;
+        const float v = vel[(size_t)b * nvis + vi];
+        const float mask = (v > g.vmin && v < g.vmax) ? 1.0f : 0.0f;
+        const int zlo = vz == 0 ? 0 : vz + g.pad;
+        const int zhi = vz == g.nvz - 1 ? g.nz - 1 : vz + g.pad;
+        const int xlo = vx == 0 ? 0 : vx + g.pad;
+        const int xhi = vx == g.nvx - 1 ? g.nx - 1 : vx + g.pad;
+        double acc = 0.0;
+        for (int z = zlo; z <= zhi; ++z) {
+            const float* row = ssum + z * g.nx;
+            int x = xlo;
+            for (; x + 3 <= xhi; x += 4) {
+                const float a0 = row[x], a1 = row[x + 1], a2 = row[x + 2], a3 = row[x + 3];
+                acc += (double)(a0 * mask);
+                acc += (double)(a1 * mask);
+                acc += (double)(a2 * mask);
+                acc += (double)(a3 * mask);
+            }
+            for (; x <= xhi; ++x) acc += (double)(row[x] * mask);
+        }
+        out[(size_t)b * nvis + vi] = bad ? __builtin_nanf("") : (float)acc;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
@@ -158,6 +246,14 @@ hipError_t launch_residual(const float* pred, const float* obs, const float* sca
 hipError_t launch_loss_sum(const float* partials, long long chunks, int B, const double* energy,
                            float* loss, hipStream_t st) {
     if (B <= 0) return hipSuccess;
+    const size_t lds = (size_t)chunks * sizeof(float);
+    if (chunks > 0 && lds <= kLossLdsMax) {
+        hipError_t e = hipFuncSetAttribute((const void*)k_loss_sum_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)lds);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_loss_sum_lds, dim3((unsigned)B), dim3(256), lds, st, partials, chunks, energy, loss);
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL(k_loss_sum, dim3((B + 63) / 64), dim3(64), 0, st, partials, chunks, B, energy,
                        loss);
     return hipGetLastError();
@@ -166,6 +262,15 @@ hipError_t launch_loss_sum(const float* partials, long long chunks, int B, const
 hipError_t launch_fold(const Grid& g, const float* grad_unit, const float* vel, const double* energy,
                        int B, float* grad_vis, hipStream_t st) {
     if (B <= 0) return hipSuccess;
+    const size_t lds = (size_t)g.nx * g.nz * sizeof(float);
+    if (lds <= kFoldLdsMax) {
+        hipError_t e = hipFuncSetAttribute((const void*)k_fold_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)lds);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_fold_lds, dim3((unsigned)B), dim3(kFoldThreads), lds, st, g, grad_unit, vel, energy,
+                           grad_vis);
+        return hipGetLastError();
+    }
     const long long n = (long long)B * g.nvx * g.nvz;
     hipLaunchKernelGGL(k_fold, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, grad_unit, vel,
                        energy, B, grad_vis);
