@@ -16,24 +16,27 @@ namespace wg {
 // ---------------------------------------------------------------------------
 // Per-model observed energy in f64 (wave_wgpu.rs:852; summation order is a
 // tree, not the reference's sequential loop) and scale = f32(1 / energy).
-__global__ __launch_bounds__(256) void k_energy(const float* obs, long long per, double* energy,
-                                                float* scale) {
-    __shared__ double part[256];
+constexpr int kEnergyThreads = 1024;
+
+__global__ __launch_bounds__(kEnergyThreads) void k_energy(const float* obs, long long per, double* energy,
+                                                           float* scale) {
+    constexpr int NTH = kEnergyThreads;
+    __shared__ double part[NTH];
     const float* o = obs + (size_t)blockIdx.x * per;
     double acc = 0.0;
     long long i = threadIdx.x;
     // eight loads in flight, added in the same order as one at a time
-    for (; i + 7 * 256 < per; i += 8 * 256) {
+    for (; i + 7 * NTH < per; i += 8 * NTH) {
         float v[8];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) v[u] = o[i + u * 256];
+        for (int u = 0; u < 8; ++u) v[u] = o[i + u * NTH];
 #pragma unroll
         for (int u = 0; u < 8; ++u) acc += (double)v[u] * (double)v[u];
     }
-    for (; i < per; i += 256) acc += (double)o[i] * (double)o[i];
+    for (; i < per; i += NTH) acc += (double)o[i] * (double)o[i];
     part[threadIdx.x] = acc;
     __syncthreads();
-    for (int st = 128; st > 0; st >>= 1) {
+    for (int st = NTH / 2; st > 0; st >>= 1) {
         if ((int)threadIdx.x < st) part[threadIdx.x] += part[threadIdx.x + st];
         __syncthreads();
     }
@@ -230,7 +233,7 @@ __global__ __launch_bounds__(kFoldThreads) void k_fold_lds(Grid g, const float* 
 hipError_t launch_energy(const float* obs, long long per, int B, double* energy, float* scale,
                          hipStream_t st) {
     if (B <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_energy, dim3(B), dim3(256), 0, st, obs, per, energy, scale);
+    hipLaunchKernelGGL(k_energy, dim3(B), dim3(kEnergyThreads), 0, st, obs, per, energy, scale);
     return hipGetLastError();
 }
 
