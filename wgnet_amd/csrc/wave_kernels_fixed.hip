@@ -240,7 +240,6 @@ __device__ __forceinline__ void f_compute(FSt<R::K, R::BX, R::BZ, STORE, BORN>& 
 #pragma unroll
     for (int k = 0; k < R::K; ++k) {
         if (k == R::K - 1 && !pl.last_ok) break;
-        const int i = i0 + k * R::RP * G::NX;
         const int li = k * R::RP * G::NX + 2 * G::NX + 2;  // slot k relative to Ub
         const float u = Ub[li];
         Pml pmz{0, 0, 0, 0};
@@ -619,7 +618,6 @@ __device__ __forceinline__ void a_phase2(ASt<R::K, R::BX, R::BZ>& S, const Place
 #pragma unroll
     for (int k = 0; k < R::K; ++k) {
         if (k == R::K - 1 && !pl.last_ok) break;
-        const int i = i0 + k * R::RP * G::NX;
         const int li = k * R::RP * G::NX + 2 * G::NX + 2;  // slot k relative to Vb
         const int z = z0 + k * R::RP;
         const float Sn = snap_t[q0 + k * R::RP * G::NI];
