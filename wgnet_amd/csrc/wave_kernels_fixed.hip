@@ -30,6 +30,22 @@ __device__ __forceinline__ float clampv(float v, float lo, float hi) { return v 
 // multiplier; the laundered offsets below are recomputed every step.
 __device__ __forceinline__ int m24(int a, int b) { return __mul24(a, b); }
 
+// One step's snapshot row (G::INNER floats) as a raw buffer resource: its
+// stores and loads take a 32-bit byte offset from the row, so the slots of a
+// thread need no 64-bit address arithmetic (accesses past the row are
+// dropped by the hardware bounds check).  Word 3: the gfx9 raw-buffer format.
+// The row address is the same in every lane; it is moved to scalar registers
+// explicitly, since a resource held in vector registers would make the
+// compiler loop over lanes at every access.
+template <class G>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t snap_rsrc(const float* row) {
+    const unsigned long long a = reinterpret_cast<unsigned long long>(row);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)a);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(a >> 32));
+    float* base = reinterpret_cast<float*>(((unsigned long long)hi << 32) | lo);
+    return __builtin_amdgcn_make_buffer_rsrc(base, 0, G::INNER * 4, 0x00020000);
+}
+
 __device__ __forceinline__ int opq(int v) {
     asm volatile("" : "+v"(v));
     return v;
@@ -237,6 +253,7 @@ __device__ __forceinline__ void f_compute(FSt<R::K, R::BX, R::BZ, STORE, BORN>& 
     const float* AXb = C.AX + (m24(zi0, SW2) + (x < G::REG ? x : x - G::NX + SW2) - 2);
     const float* AZb = C.AZ + (m24(R::ZOFF == 0 ? z0 : z0 - G::NZ + SW2, G::NI) + xi - 2 * G::NI);
     float dummy = 0.0f;
+    [[maybe_unused]] const __amdgpu_buffer_rsrc_t srs = snap_rsrc<G>(snap_t);
 #pragma unroll
     for (int k = 0; k < R::K; ++k) {
         if (k == R::K - 1 && !pl.last_ok) break;
@@ -251,7 +268,8 @@ __device__ __forceinline__ void f_compute(FSt<R::K, R::BX, R::BZ, STORE, BORN>& 
         const float w = fwd_w<R::BX, R::BZ>(Ub, li, G::NX, u, C.rh, C.rh2, AXb + (k * R::RP * SW2 + 2), pmx, px, zx,
                                             AZb + (k * R::RP * G::NI + 2 * G::NI), G::NI, pmz, pz, zz);
         const int q = q0 + k * R::RP * G::NI;
-        if constexpr (STORE) snap_t[q] = S.c2[k] * w;  // 2 v dt^2 w (splatForward.wgsl:108-110)
+        if constexpr (STORE)  // 2 v dt^2 w (splatForward.wgsl:108-110)
+            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(S.c2[k] * w), srs, q * 4, 0, 0);
         float next = S.c1[k] * w + 2.0f * u - S.uprev[k];
         if constexpr (BORN) next += S.dv[k] * bsnap_t[q];
         S.unext[k] = next;
@@ -614,13 +632,14 @@ __device__ __forceinline__ void a_phase2(ASt<R::K, R::BX, R::BZ>& S, const Place
     const int tzb = m24(R::ZOFF == 0 ? z0 : z0 - G::NZ + SW2, G::NI) + xi - 2 * G::NI;
     const float* TZb = C.TZ + tzb;
     const float* UZb = C.UZ + tzb;
+    const __amdgpu_buffer_rsrc_t srs = snap_rsrc<G>(snap_t);
     const float rh = C.rh;
 #pragma unroll
     for (int k = 0; k < R::K; ++k) {
         if (k == R::K - 1 && !pl.last_ok) break;
         const int li = k * R::RP * G::NX + 2 * G::NX + 2;  // slot k relative to Vb
         const int z = z0 + k * R::RP;
-        const float Sn = snap_t[q0 + k * R::RP * G::NI];
+        const float Sn = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(srs, (q0 + k * R::RP * G::NI) * 4, 0, 0));
         const float v0 = Vb[li];
         float w;  // x term, then += z term (0 + x skipped: equal up to the sign of zero)
         if constexpr (R::BX) {
