@@ -336,7 +336,24 @@ __device__ __forceinline__ void forward_run(const FwdParams& P, float* lds, int 
         return;
     }
 
-    for (int j = tid; j < G::F_PR; j += NT) lds[j] = st_in ? st_in[j] : 0.0f;
+    // the LDS field and strips: restored from the previous part (eight loads
+    // in flight per thread, not one load and wait at a time) or cleared
+    {
+        constexpr int RU = 8;
+        for (int j0 = tid; j0 < G::F_PR; j0 += RU * NT) {
+            float v[RU];
+#pragma unroll
+            for (int u = 0; u < RU; ++u) {
+                const int j = j0 + u * NT;
+                v[u] = (st_in && j < G::F_PR) ? st_in[j] : 0.0f;
+            }
+#pragma unroll
+            for (int u = 0; u < RU; ++u) {
+                const int j = j0 + u * NT;
+                if (j < G::F_PR) lds[j] = v[u];
+            }
+        }
+    }
     // profile tables: a, 1 + b, b, db for x then z (geometry prof: ax bx dbx az bz dbz)
     for (int j = tid; j < G::NX; j += NT) {
         const float bb = P.geo.prof[G::NX + j];
