@@ -208,17 +208,17 @@ __device__ __forceinline__ void f_init(FSt<R::K, R::BX, R::BZ, STORE, BORN>& S, 
         if constexpr (R::BX) { S.px[k] = 0.0f; S.zx[k] = 0.0f; }
         if constexpr (R::BZ) { S.pz[k] = 0.0f; S.zz[k] = 0.0f; }
         if constexpr (BORN) S.dv[k] = 0.0f;
-        if (pl.valid(k)) {
-            const int x = pl.xi + 2, z = pl.zi0 + k * R::RP + 2;
-            const int vz = clampi(z - G::PAD, 0, G::NVZ - 1), vx = clampi(x - G::PAD, 0, G::NVX - 1);
-            const float vr = vvis[vz * G::NVX + vx];
-            const float v = clampv(vr, g.vmin, g.vmax);
-            S.c1[k] = v * v * g.dt2;
-            if constexpr (STORE) S.c2[k] = 2.0f * v * g.dt2;
-            if constexpr (BORN) {
-                const float m = (vr > g.vmin && vr < g.vmax) ? 1.0f : 0.0f;
-                S.dv[k] = dvis[vz * G::NVX + vx] * m;
-            }
+        // clamped model index: every thread loads, valid slots keep the values
+        const int x = pl.xi + 2, z = pl.zi0 + k * R::RP + 2;
+        const int vz = clampi(z - G::PAD, 0, G::NVZ - 1), vx = clampi(x - G::PAD, 0, G::NVX - 1);
+        const float vr = vvis[vz * G::NVX + vx];
+        const float v = clampv(vr, g.vmin, g.vmax);
+        const bool ok = pl.valid(k);
+        S.c1[k] = ok ? v * v * g.dt2 : 0.0f;
+        if constexpr (STORE) S.c2[k] = ok ? 2.0f * v * g.dt2 : 0.0f;
+        if constexpr (BORN) {
+            const float m = (vr > g.vmin && vr < g.vmax) ? 1.0f : 0.0f;
+            S.dv[k] = ok ? dvis[vz * G::NVX + vx] * m : 0.0f;
         }
     }
 }
@@ -560,12 +560,12 @@ __device__ __forceinline__ void a_init(ASt<R::K, R::BX, R::BZ>& S, const Place<R
         S.lam[k] = S.lamp[k] = S.gr[k] = S.c1[k] = 0.0f;
         if constexpr (R::BX) { S.Px[k] = 0.0f; S.Zx[k] = 0.0f; }
         if constexpr (R::BZ) { S.Pz[k] = 0.0f; S.Zz[k] = 0.0f; }
-        if (pl.valid(k)) {
-            const int x = pl.xi + 2, z = pl.zi0 + k * R::RP + 2;
-            const int vz = clampi(z - G::PAD, 0, G::NVZ - 1), vx = clampi(x - G::PAD, 0, G::NVX - 1);
-            const float v = clampv(vvis[vz * G::NVX + vx], g.vmin, g.vmax);
-            S.c1[k] = v * v * g.dt2;  // the factor of vw(), wgsl:53-56
-        }
+        // The model index is clamped, so every thread may load (the loads of
+        // all slots then overlap); only valid slots keep the factor.
+        const int x = pl.xi + 2, z = pl.zi0 + k * R::RP + 2;
+        const int vz = clampi(z - G::PAD, 0, G::NVZ - 1), vx = clampi(x - G::PAD, 0, G::NVX - 1);
+        const float v = clampv(vvis[vz * G::NVX + vx], g.vmin, g.vmax);
+        S.c1[k] = pl.valid(k) ? v * v * g.dt2 : 0.0f;  // the factor of vw(), wgsl:53-56
     }
 }
 
