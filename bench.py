@@ -42,8 +42,38 @@ FP32_PEAK_TFLOPS = 157.3
 SURFACE_SRC = np.array([[0, 10], [170, 10], [340, 10], [520, 10], [690, 10]], np.float32)
 
 
-def cpu_baseline(start, true, nt, dt, threads, units):
-    """The float32 CPU solver (forward + adjoint) on `units` (model, shot) solves."""
+CHECK_MODELS = (0, 1, 2, 63)  # one per family (flat, curved, faulted = index % 3) and the last model
+
+
+def host_cpu() -> dict:
+    """The host's CPU model and core counts (all cores, and those this process
+    may run on)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "host_cores": os.cpu_count(), "affinity_cores": len(os.sched_getaffinity(0))}
+
+
+def relerr(a, b) -> float:
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def cpu_baseline(start, true, nt, dt, threads, units, check=None):
+    """The float32 CPU solver (forward + adjoint) on `units` (model, shot) solves.
+
+    With `check` (the timed batch's last losses, gradients and gathers), the
+    CPU leg also solves `check["models"]` with the oracle in float32 and
+    float64 and returns the GPU's errors against both."""
+    from concurrent.futures import ThreadPoolExecutor
+
     from oracle import oracle as O
 
     g = O.Geometry(nt=nt, dt=dt)
@@ -58,7 +88,46 @@ def cpu_baseline(start, true, nt, dt, threads, units):
     t0 = time.perf_counter()
     O.batch_fwd_adj_f32(g, src, vel, w, obs, threads)
     el = time.perf_counter() - t0
-    return units / el, el
+    if check is None:
+        return units / el, el, None
+
+    wv = check["wavelet"]
+
+    def one(m):
+        v, ob = check["start"][m], check["obs"][m]
+        l32, g32 = O.evaluate_adjoint_f32(g, SURFACE_SRC, v, wv, ob)
+        l64, g64 = O.evaluate_adjoint_f64(g, SURFACE_SRC, v, wv.astype(np.float64), ob.astype(np.float64))
+        tr32, tr64 = [], []
+        for x, z in SURFACE_SRC:
+            gs = O.Geometry(nt=nt, dt=dt, src_x=float(x), src_z=float(z))
+            tr32.append(O.forward_f32(gs, v, wv))
+            tr64.append(O.forward_f64(gs, v, wv.astype(np.float64)))
+        return m, l32, g32, l64, g64, np.stack(tr32), np.stack(tr64)
+
+    with ThreadPoolExecutor(min(threads, len(check["models"]))) as pool:
+        res = list(pool.map(one, check["models"]))
+    worst = {"gathers_f32": 0.0, "gathers_f64": 0.0, "loss_f32": 0.0, "loss_f64": 0.0, "grad_f32": 0.0,
+             "grad_f64": 0.0}
+    bit_equal = True
+    for m, l32, g32, l64, g64, tr32, tr64 in res:
+        loss, gk, tr = float(check["losses"][m]), check["grads_km"][m], check["traces"][m]
+        gms = gk.astype(np.float64) / 1000.0
+        worst["gathers_f32"] = max(worst["gathers_f32"], relerr(tr, tr32))
+        worst["gathers_f64"] = max(worst["gathers_f64"], relerr(tr, tr64))
+        worst["loss_f32"] = max(worst["loss_f32"], abs(loss - l32) / l32)
+        worst["loss_f64"] = max(worst["loss_f64"], abs(loss - l64) / l64)
+        worst["grad_f32"] = max(worst["grad_f32"], relerr(gms, g32))
+        worst["grad_f64"] = max(worst["grad_f64"], relerr(gms, g64))
+        bit_equal &= bool(np.array_equal(tr, tr32)) and loss == l32 and bool(
+            np.array_equal(gk, g32 * np.float32(1000.0)))
+    chk = {"models": list(check["models"]), "what": "the timed batch's last losses and gradients, and the gathers "
+           "of the same call, against the CPU oracle (float32 and float64) on the same inputs",
+           "max_rel_err": {k: float(f"{v:.3e}") for k, v in worst.items()},
+           "bit_equal_f32": bit_equal, "repeat_bit_equal": check["repeat_equal"],
+           "gates": {"gathers_f32": 1e-5, "loss_f32": 1e-6, "grad_f32": 1e-6}}
+    chk["pass"] = bool(worst["gathers_f32"] <= 1e-5 and worst["loss_f32"] <= 1e-6 and worst["grad_f32"] <= 1e-6
+                       and check["repeat_equal"])
+    return units / el, el, chk
 
 
 def cpu_baseline_large(start, src, nvx, nvz, nt, dt, threads, units):
@@ -77,32 +146,30 @@ def cpu_baseline_large(start, src, nvx, nvz, nt, dt, threads, units):
     return units / el, el
 
 
-def valu_from_profiles(kernel: str):
-    """VALU instructions per launch of `kernel` (rocprofv3 SQ_INSTS_VALU) from
-    profiles/valu_<kernel>.json, or None."""
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", f"valu_{kernel}.json")
-    if os.path.exists(path):
-        with open(path) as f:
-            return json.load(f).get("sq_insts_valu_per_launch")
-    return None
-
-
-def traffic_from_profiles(kernel: str):
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", f"traffic_{kernel}.json")
-    if os.path.exists(path):
-        with open(path) as f:
-            return json.load(f).get("bytes_per_launch")
-    return None
+def profile_figure(kind: str, kernel: str, key: str, run_cfg: dict):
+    """(value, source) of `key` in profiles/<kind>_<kernel>.json when that
+    profile was taken on this run's workload (its "config" equals run_cfg),
+    else None: a counter read on another configuration is not this run's."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", f"{kind}_{kernel}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    if d.get("config") != run_cfg or key not in d:
+        return None
+    return d[key], f"profiles/{kind}_{kernel}.json (rocprofv3 --pmc, same workload, measured before this run)"
 
 
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=150)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--models", type=int, default=64)
     ap.add_argument("--nt", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-refine", action="store_true", help="default workload: skip the in-run refine "
+                    "solves-to-misfit measurement (about two minutes)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--workload", choices=("batch70", "large", "copy", "refine", "uneven", "wavecache", "physstage",
                                            "fwi"),
@@ -171,8 +238,9 @@ def main() -> int:
     def step():
         losses = ph.misfit(fields, obs)
         losses.sum().backward()
+        g = fields.grad
         fields.grad = None
-        return losses
+        return losses, g
 
     for _ in range(args.warmup):
         step()
@@ -184,7 +252,7 @@ def main() -> int:
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        losses = step()
+        losses, grads_km = step()
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     lib.wg_timing_read(h, ctypes.byref(fms), ctypes.byref(fn), ctypes.byref(ams), ctypes.byref(an))
@@ -193,6 +261,26 @@ def main() -> int:
 
     units = B * S
     value = units * args.steps / el
+    # A timed region this short is over before a coarse utilisation sampler
+    # looks: run the same step again, untimed, until 3 s of it have passed and
+    # report that rate beside the timed one.
+    sustained = None
+    if el < 3.0:
+        n_s, t1 = 0, time.perf_counter()
+        while time.perf_counter() - t1 < 3.0:
+            step()
+            n_s += 1
+            if n_s % 16 == 0:
+                torch.cuda.synchronize()
+        torch.cuda.synchronize()
+        el_s = time.perf_counter() - t1
+        sustained = {"steps": n_s, "seconds": round(el_s, 3), "value": round(units * n_s / el_s, 2)}
+    # the last timed step's results, with the predicted gathers of the same
+    # call (the queue is deterministic: the losses must repeat bit for bit),
+    # for the oracle check of the CPU leg
+    traces = torch.empty((B, S, nt, ph.engine.nrec), device="cuda")
+    l_again, _ = ph.loss_and_grad_torch(fields.detach(), obs, traces_out=traces)
+    repeat_equal = bool(torch.equal(l_again, losses.detach()))
     nx, nz, _ = ph.engine.dims()
     inner, R = (nx - 4) * (nz - 4), ph.engine.nrec
     # Algorithmic HBM bytes per unit: forward writes the snapshot (4 B per inner
@@ -234,6 +322,7 @@ def main() -> int:
             "workload": f"{B} models x {S} shots, 70x70 grid (114x114 padded), nt={nt}, dt=1 ms, "
                         "forward + adjoint gradient through torch.autograd.Function",
             "models": B, "shots": S, "nt": nt, "grid": "70x70", "pml": 20, "parallelism": "1 GPU",
+            "time_split": args.time_split,
         },
         "roofline": {
             "bound": "hbm",
@@ -242,22 +331,34 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic_from_profiles(kern),
+            "traffic": None,
             "algorithmic_bytes_per_launch": int(kbytes),
             "launch_ms": launch_ms,
             "launches_per_step": launches,
             "shot_solve_hbm_frac": round(ss_bytes * value / 1e9 / HBM_PEAK_GBS, 4),
         },
     }
-    valu = valu_from_profiles(kern)
+    if sustained:
+        out["sustained"] = sustained
+    run_cfg = {"models": B, "shots": S, "nt": nt, "time_split": args.time_split}
+    traffic = profile_figure("traffic", kern, "bytes_per_launch", run_cfg)
+    if traffic:
+        out["roofline"]["traffic"] = traffic[0]
+        out["roofline"]["traffic_source"] = traffic[1]
+    valu = profile_figure("valu", kern, "sq_insts_valu_per_launch", run_cfg)
     if valu:
         # the kernel's real bound: VALU issue (a wave64 instruction takes 4 cycles
-        # on a 16-lane SIMD; 256 CUs x 4 SIMDs at 2.4 GHz), not HBM
-        out["roofline"]["valu_issue_frac"] = round(valu * 4 / (1024 * VALU_CLOCK_HZ * kavg), 4)
+        # on a 16-lane SIMD; 256 CUs x 4 SIMDs at 2.4 GHz), not HBM: the profiled
+        # instruction count of this workload over this run's launch time
+        out["roofline"]["valu_issue_frac"] = round(valu[0] * 4 / (1024 * VALU_CLOCK_HZ * kavg), 4)
+        out["roofline"]["valu_source"] = valu[1]
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
         units_cpu = 64 * threads  # about 10-15 s of CPU work
-        rate, cel = cpu_baseline(start, true, nt, dt, threads, units_cpu)
+        check = {"models": CHECK_MODELS, "obs": obs.cpu().numpy(), "losses": losses.detach().cpu().numpy(),
+                 "grads_km": grads_km.cpu().numpy(), "traces": traces.cpu().numpy(),
+                 "wavelet": ph.engine.wavelet(), "start": start, "repeat_equal": repeat_equal}
+        rate, cel, chk = cpu_baseline(start, true, nt, dt, threads, units_cpu, check)
         out["cpu_baseline"] = {
             "value": round(rate, 3),
             "unit": "shot-solves/s",
@@ -265,10 +366,35 @@ def main() -> int:
             "kind": "port",
             "sample": f"{units_cpu} (model, shot) forward+adjoint solves of the same workload, "
                       f"{threads} threads, {cel:.1f} s wall; float32 C restatement of the reference solver",
+            **host_cpu(),
         }
         out["speedup_vs_cpu"] = round(value / rate, 1)
+        out["oracle_check"] = chk
+    if not args.no_refine:
+        out["refine"] = refine_summary(args)
     print(json.dumps(out), flush=True)
     return 0
+
+
+def refine_summary(args) -> dict:
+    """The second half of the metric, refine solves-to-misfit, measured in the
+    same run: the network trained in-run on solver pairs, then plain projected
+    L-BFGS at the full budget (its final misfit is the target) and the
+    learned arm at a quarter and at half of that budget."""
+    r = run_refine(args, plain_fracs=(1.0,), learned_fracs=(0.25, 0.5), pseudo=False)
+    v = r["value"]
+    return {
+        "metric": "refine solves-to-misfit (learned / plain L-BFGS)",
+        "value": v if np.isfinite(v) else None,
+        "unit": "fraction of plain L-BFGS's solve budget to reach its final misfit (median over models; "
+                "null = above one half)",
+        "target": 0.5,
+        "reached_at": r["reached_at"],
+        "median_final_misfit": r["median_final_misfit"],
+        "config": r["config"],
+        "train": r["train"],
+        "wall_s": r["wall_s"],
+    }
 
 
 def refine_scenes(n: int, seed: int):
@@ -301,15 +427,21 @@ def refine_lattice_field():
     return render_residual(torch.from_numpy(lat[act])[None], torch.from_numpy(act))[0]
 
 
-def bench_refine(args) -> int:
+def run_refine(args, plain_fracs=(0.25, 0.5, 0.75, 1.0), learned_fracs=(0.25, 0.5, 0.75, 1.0),
+               pseudo: bool = True) -> dict:
     """Learned-preconditioned refinement against plain projected L-BFGS:
-    solves needed to reach plain L-BFGS's final data misfit."""
+    solves needed to reach plain L-BFGS's final data misfit.  The network is
+    trained in the run on pairs from the batched solver; plain L-BFGS runs at
+    `plain_fracs` of the budget (its final misfit at the full budget is the
+    target), the learned arm at `learned_fracs` and, with `pseudo`, the
+    illumination-scaled arm at the plain fractions."""
     import torch
 
     from wgnet_amd.physics import WavePhysics
     from wgnet_amd.precond import Preconditioner, make_pairs, pseudo_hessian_hook, train
     from wgnet_amd.refine import refine_batch
 
+    t_all = time.perf_counter()
     S, nt, dt = 5, 1000, 0.001
     bands = (6.0, 9.0, 12.0, 15.0)
     ph = WavePhysics(shots=S, nt=nt, dt=dt)
@@ -348,52 +480,54 @@ def bench_refine(args) -> int:
     start_km, true_km, gauss = refine_scenes(args.eval_models, seed=900000)
     obs = ph.simulate_torch(torch.from_numpy(true_km).cuda())
     B = args.budget
-    fracs = (0.25, 0.5, 0.75, 1.0)
     plain_final = {}
     t_plain = 0.0
-    for f in fracs:
+    for f in plain_fracs:
         t0 = time.perf_counter()
         plain = refine_batch(start_km, gauss, obs, ph, budget_solves=int(B * f), bands_hz=bands)
         t_plain = time.perf_counter() - t0
         plain_final[f] = np.array([r.final_loss for r in plain])
         print(f"refine: plain L-BFGS {int(B * f)} solves in {t_plain:.1f} s, median final misfit "
               f"{np.median(plain_final[f]):.4e}", file=sys.stderr, flush=True)
-    m_plain = plain_final[1.0]
+    m_plain = plain_final[max(plain_fracs)]
     pc_final = {}
     if args.pc_mode == "jump":
         learned = {"update": net.update_hook(), "jumps_per_band": args.jumps}
     else:
         learned = {"precondition": net.hook(), "pc_mode": args.pc_mode}
-    for f in fracs:
+    for f in learned_fracs:
         res = refine_batch(start_km, gauss, obs, ph, budget_solves=int(B * f), bands_hz=bands, **learned)
         pc_final[f] = np.array([r.final_loss for r in res])
         print(f"refine: learned preconditioner {int(B * f)} solves: median final misfit "
               f"{np.median(pc_final[f]):.4e}, reaches plain in {np.mean(pc_final[f] <= m_plain):.3f} of models",
               file=sys.stderr, flush=True)
-    # the physics baseline: diagonal pseudo-Hessian (illumination) scaling; its
-    # one extra forward per model is counted as one solve
-    ph_hook = pseudo_hessian_hook(ph, torch.from_numpy(start_km).cuda())
     phs_final = {}
-    for f in fracs:
-        res = refine_batch(start_km, gauss, obs, ph, budget_solves=int(B * f) - 1, bands_hz=bands,
-                           precondition=ph_hook, pc_mode="replace")
-        phs_final[f] = np.array([r.final_loss for r in res])
-        print(f"refine: pseudo-Hessian {int(B * f)} solves: median final misfit {np.median(phs_final[f]):.4e}, "
-              f"reaches plain in {np.mean(phs_final[f] <= m_plain):.3f} of models", file=sys.stderr, flush=True)
+    if pseudo:
+        # the physics baseline: diagonal pseudo-Hessian (illumination) scaling;
+        # its one extra forward per model is counted as one solve
+        ph_hook = pseudo_hessian_hook(ph, torch.from_numpy(start_km).cuda())
+        for f in plain_fracs:
+            res = refine_batch(start_km, gauss, obs, ph, budget_solves=int(B * f) - 1, bands_hz=bands,
+                               precondition=ph_hook, pc_mode="replace")
+            phs_final[f] = np.array([r.final_loss for r in res])
+            print(f"refine: pseudo-Hessian {int(B * f)} solves: median final misfit "
+                  f"{np.median(phs_final[f]):.4e}, reaches plain in {np.mean(phs_final[f] <= m_plain):.3f} of "
+                  f"models", file=sys.stderr, flush=True)
 
     def solves_needed(final):
+        # the smallest tried fraction at which a model reaches plain's final
+        # misfit (inf: none of them)
         need = np.full(len(m_plain), np.inf)
-        for f in reversed(fracs):
+        for f in sorted(final, reverse=True):
             need[final[f] <= m_plain] = f
         return need
 
     need = solves_needed(pc_final)
-    need_ph = solves_needed(phs_final)
     init = np.array([r.initial_loss for r in plain])
     out = {
         "metric": "refine solves-to-misfit: preconditioned solves to reach plain L-BFGS's final misfit / plain budget",
         "value": float(np.median(need)),
-        "unit": "fraction of the plain budget (median over models; inf = not reached)",
+        "unit": "fraction of the plain budget (median over models; inf = not reached at the fractions tried)",
         "n_gpus": 1,
         "higher_is_better": False,
         "vs_baseline": None,
@@ -405,21 +539,30 @@ def bench_refine(args) -> int:
                    "epochs": args.epochs, "stages": args.stages, "jumps_per_band": args.jumps,
                    "pc_mode": args.pc_mode, "compress": net.compress,
                    "layers": net.layers()},
-        "reached_at": {str(f): float(np.mean(need <= f)) for f in fracs},
+        "fractions_tried": {"plain": list(plain_fracs), "learned": list(learned_fracs)},
+        "reached_at": {str(f): float(np.mean(need <= f)) for f in learned_fracs},
         "median_final_misfit": {"initial": float(np.median(init)),
-                                **{f"plain_{f}": float(np.median(plain_final[f])) for f in fracs},
-                                **{f"learned_{f}": float(np.median(pc_final[f])) for f in fracs},
-                                **{f"pseudo_hessian_{f}": float(np.median(phs_final[f])) for f in fracs}},
-        "learned_better_at_equal_budget": {str(f): float(np.mean(pc_final[f] < plain_final[f])) for f in fracs},
-        "pseudo_hessian": {"solves_needed_median": float(np.median(need_ph)),
-                           "reached_at": {str(f): float(np.mean(need_ph <= f)) for f in fracs},
-                           "better_at_equal_budget": {str(f): float(np.mean(phs_final[f] < plain_final[f]))
-                                                      for f in fracs}},
+                                **{f"plain_{f}": float(np.median(plain_final[f])) for f in plain_fracs},
+                                **{f"learned_{f}": float(np.median(pc_final[f])) for f in learned_fracs},
+                                **{f"pseudo_hessian_{f}": float(np.median(phs_final[f])) for f in phs_final}},
+        "learned_better_at_equal_budget": {str(f): float(np.mean(pc_final[f] < plain_final[f]))
+                                           for f in learned_fracs if f in plain_final},
         "train": {"pairs_s": round(t_pairs, 2), "train_s": round(t_train, 2), "loss_first": hist[0],
                   "loss_last": hist[-1], "zero_prediction_loss": baseline, "grad_scale": net.grad_scale},
         "plain_refine_s": round(t_plain, 2),
+        "wall_s": round(time.perf_counter() - t_all, 1),
     }
-    print(json.dumps(out), flush=True)
+    if pseudo:
+        need_ph = solves_needed(phs_final)
+        out["pseudo_hessian"] = {"solves_needed_median": float(np.median(need_ph)),
+                                 "reached_at": {str(f): float(np.mean(need_ph <= f)) for f in plain_fracs},
+                                 "better_at_equal_budget": {str(f): float(np.mean(phs_final[f] < plain_final[f]))
+                                                            for f in plain_fracs}}
+    return out
+
+
+def bench_refine(args) -> int:
+    print(json.dumps(run_refine(args)), flush=True)
     return 0
 
 
@@ -582,7 +725,7 @@ def bench_wavecache(args) -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
-            "traffic": traffic_from_profiles(kern),
+            "traffic": None,
             "algorithmic_bytes_per_launch": f_bytes,
             "launch_ms": {kern: round(1e3 * f_avg, 3)},
             "launches_per_step": fn.value / args.steps,
@@ -591,9 +734,14 @@ def bench_wavecache(args) -> int:
                            "note": "engine.simulate of one model at a time (data.py:160-163), same engine"},
         "gpu_forward_share": round(fms.value / 1e3 / el, 3),
     }
-    valu = valu_from_profiles(kern)
+    run_cfg = {"workload": "wavecache", "batch": args.cache_batch, "shots": S, "nt": nt}
+    traffic = profile_figure("traffic", kern, "bytes_per_launch", run_cfg)
+    if traffic:
+        out["roofline"]["traffic"], out["roofline"]["traffic_source"] = traffic
+    valu = profile_figure("valu", kern, "sq_insts_valu_per_launch", run_cfg)
     if valu:
-        out["roofline"]["valu_issue_frac"] = round(valu * 4 / (1024 * VALU_CLOCK_HZ * f_avg), 4)
+        out["roofline"]["valu_issue_frac"] = round(valu[0] * 4 / (1024 * VALU_CLOCK_HZ * f_avg), 4)
+        out["roofline"]["valu_source"] = valu[1]
     if not args.no_cpu:
         from oracle import oracle as O
 
@@ -735,7 +883,7 @@ def bench_physstage(args) -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
-            "traffic": traffic_from_profiles("k_adjoint_fx_physstage") if ak == "k_adjoint_fx" and nb == 8 else None,
+            "traffic": None,
             "algorithmic_bytes_per_launch": int(adj_bytes),
             "launch_ms": {"k_forward_fx (observed and storing, side by side), mean": round(1e3 * f_avg, 3),
                           ak: round(1e3 * a_avg, 3)},
@@ -1014,7 +1162,7 @@ def bench_large(args) -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic_from_profiles(adj_kern),
+            "traffic": None,
             "algorithmic_bytes_per_launch": adj_bytes,
             "launch_ms": {"k_fwd_step": round(1e3 * f_avg, 4), adj_kern: round(1e3 * a_avg, 4)},
             "adjoint_steps_per_launch": round(adj_spl, 3),

@@ -131,6 +131,17 @@ int32_t wg_loss_grad_wavelets_dev(wg_engine *e, int32_t B, const float *vel_dev,
                                   float *traces_dev, double *energy_dev,
                                   void *stream);
 
+/* The loss alone, for evaluations that need no gradient (a misfit taken
+ * without autograd; the reference runs the adjoint inside every
+ * evaluate_adjoint, wave_wgpu.rs:848-988): the forward without snapshots,
+ * the residual and the loss of
+ * wg_loss_grad_dev, whose losses it returns bit for bit.  wav_dev [B, shots,
+ * nt] per-(model, shot) wavelets, or NULL for the engine wavelet; traces_dev
+ * and energy_dev may be NULL. */
+int32_t wg_loss_dev(wg_engine *e, int32_t B, const float *vel_dev,
+                    const float *wav_dev, const float *obs_dev, float *loss_dev,
+                    float *traces_dev, double *energy_dev, void *stream);
+
 /* Uneven batches (no reference counterpart; the reference solves one model
  * with the engine's own source list per call, wave_wgpu.rs:761-785).  Each
  * of the B models has `shots` unit slots; unit u = b * shots + k carries its
@@ -223,6 +234,13 @@ int32_t wg_set_time_block(wg_engine *e, int32_t steps);
  * (nothing to balance) or its snapshots exceed the snapshot budget.
  * Timing (below) counts the forward and adjoint queue launches. */
 int32_t wg_set_time_split(wg_engine *e, int32_t parts);
+
+/* Polls a work-queue task makes for its unit's previous part before it gives
+ * up (no reference counterpart; default 1 << 24, seconds, where a part takes
+ * milliseconds).  A wait that gives up makes every loss and gradient of that
+ * call NaN.  A negative value makes every wait give up at once: it drives
+ * that failure path in tests, with every kernel still ending normally. */
+int32_t wg_set_queue_wait_limit(wg_engine *e, int32_t polls);
 
 /* Kernel timing (no reference counterpart; the reference records no
  * timestamps, wave_wgpu.rs:829).  When enabled, HIP events are recorded on the

@@ -81,6 +81,7 @@ def lib() -> ctypes.CDLL:
             "or_energy": (ctypes.c_double, [ctypes.c_int64, _f32p]),
             "or_evaluate_adjoint_f32": (ctypes.c_int32, [P, ctypes.c_int32, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p]),
             "or_evaluate_adjoint_f64": (ctypes.c_int32, [P, ctypes.c_int32, _f32p, _f32p, _f64p, _f64p, _f64p, _f64p]),
+            "or_loss_f64": (ctypes.c_int32, [P, ctypes.c_int32, _f32p, _f32p, _f64p, _f64p, _f64p]),
             "or_batch_fwd_adj_f32": (None, [P, ctypes.c_int32, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, ctypes.c_int32]),
             "or_batch_forward_f32": (None, [P, ctypes.c_int32, _f32p, _f32p, _f32p, _f32p, ctypes.c_int32]),
         }
@@ -273,6 +274,20 @@ def evaluate_adjoint_f64(g: Geometry, src_xz: np.ndarray, vel_vis, wavelet, obs)
     if rc != 0:
         raise ValueError("Observed traces have zero energy")
     return float(loss[0]), out.reshape(g.nvz, g.nvx)
+
+
+def loss_f64(g: Geometry, src_xz: np.ndarray, vel_vis, wavelet, obs) -> float:
+    """The float64 misfit alone (or_evaluate_adjoint_f64's loss, bit for bit).
+    The call releases the GIL, so threads of a pool run side by side."""
+    src = np.ascontiguousarray(src_xz, np.float32)
+    loss = np.zeros(1, np.float64)
+    rc = lib().or_loss_f64(ctypes.byref(g.problem()), src.shape[0], _p(src, np.float32),
+                           _p(np.ascontiguousarray(vel_vis, np.float32).reshape(-1), np.float32),
+                           _p(np.ascontiguousarray(wavelet, np.float64), np.float64),
+                           _p(np.ascontiguousarray(obs, np.float64).reshape(-1), np.float64), _p(loss, np.float64))
+    if rc != 0:
+        raise ValueError("Observed traces have zero energy")
+    return float(loss[0])
 
 
 def batch_fwd_adj_f32(g: Geometry, src_xz, vel_vis, wavelet, obs, threads: int):

@@ -352,6 +352,32 @@ int32_t or_evaluate_adjoint_f64(const or_problem *p0, int32_t shots,
     return 0;
 }
 
+int32_t or_loss_f64(const or_problem *p0, int32_t shots, const float *src_xz,
+                    const float *vel_vis, const double *wavelet,
+                    const double *obs, double *loss) {
+    or_problem p = *p0;
+    const int64_t per = (int64_t)p.nt * p.nrec;
+    const int64_t count = per * shots;
+    double energy = 0.0;
+    for (int64_t i = 0; i < count; ++i) energy += obs[i] * obs[i];
+    if (!(energy > 0.0)) return -1;
+    const double scale = 1.0 / energy;
+    double *pred = (double *)malloc(sizeof(double) * (size_t)per);
+    double l = 0.0;
+    for (int s = 0; s < shots; ++s) {
+        p.src_x = src_xz[2 * s];
+        p.src_z = src_xz[2 * s + 1];
+        or_forward_f64(&p, vel_vis, wavelet, pred, NULL);
+        for (int64_t i = 0; i < per; ++i) {
+            const double dd = pred[i] - obs[s * per + i];
+            l += 0.5 * dd * dd * scale;
+        }
+    }
+    *loss = l;
+    free(pred);
+    return 0;
+}
+
 /* ---- multi-threaded CPU baseline over independent (model, shot) units ---- */
 
 typedef struct {

@@ -135,6 +135,14 @@ int32_t or_evaluate_adjoint_f64(const or_problem *p, int32_t shots,
                                 const double *wavelet, const double *obs,
                                 double *loss, double *grad_vis);
 
+/* The loss of or_evaluate_adjoint_f64 alone (forward of every shot, no
+ * snapshots, no adjoint), with the same additions in the same order, so the
+ * two return the same double.  Used for central finite differences of the
+ * float64 misfit.  Returns 0, or -1 on zero observed energy. */
+int32_t or_loss_f64(const or_problem *p, int32_t shots, const float *src_xz,
+                    const float *vel_vis, const double *wavelet,
+                    const double *obs, double *loss);
+
 /* CPU baseline: `units` independent (model, shot) forward+adjoint solves on
  * `threads` POSIX threads.  vel_vis [units, nvz*nvx], obs [units, nt, nrec],
  * src_xz [units, 2].  grads [units, cells] (padded), losses [units].

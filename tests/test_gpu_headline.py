@@ -1,0 +1,234 @@
+"""The headline workload at its own size, through the path bench.py times.
+
+64 synthetic models (flat / curved / faulted, observed data from each model plus
+a Gaussian anomaly) x 5 shots at x-cells {0, 17, 34, 52, 69}, 70 x 70 (114 x 114
+padded), nt = 1000 of 1 ms: one loss through the torch.autograd.Function and its
+backward, which run the resident work queues (every unit's forward and adjoint
+cut into 4 time parts of 250 steps, 1280 tasks per launch).
+
+Checked against the CPU oracle on the same inputs (wave_wgpu.rs:848-988 restated):
+  - losses, gradients and predicted gathers of one model per family and of the
+    last model (its units are the last tasks of both queues):
+      gathers   rel-L2 <= 1e-5 vs the float32 oracle
+      loss      |rel| <= 1e-6, gradient rel-L2 <= 1e-6 vs the float32 oracle
+      (the kernels keep the reference's operation order, so both are expected
+      to be bit-equal; the error against the float64 oracle is printed and
+      bounded: gathers 1e-4, gradient 1e-3)
+  - the FP64-accumulated dot-product tests of the wavelet and the Born operator
+    at 5 shots and nt = 1000 on the three family models: <= 1e-6 with y = F x;
+    <= 1e-5 with y = F x' for an independent x'; <= 5e-5 with white-noise y.
+    The last two have cancellation in the inner product: for white noise
+    y, <J dv, y> of the float32 Born forward alone differs from the float64
+    forward's by 1e-4 relative (oracle, one 70 x 70 x 1000 shot), so float32
+    roundoff, not transposition, sets their level; the float32 oracle pair
+    (restated forward and WGSL adjoint) shows 1.5-3.4e-6 and up to 1.5e-5
+    on the same cases
+  - central finite differences with h = 1 m/s at 16 random visible cells of
+    each of those four models, each difference two float64 oracle losses
+    (thread pool), against the work-queue gradient:
+    |fd - g| <= 1e-3 |fd| + 1e-3 max|g|
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+SURFACE_SRC = np.array([[0, 10], [170, 10], [340, 10], [520, 10], [690, 10]], np.float32)
+B, S, NT, DT = 64, 5, 1000, 0.001
+CHECKED = (0, 1, 2, 63)  # flat, curved, faulted (family = index % 3), and the last model
+
+
+def relerr(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def _threads() -> int:
+    return max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+@pytest.fixture(scope="module")
+def run():
+    from wgnet_amd import synthetic
+    from wgnet_amd.physics import WavePhysics
+
+    ph = WavePhysics(shots=S, nt=NT, dt=DT)
+    ph.engine.set_sources(SURFACE_SRC)
+    lib, h = ph.engine._lib, ph.engine.handle
+    assert lib.wg_set_time_split(h, 4) == 0
+    start, true = synthetic.batch(B, seed=1000)
+    obs = ph.simulate_torch(torch.from_numpy(true / 1000.0).cuda())
+    fields = torch.from_numpy(start / 1000.0).cuda().requires_grad_(True)
+    fms, fn, ams, an = ctypes.c_double(), ctypes.c_int64(), ctypes.c_double(), ctypes.c_int64()
+    lib.wg_timing_enable(h, 1)
+    lib.wg_timing_read(h, ctypes.byref(fms), ctypes.byref(fn), ctypes.byref(ams), ctypes.byref(an))
+    losses = ph.misfit(fields, obs)  # bench.py's step: the Function, then its backward
+    losses.sum().backward()
+    lib.wg_timing_read(h, ctypes.byref(fms), ctypes.byref(fn), ctypes.byref(ams), ctypes.byref(an))
+    lib.wg_timing_enable(h, 0)
+    launches = (fn.value, an.value)
+    traces = torch.empty((B, S, NT, ph.engine.nrec), device="cuda")
+    l2, _ = ph.loss_and_grad_torch(fields.detach(), obs, traces_out=traces)
+    torch.cuda.synchronize()
+    return {
+        "ph": ph, "start": start, "true": true, "obs": obs.cpu().numpy(), "losses": losses.detach().cpu().numpy(),
+        "losses2": l2.cpu().numpy(), "grads_km": fields.grad.cpu().numpy(), "traces": traces.cpu().numpy(),
+        "wavelet": ph.engine.wavelet(), "launches": launches,
+    }
+
+
+def test_headline_runs_the_work_queues(run):
+    """320 units on the device's compute units: one forward queue launch and
+    one adjoint queue launch (the chunk pipeline would take two of each)."""
+    print(f"forward launches {run['launches'][0]}, adjoint launches {run['launches'][1]}")
+    assert run["launches"] == (1, 1)
+    assert np.array_equal(run["losses"], run["losses2"])  # deterministic
+    assert np.isfinite(run["losses"]).all() and (run["losses"] > 0).all()
+
+
+def test_headline_matches_oracle(run, orc):
+    g = orc.Geometry(nt=NT, dt=DT)
+    w32 = run["wavelet"]
+    w64 = w32.astype(np.float64)
+
+    def one(b):
+        vel = run["start"][b]
+        obs = run["obs"][b]
+        l32, g32 = orc.evaluate_adjoint_f32(g, SURFACE_SRC, vel, w32, obs)
+        l64, g64 = orc.evaluate_adjoint_f64(g, SURFACE_SRC, vel, w64, obs.astype(np.float64))
+        tr32 = np.stack([orc.forward_f32(orc.Geometry(nt=NT, dt=DT, src_x=float(x), src_z=float(z)), vel, w32)
+                         for x, z in SURFACE_SRC])
+        tr64 = np.stack([orc.forward_f64(orc.Geometry(nt=NT, dt=DT, src_x=float(x), src_z=float(z)), vel, w64)
+                         for x, z in SURFACE_SRC])
+        ob32 = np.stack([orc.forward_f32(orc.Geometry(nt=NT, dt=DT, src_x=float(x), src_z=float(z)),
+                                         run["true"][b], w32) for x, z in SURFACE_SRC])
+        return b, l32, g32, l64, g64, tr32, tr64, ob32
+
+    with ThreadPoolExecutor(_threads()) as pool:
+        results = list(pool.map(one, CHECKED))
+    for b, l32, g32, l64, g64, tr32, tr64, ob32 in results:
+        loss, gk, tr = float(run["losses"][b]), run["grads_km"][b], run["traces"][b]
+        g_ms = gk.astype(np.float64) / 1000.0
+        e_tr, e_tr64 = relerr(tr, tr32), relerr(tr, tr64)
+        e_obs = relerr(run["obs"][b], ob32)
+        e_l, e_l64 = abs(loss - l32) / l32, abs(loss - l64) / l64
+        e_g, e_g64 = relerr(g_ms, g32), relerr(g_ms, g64)
+        same_g = float(np.mean(gk == g32 * np.float32(1000.0)))
+        print(f"model {b} (family {b % 3}): gathers rel-L2 {e_tr:.2e} vs FP32 (bit-equal {np.mean(tr == tr32):.4f}), "
+              f"{e_tr64:.2e} vs FP64; observed {e_obs:.2e}; loss {loss:.9e} vs FP32 {l32:.9e} (rel {e_l:.1e}), "
+              f"vs FP64 {e_l64:.2e}; gradient rel-L2 {e_g:.2e} vs FP32 (bit-equal {same_g:.4f}), {e_g64:.2e} vs FP64",
+              flush=True)
+        assert e_tr <= 1e-5 and e_obs <= 1e-5
+        assert e_tr64 <= 1e-4
+        assert e_l <= 1e-6
+        assert e_g <= 1e-6
+        assert e_g64 <= 1e-3
+
+
+def _dev(x):
+    return torch.from_numpy(np.ascontiguousarray(x)).cuda()
+
+
+def _dot(a, b):
+    return float((a.double() * b.double()).sum())
+
+
+def test_headline_dot_products(run):
+    """<F x, y> = <x, F^T y> for the wavelet operator and <J dv, y> = <dv, J^T y>
+    for the Born operator at 5 shots, nt = 1000, on one model per family."""
+    from wgnet_amd import _lib
+    from wgnet_amd.physics import _stream
+
+    ph = run["ph"]
+    E = ph.engine
+    nb = 3
+    vel = _dev(run["start"][:nb])
+    P = lambda t: ctypes.c_void_p(t.data_ptr())
+    rng = np.random.default_rng(77)
+
+    def F(x):
+        out = torch.empty((nb, S, NT, 70), device="cuda")
+        _lib.check(E._lib.wg_simulate_wavelets_dev(E.handle, nb, P(vel), P(x), P(out), _stream()))
+        return out
+
+    def Ft(y):
+        out = torch.empty((nb, S, NT), device="cuda")
+        _lib.check(E._lib.wg_adjoint_apply_dev(E.handle, nb, P(vel), P(y), None, P(out), None, _stream()))
+        return out
+
+    def J(dv):
+        out = torch.empty((nb, S, NT, 70), device="cuda")
+        _lib.check(E._lib.wg_born_dev(E.handle, nb, P(vel), P(dv), P(out), _stream()))
+        return out
+
+    def Jt(y):
+        out = torch.empty((nb, 70, 70), device="cuda")
+        _lib.check(E._lib.wg_adjoint_apply_dev(E.handle, nb, P(vel), P(y), P(out), None, None, _stream()))
+        return out
+
+    x = _dev(rng.standard_normal((nb, S, NT)).astype(np.float32))
+    x2 = _dev(rng.standard_normal((nb, S, NT)).astype(np.float32))
+    fx = F(x)
+    cases = [("wavelet", fx, x, Ft, "y = F x", fx, 1e-6),
+             ("wavelet", fx, x, Ft, "y = F x'", F(x2), 1e-5),
+             ("wavelet", fx, x, Ft, "y white noise", _dev(rng.standard_normal((nb, S, NT, 70)).astype(np.float32)),
+              5e-5)]
+    dv = _dev(rng.standard_normal((nb, 70, 70)).astype(np.float32))
+    dv2 = _dev(rng.standard_normal((nb, 70, 70)).astype(np.float32))
+    jdv = J(dv)
+    cases += [("Born", jdv, dv, Jt, "y = J dv", jdv, 1e-6),
+              ("Born", jdv, dv, Jt, "y = J dv'", J(dv2), 1e-5),
+              ("Born", jdv, dv, Jt, "y white noise", _dev(rng.standard_normal((nb, S, NT, 70)).astype(np.float32)),
+               5e-5)]
+    for name, fx_, x_, adj, label, y, tol in cases:
+        lhs, rhs = _dot(fx_, y), _dot(x_, adj(y.contiguous()))
+        rel = abs(lhs - rhs) / abs(lhs)
+        print(f"{name} dot test ({label}), 3 models x 5 shots x nt 1000: <Fx,y>={lhs:.12e} <x,F^T y>={rhs:.12e} "
+              f"rel={rel:.3e} (gate {tol:.0e})", flush=True)
+        assert rel <= tol
+
+
+def test_headline_finite_differences(run, orc):
+    """Central differences, h = 1 m/s, at 16 random visible cells of each
+    checked model: float64 oracle losses against the observed gathers the GPU
+    gradient was taken against."""
+    g = orc.Geometry(nt=NT, dt=DT)
+    w64 = run["wavelet"].astype(np.float64)
+    rng = np.random.default_rng(1601)
+    jobs = []
+    for b in CHECKED:
+        cells = rng.choice(70 * 70, 16, replace=False)
+        for c in cells:
+            for sgn in (1.0, -1.0):
+                jobs.append((b, int(c), sgn))
+
+    def loss(job):
+        b, c, sgn = job
+        v = run["start"][b].copy().reshape(-1)
+        v[c] += np.float32(sgn)
+        return orc.loss_f64(g, SURFACE_SRC, v, w64, run["obs"][b].astype(np.float64))
+
+    with ThreadPoolExecutor(_threads()) as pool:
+        vals = list(pool.map(loss, jobs))
+    fd = {}
+    for (b, c, sgn), v in zip(jobs, vals):
+        fd.setdefault((b, c), {})[sgn] = v
+    worst = 0.0
+    for (b, c), lv in fd.items():
+        d = (lv[1.0] - lv[-1.0]) / 2.0
+        gk = run["grads_km"][b]
+        gpu = float(gk.reshape(-1)[c]) / 1000.0
+        tol = 1e-3 * abs(d) + 1e-3 * float(np.abs(gk).max()) / 1000.0
+        worst = max(worst, abs(d - gpu) / tol)
+        assert abs(d - gpu) <= tol, (b, divmod(c, 70), d, gpu)
+    print(f"finite differences: {len(fd)} cells ({len(CHECKED)} models x 16), h = 1 m/s, FP64 oracle losses; "
+          f"largest |fd - g| / tolerance {worst:.3f}", flush=True)

@@ -358,12 +358,15 @@ def test_overlapped_chunk_pipeline_is_bit_identical(B):
     assert torch.equal(g_on, g_off)
 
 
-@pytest.mark.parametrize("B,shots,nt", [(1, 5, 300), (13, 5, 301), (64, 5, 300), (130, 2, 257)])
+@pytest.mark.parametrize("B,shots,nt", [(52, 5, 301), (64, 5, 300), (130, 2, 257), (257, 1, 257)])
 def test_time_split_work_queue_is_bit_identical(B, shots, nt):
     """The resident work queue (each unit's forward and adjoint cut into 1,
     2, 3, 4 or 7 time parts, state handed over through HBM, one launch for
     the batch) gives the same traces, losses and gradients as whole-unit
-    chunk launches."""
+    chunk launches.  Every shape has more units than the device has compute
+    units (the queues step aside otherwise), with an uneven last round."""
+    import ctypes
+
     import torch
 
     from wgnet_amd import synthetic
@@ -371,6 +374,8 @@ def test_time_split_work_queue_is_bit_identical(B, shots, nt):
 
     ph = WavePhysics(shots=shots, nt=nt, dt=0.001)
     lib, h = ph.engine._lib, ph.engine.handle
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    assert B * shots > cus, (B * shots, cus)
     start, true = synthetic.batch(B, seed=4343 + B)
     obs = ph.simulate_torch(torch.from_numpy(true / 1000.0).cuda())
     f = torch.from_numpy(start / 1000.0).cuda()
@@ -379,11 +384,17 @@ def test_time_split_work_queue_is_bit_identical(B, shots, nt):
     tr0 = torch.empty((B, shots, nt, R), device="cuda")
     l0, g0 = ph.loss_and_grad_torch(f, obs, traces_out=tr0)
     assert bool(torch.isfinite(l0).all()) and float(g0.abs().max()) > 0.0
+    fms, fn, ams, an = ctypes.c_double(), ctypes.c_int64(), ctypes.c_double(), ctypes.c_int64()
     for parts in (1, 2, 3, 4, 7):
         lib.wg_set_time_split(h, parts)
         tr = torch.empty_like(tr0)
+        lib.wg_timing_enable(h, 1)
+        lib.wg_timing_read(h, ctypes.byref(fms), ctypes.byref(fn), ctypes.byref(ams), ctypes.byref(an))
         l, g = ph.loss_and_grad_torch(f, obs, traces_out=tr)
         torch.cuda.synchronize()
+        lib.wg_timing_read(h, ctypes.byref(fms), ctypes.byref(fn), ctypes.byref(ams), ctypes.byref(an))
+        lib.wg_timing_enable(h, 0)
+        assert (fn.value, an.value) == (1, 1), (parts, fn.value, an.value)  # the queues ran
         assert torch.equal(tr, tr0), parts
         assert torch.equal(l, l0), parts
         assert torch.equal(g, g0), parts

@@ -219,6 +219,27 @@ def test_gradient_central_fd_fp64(orc):
     assert lp != loss
 
 
+def test_loss_f64_equals_evaluate_adjoint_loss(orc):
+    """or_loss_f64 (forward and misfit only) returns the loss of
+    or_evaluate_adjoint_f64, which the FD and dot tests above pin, to the bit;
+    and the closed form 0.5 ||p - d||^2 / ||d||^2 on its own forward."""
+    g = geom(orc)
+    true = layered(g.nvz, g.nvx)
+    cur = true.copy()
+    cur[5:9, 20:30] += 90.0
+    src = np.array([[40.0, 10.0], [123.4, 17.0]], np.float32)
+    w = orc.ricker_f64(g.nt, g.dt, g.peak_hz)
+    obs = np.stack([orc.forward_f64(geom(orc, src_x=float(s[0]), src_z=float(s[1])), true, w) for s in src])
+    l_adj, _ = orc.evaluate_adjoint_f64(g, src, cur, w, obs)
+    l_only = orc.loss_f64(g, src, cur, w, obs)
+    assert l_only == l_adj
+    pred = np.stack([orc.forward_f64(geom(orc, src_x=float(s[0]), src_z=float(s[1])), cur, w) for s in src])
+    ref = 0.5 * float(((pred - obs) ** 2).sum()) / float((obs ** 2).sum())
+    assert abs(l_only - ref) <= 1e-12 * ref
+    with pytest.raises(ValueError, match="zero energy"):
+        orc.loss_f64(g, src, cur, w, np.zeros_like(obs))
+
+
 def test_evaluate_adjoint_f32_vs_f64(orc):
     g = geom(orc)
     true = layered(g.nvz, g.nvx)

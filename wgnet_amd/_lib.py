@@ -62,6 +62,7 @@ SIGNATURES = {
     "wg_loss_grad_dev": (_i32, [_vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "wg_loss_grad_true_dev": (_i32, [_vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "wg_loss_grad_wavelets_dev": (_i32, [_vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "wg_loss_dev": (_i32, [_vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "wg_source_corners": (_i32, [_vp, _i32, _vp, _vp, _vp]),
     "wg_loss_grad_units_dev": (_i32, [_vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "wg_simulate_units_dev": (_i32, [_vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
@@ -75,6 +76,7 @@ SIGNATURES = {
     "wg_set_overlap": (_i32, [_vp, _i32]),
     "wg_set_time_block": (_i32, [_vp, _i32]),
     "wg_set_time_split": (_i32, [_vp, _i32]),
+    "wg_set_queue_wait_limit": (_i32, [_vp, _i32]),
     "wg_causal_filter_dev": (_i32, [_vp, ctypes.c_int64, _i32, _i32, _vp, _i32, _vp, _vp]),
     "wg_set_stream_tile": (_i32, [_vp, _i32]),
     "wg_timing_enable": (_i32, [_vp, _i32]),

@@ -110,6 +110,7 @@ struct wg_engine {
     bool overlap = true;           // overlap chunk forwards with the previous chunk's adjoint
     int time_block = 1;            // adjoint steps per streamed launch (1 or 2)
     int qparts = 4;                // time parts per unit of the resident work queues (0: chunk pipeline)
+    int qspin = 1 << 24;           // polls before a work-queue wait gives up
     DevBuf qstate, qflags;         // work queues: per-unit part state; counters, error flag, done flags
     hipStream_t last = nullptr;    // stream of the last enqueued call
     bool last_valid = false;
@@ -576,6 +577,7 @@ int queue_loss_grad(wg_engine* e, int B, const float* vel, const float* obs, flo
     q.done_a = q.done_f + (size_t)parts * U;
     q.units = U;
     q.parts = parts;
+    q.spin_limit = e->qspin;
     const int wgs = std::min(e->cus, parts * U);
     hipError_t err = hipSuccess;
     {
@@ -640,8 +642,8 @@ int loss_grad_core(wg_engine* e, int B, const float* vel, const float* obs, floa
     if ((rc = queue_loss_grad(e, B, vel, obs, traces, scale, st, &queued))) return rc;
     if (queued) {
         LAUNCH(wg::launch_loss_sum((const float*)e->partials.p, chunks, B, energy, loss, st));
-        LAUNCH(wg::launch_flag_nan((const int*)e->qflags.p + 2, loss, B, st));
         LAUNCH(wg::launch_fold(g, (const float*)e->gunit.p, vel, energy, B, grad, st));
+        LAUNCH(wg::launch_flag_nan((const int*)e->qflags.p + 2, loss, B, grad, (long long)g.nvx * g.nvz, st));
         return WG_OK;
     }
     const int mc = models_per_chunk(e, B);
@@ -1077,6 +1079,42 @@ int32_t wg_loss_grad_wavelets_dev(wg_engine* e, int32_t B, const float* vel_dev,
     return loss_grad_entry(e, B, vel_dev, wav_dev, obs_dev, loss_dev, grad_dev, traces_dev, energy_dev, stream);
 }
 
+int32_t wg_loss_dev(wg_engine* e, int32_t B, const float* vel_dev, const float* wav_dev, const float* obs_dev,
+                    float* loss_dev, float* traces_dev, double* energy_dev, void* stream) {
+    if (!e || !vel_dev || !obs_dev || !loss_dev || B < 0) return fail(WG_E_INVALID, "invalid argument");
+    Guard G(e);
+    WavScope W(e, wav_dev);
+    HIPCK(hipSetDevice(e->device));
+    hipStream_t st = (hipStream_t)stream;
+    int rc;
+    if ((rc = fence(e, st))) return rc;
+    if (B == 0) return WG_OK;
+    const wg::Grid& g = e->g;
+    const long long per = (long long)e->cfg.shots * g.nt * g.nrec;
+    const long long chunks = (per + 255) / 256;
+    if ((rc = ensure(e, e->scale, (size_t)B * sizeof(float)))) return rc;
+    if ((rc = ensure(e, e->resid, (size_t)B * per * sizeof(float)))) return rc;
+    if ((rc = ensure(e, e->partials, (size_t)B * chunks * sizeof(float)))) return rc;
+    double* energy = energy_dev;
+    if (!energy) {
+        if ((rc = ensure(e, e->energy, (size_t)B * sizeof(double)))) return rc;
+        energy = (double*)e->energy.p;
+    }
+    float* traces = traces_dev;
+    if (!traces) {
+        if ((rc = ensure(e, e->traces, (size_t)B * per * sizeof(float)))) return rc;
+        traces = (float*)e->traces.p;
+    }
+    float* scale = (float*)e->scale.p;
+    LAUNCH(wg::launch_energy(obs_dev, per, B, energy, scale, st));
+    // the forward without snapshots gives the storing forward's gathers bit for
+    // bit, so the loss is the one wg_loss_grad_dev returns
+    if ((rc = run_chunk_forward(e, 0, B, vel_dev, traces, false, st))) return rc;
+    LAUNCH(wg::launch_residual(traces, obs_dev, scale, per, B, (float*)e->resid.p, (float*)e->partials.p, st));
+    LAUNCH(wg::launch_loss_sum((const float*)e->partials.p, chunks, B, energy, loss_dev, st));
+    return WG_OK;
+}
+
 int32_t wg_source_corners(const wg_engine* e, int32_t n, const float* xz, int32_t* cell, float* w) {
     if (!e || n < 0 || (n > 0 && (!xz || !cell || !w))) return fail(WG_E_INVALID, "invalid argument");
     for (int k = 0; k < n; ++k) {
@@ -1274,6 +1312,13 @@ int32_t wg_set_time_split(wg_engine* e, int32_t parts) {
     if (!e || parts < 0 || parts > 16) return fail(WG_E_INVALID, "parts must be in [0, 16]");
     Guard G(e);
     e->qparts = parts;
+    return WG_OK;
+}
+
+int32_t wg_set_queue_wait_limit(wg_engine* e, int32_t polls) {
+    if (!e) return fail(WG_E_INVALID, "null engine");
+    Guard G(e);
+    e->qspin = polls < 0 ? -1 : polls;
     return WG_OK;
 }
 

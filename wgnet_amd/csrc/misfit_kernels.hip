@@ -295,14 +295,20 @@ hipError_t launch_zero(float* p, size_t n, hipStream_t st) {
 }
 
 // Marks every loss NaN when a device-side wait gave up (err != 0).
-__global__ void k_flag_nan(const int* err, float* loss, int B) {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b < B && *err != 0) loss[b] = __builtin_nanf("");
+// A work-queue wait that gave up (its predecessor's state never arrived) sets
+// *err; then every model's loss and folded gradient of that call are NaN, so
+// no caller can take a gradient built on an unfinished state for a good one.
+__global__ void k_flag_nan(const int* err, float* loss, int B, float* grad, long long nvis) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (*err == 0) return;
+    if (i < B) loss[i] = __builtin_nanf("");
+    if (grad && i < (long long)B * nvis) grad[i] = __builtin_nanf("");
 }
 
-hipError_t launch_flag_nan(const int* err, float* loss, int B, hipStream_t st) {
+hipError_t launch_flag_nan(const int* err, float* loss, int B, float* grad, long long nvis, hipStream_t st) {
     if (B <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_flag_nan, dim3((B + 63) / 64), dim3(64), 0, st, err, loss, B);
+    const long long n = std::max<long long>(B, grad ? (long long)B * nvis : 0);
+    hipLaunchKernelGGL(k_flag_nan, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, err, loss, B, grad, nvis);
     return hipGetLastError();
 }
 

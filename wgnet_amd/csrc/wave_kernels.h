@@ -151,10 +151,13 @@ struct QueueParams {
     int *next_f, *next_a, *err;
     int *done_f, *done_a;
     int units, parts;
+    int spin_limit;  // polls before a wait gives up and sets *err (< 0: at once)
 };
 bool queue_state_fx(const Grid& g, size_t* fwd_floats, size_t* adj_floats);
 bool launch_queue_fx(const QueueParams& q, bool adjoint, int workgroups, hipStream_t st, hipError_t* err);
-hipError_t launch_flag_nan(const int* err, float* loss, int B, hipStream_t st);
+// After a queued loss and gradient: NaN losses and gradients [B, nvis] (grad
+// may be null) when a queue wait timed out (*err != 0).
+hipError_t launch_flag_nan(const int* err, float* loss, int B, float* grad, long long nvis, hipStream_t st);
 size_t forward_lds_bytes_fx(const Grid& g);
 size_t adjoint_lds_bytes_fx(const Grid& g, const Geometry& geo);
 hipError_t launch_adjoint(const AdjParams& p, int units, hipStream_t st);

@@ -934,11 +934,17 @@ __global__ __launch_bounds__(NT) void k_adjoint_fx(AdjParams P) {
 // The two bodies are separate kernels: in one kernel their registers would
 // be allocated together and the forward's time loop would spill.
 // ===========================================================================
-constexpr int kSpinLimit = 1 << 24;  // ~ seconds of s_sleep polling; tasks take milliseconds
-
-__device__ __forceinline__ void wait_done(const int* flag, int* err) {
+// Polls of a wait before it gives up (QueueParams::spin_limit; the engine's
+// default is 1 << 24, seconds of s_sleep polling where tasks take
+// milliseconds).  A negative limit makes every wait give up at once, which
+// lets a test drive the timeout path with every kernel still ending normally.
+__device__ __forceinline__ void wait_done(const int* flag, int* err, int limit) {
+    if (limit < 0) {
+        __hip_atomic_fetch_or(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return;
+    }
     for (int n = 0; __hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0; ++n) {
-        if (n > kSpinLimit) {
+        if (n > limit) {
             __hip_atomic_fetch_or(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             return;
         }
@@ -948,8 +954,8 @@ __device__ __forceinline__ void wait_done(const int* flag, int* err) {
 
 // Takes the next task of a queue and waits for its predecessor; returns
 // false when the queue is empty.  All threads of the workgroup call it.
-__device__ __forceinline__ bool next_task(int* next, const int* done, int ntask, int units, int* err, int* ts,
-                                          int& p, int& u) {
+__device__ __forceinline__ bool next_task(int* next, const int* done, int ntask, int units, int* err, int limit,
+                                          int* ts, int& p, int& u) {
     __syncthreads();  // the previous task is done with LDS and *ts
     if (threadIdx.x == 0) *ts = __hip_atomic_fetch_add(next, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
@@ -960,7 +966,7 @@ __device__ __forceinline__ bool next_task(int* next, const int* done, int ntask,
     if (p > 0) {
         // the acquiring load invalidates this compute unit's L1 and this
         // XCD's L2 before the barrier lets the other waves read
-        if (threadIdx.x == 0) wait_done(done + (size_t)(p - 1) * units + u, err);
+        if (threadIdx.x == 0) wait_done(done + (size_t)(p - 1) * units + u, err, limit);
         __syncthreads();
     }
     return true;
@@ -980,7 +986,7 @@ __global__ __launch_bounds__(NT) void k_queue_fwd_fx(QueueParams Q) {
     __shared__ int task_s;
     const int parts = Q.parts, units = Q.units, nt = Q.f.g.nt;
     int p, u;
-    while (next_task(Q.next_f, Q.done_f, parts * units, units, Q.err, &task_s, p, u)) {
+    while (next_task(Q.next_f, Q.done_f, parts * units, units, Q.err, Q.spin_limit, &task_s, p, u)) {
         float* st = Q.fstate + (size_t)u * FState<G>::FLOATS;
         const int t0 = (int)((long long)nt * p / parts), t1 = (int)((long long)nt * (p + 1) / parts);
         forward_run<G, true, false>(Q.f, lds, u, t0, t1, p > 0 ? st : nullptr, p < parts - 1 ? st : nullptr);
@@ -994,7 +1000,7 @@ __global__ __launch_bounds__(NT) void k_queue_adj_fx(QueueParams Q) {
     __shared__ int task_s;
     const int parts = Q.parts, units = Q.units, nt = Q.a.g.nt;
     int j, u;
-    while (next_task(Q.next_a, Q.done_a, parts * units, units, Q.err, &task_s, j, u)) {
+    while (next_task(Q.next_a, Q.done_a, parts * units, units, Q.err, Q.spin_limit, &task_s, j, u)) {
         float* st = Q.astate + (size_t)u * AState<G>::FLOATS;
         const int sg = parts - 1 - j;
         const int t_lo = (int)((long long)nt * sg / parts), t_hi = (int)((long long)nt * (sg + 1) / parts);

@@ -13,6 +13,7 @@ from __future__ import annotations
 
 import ctypes
 import math
+import weakref
 
 import numpy as np
 import torch
@@ -85,11 +86,14 @@ class WaveMisfit(torch.autograd.Function):
     """losses[b] = 0.5 ||p_b - d_b||^2 / ||d_b||^2 for fields in km/s.
 
     forward runs the HIP forward (storing snapshots), the misfit and the HIP
-    adjoint in one batched call; backward scales the adjoint gradient by the
-    upstream gradient of each model's loss."""
+    adjoint in one batched call when the fields need a gradient, and the
+    forward and misfit alone when they do not; backward scales the adjoint
+    gradient by the upstream gradient of each model's loss."""
 
     @staticmethod
     def forward(ctx, fields_km: torch.Tensor, observed: torch.Tensor, physics: "WavePhysics"):
+        if not ctx.needs_input_grad[0]:
+            return physics.loss_torch(fields_km.detach(), observed)
         losses, grads_km = physics.loss_and_grad_torch(fields_km.detach(), observed)
         ctx.save_for_backward(grads_km)
         return losses
@@ -112,6 +116,7 @@ class WavePhysics:
         self.full_wavelet = self.engine.wavelet().copy()
         self.kernel: np.ndarray | None = None
         self.fmax: float | None = None
+        self._checked_obs = None  # (weakref to an observed tensor, its version) whose energies were checked
 
     # -- band --------------------------------------------------------------
     def set_band(self, fmax_hz: float | None) -> None:
@@ -194,6 +199,42 @@ class WavePhysics:
                                                         _dev_ptr(out), _stream()))
         return out
 
+    def _check_energy(self, observed, energy: torch.Tensor) -> None:
+        """The reference asserts that the observed energy is positive
+        (wave_wgpu.rs:853; NaN fails the assert too) and raises from the call.
+        Reading the device energies waits for the call, so the check runs once
+        per observed tensor (the same object at the same version has the same
+        energies); the engine gives such models NaN losses and gradients
+        either way."""
+        key = self._checked_obs
+        if isinstance(observed, torch.Tensor) and key is not None and key[0]() is observed \
+                and key[1] == observed._version:
+            return
+        if bool((~(energy > 0)).any()):
+            raise ValueError("Observed traces have zero energy")
+        if isinstance(observed, torch.Tensor):
+            self._checked_obs = (weakref.ref(observed), observed._version)
+
+    def loss_torch(self, fields_km: torch.Tensor, observed: torch.Tensor, traces_out: torch.Tensor | None = None,
+                   wavelets=None) -> torch.Tensor:
+        """Per-model normalised misfit alone (no adjoint), on the device: the
+        losses loss_and_grad_torch returns, bit for bit, at the cost of the
+        forward."""
+        vel = (_device_f32(fields_km, self.device) * 1000.0).contiguous()
+        obs = _device_f32(observed, self.device)
+        B = vel.shape[0]
+        if obs.numel() != B * self.shots * self.nt * self.engine.nrec:
+            raise ValueError("observed length does not match shots * nt * nrec")
+        losses = torch.empty(B, dtype=torch.float32, device=self.device)
+        energy = torch.empty(B, dtype=torch.float64, device=self.device)
+        tr = _dev_ptr(traces_out) if traces_out is not None else None
+        w = None if wavelets is None else self._unit_wavelets(wavelets, B)
+        _lib.check(self.engine._lib.wg_loss_dev(self.engine.handle, B, _dev_ptr(vel),
+                                                None if w is None else _dev_ptr(w), _dev_ptr(obs),
+                                                _dev_ptr(losses), tr, _dev_ptr(energy), _stream()))
+        self._check_energy(observed, energy)
+        return losses
+
     def loss_and_grad_torch(self, fields_km: torch.Tensor, observed: torch.Tensor,
                             traces_out: torch.Tensor | None = None,
                             wavelets=None) -> tuple[torch.Tensor, torch.Tensor]:
@@ -218,8 +259,7 @@ class WavePhysics:
             _lib.check(lib.wg_loss_grad_wavelets_dev(h, B, _dev_ptr(vel), _dev_ptr(w), _dev_ptr(obs),
                                                      _dev_ptr(losses), _dev_ptr(grads), tr, _dev_ptr(energy),
                                                      _stream()))
-        if bool((energy <= 0).any()):
-            raise ValueError("Observed traces have zero energy")
+        self._check_energy(observed, energy)
         return losses, grads * 1000.0  # d/d(km/s) = 1000 d/d(m/s)
 
     def loss_and_grad_true_torch(self, fields_km: torch.Tensor, true_km: torch.Tensor
@@ -240,8 +280,7 @@ class WavePhysics:
         lib, h = self.engine._lib, self.engine.handle
         _lib.check(lib.wg_loss_grad_true_dev(h, B, _dev_ptr(vel), _dev_ptr(tvel), _dev_ptr(obs), _dev_ptr(losses),
                                              _dev_ptr(grads), None, _dev_ptr(energy), _stream()))
-        if bool((energy <= 0).any()):
-            raise ValueError("Observed traces have zero energy")
+        self._check_energy(obs, energy)
         return losses, grads * 1000.0, obs
 
     def loss_and_grad(self, fields_km, observed):
@@ -253,5 +292,9 @@ class WavePhysics:
         return losses.cpu().numpy(), grads.cpu().numpy()
 
     def misfit(self, fields_km: torch.Tensor, observed: torch.Tensor) -> torch.Tensor:
-        """Differentiable per-model losses [B] (torch.autograd.Function)."""
+        """Differentiable per-model losses [B] (torch.autograd.Function).
+        Without autograd (no_grad, or fields that need no gradient) only the
+        forward and the misfit run."""
+        if not (torch.is_grad_enabled() and fields_km.requires_grad):
+            return self.loss_torch(fields_km.detach(), observed)
         return WaveMisfit.apply(fields_km, _device_f32(observed, self.device), self)
