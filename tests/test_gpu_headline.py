@@ -21,14 +21,15 @@ misfit, and the last model (its units are the last tasks of both queues).
     gradient whatever the implementation.  Gates: loss 1e-5 + 10 kappa,
     gradient max(1e-3, 50 kappa).
   - The FP64-accumulated dot-product tests of the wavelet and the Born operator
-    at 5 shots and nt = 1000 on one model per family: <= 1e-6 with y = F x;
-    <= 1e-5 with y = F x' for an independent x'; <= 5e-5 with white-noise y.
-    The last two have cancellation in the inner product: for white noise y,
-    <J dv, y> of the float32 Born forward alone differs from the float64
-    forward's by 1e-4 relative (oracle, one 70 x 70 x 1000 shot), so float32
-    roundoff, not transposition, sets their level; the float32 oracle pair
-    (restated forward and WGSL adjoint) shows 1.5-3.4e-6 and up to 1.5e-5 on
-    the same cases.
+    at 5 shots and nt = 1000 on one model per family: with y = F x,
+    |<Fx, y> - <x, F^T y>| <= 1e-6 |<Fx, y>|.  With y = F x' for an
+    independent x', or white-noise y, <Fx, y> is a sum with heavy cancellation
+    (100x for F x' here; for white noise <J dv, y> of the float32 Born forward
+    alone differs from the float64 forward's by 1e-4 relative, oracle, one
+    70 x 70 x 1000 shot), so float32 roundoff, not transposition, sets the
+    plain relative difference; those cases are gated on the difference over
+    ||Fx|| ||y|| (the Cauchy-Schwarz scale of the product) <= 1e-6, and the
+    plain relative difference is printed.
   - Central finite differences with h = 1 m/s at 16 random visible cells of
     each checked model, each difference two float64 oracle losses against the
     observed gathers the GPU gradient was taken against (thread pool):
@@ -199,23 +200,24 @@ def test_headline_dot_products(run):
     x = _dev(rng.standard_normal((nb, S, NT)).astype(np.float32))
     x2 = _dev(rng.standard_normal((nb, S, NT)).astype(np.float32))
     fx = F(x)
-    cases = [("wavelet", fx, x, Ft, "y = F x", fx, 1e-6),
-             ("wavelet", fx, x, Ft, "y = F x'", F(x2), 1e-5),
-             ("wavelet", fx, x, Ft, "y white noise", _dev(rng.standard_normal((nb, S, NT, 70)).astype(np.float32)),
-              5e-5)]
+    white = lambda: _dev(rng.standard_normal((nb, S, NT, 70)).astype(np.float32))
+    cases = [("wavelet", fx, x, Ft, "y = F x", fx, False),
+             ("wavelet", fx, x, Ft, "y = F x'", F(x2), True),
+             ("wavelet", fx, x, Ft, "y white noise", white(), True)]
     dv = _dev(rng.standard_normal((nb, 70, 70)).astype(np.float32))
     dv2 = _dev(rng.standard_normal((nb, 70, 70)).astype(np.float32))
     jdv = J(dv)
-    cases += [("Born", jdv, dv, Jt, "y = J dv", jdv, 1e-6),
-              ("Born", jdv, dv, Jt, "y = J dv'", J(dv2), 1e-5),
-              ("Born", jdv, dv, Jt, "y white noise", _dev(rng.standard_normal((nb, S, NT, 70)).astype(np.float32)),
-               5e-5)]
-    for name, fx_, x_, adj, label, y, tol in cases:
+    cases += [("Born", jdv, dv, Jt, "y = J dv", jdv, False),
+              ("Born", jdv, dv, Jt, "y = J dv'", J(dv2), True),
+              ("Born", jdv, dv, Jt, "y white noise", white(), True)]
+    for name, fx_, x_, adj, label, y, normed in cases:
         lhs, rhs = _dot(fx_, y), _dot(x_, adj(y.contiguous()))
         rel = abs(lhs - rhs) / abs(lhs)
+        scale = float(fx_.double().norm() * y.double().norm())
+        normd = abs(lhs - rhs) / scale
         print(f"{name} dot test ({label}), 3 models x 5 shots x nt 1000: <Fx,y>={lhs:.12e} <x,F^T y>={rhs:.12e} "
-              f"rel={rel:.3e} (gate {tol:.0e})", flush=True)
-        assert rel <= tol
+              f"rel={rel:.3e}, over ||Fx|| ||y||: {normd:.3e}, cancellation {scale / abs(lhs):.1f}x", flush=True)
+        assert (normd if normed else rel) <= 1e-6
 
 
 def test_headline_finite_differences(run, orc, oracle_runs):
