@@ -142,6 +142,24 @@ int32_t wg_loss_dev(wg_engine *e, int32_t B, const float *vel_dev,
                     const float *wav_dev, const float *obs_dev, float *loss_dev,
                     float *traces_dev, double *energy_dev, void *stream);
 
+/* wg_loss_dev that allocates nothing (no reference counterpart: it is for
+ * callers that cannot allocate device memory, such as code under HIP graph
+ * capture, tight serving loops and harnesses that forbid hipMalloc).  Its
+ * kernels keep their working data (per-model loss scale, observed energy
+ * when energy_dev is NULL, the residual tree's partial sums) in one fixed
+ * device array of 4 MiB per GPU, held until the process ends: 4 + 8 bytes per
+ * model plus 4 bytes per 256 samples of its gathers, about 760 models of 5
+ * shots x nt 1000 x 70 receivers per call; larger batches return
+ * WG_E_TOO_LARGE with the limit in the message.  traces_dev [B, shots, nt, nrec] is required;
+ * resident grids only (WG_E_TOO_LARGE on an HBM-streamed engine).  Losses
+ * equal wg_loss_dev's bit for bit.  Every call on one GPU shares that GPU's
+ * array: calls that overlap on one GPU (other streams, other engines, other
+ * threads) race. */
+int32_t wg_loss_noalloc_dev(wg_engine *e, int32_t B, const float *vel_dev,
+                            const float *wav_dev, const float *obs_dev,
+                            float *loss_dev, float *traces_dev,
+                            double *energy_dev, void *stream);
+
 /* Uneven batches (no reference counterpart; the reference solves one model
  * with the engine's own source list per call, wave_wgpu.rs:761-785).  Each
  * of the B models has `shots` unit slots; unit u = b * shots + k carries its

@@ -63,6 +63,7 @@ SIGNATURES = {
     "wg_loss_grad_true_dev": (_i32, [_vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "wg_loss_grad_wavelets_dev": (_i32, [_vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "wg_loss_dev": (_i32, [_vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "wg_loss_noalloc_dev": (_i32, [_vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "wg_source_corners": (_i32, [_vp, _i32, _vp, _vp, _vp]),
     "wg_loss_grad_units_dev": (_i32, [_vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "wg_simulate_units_dev": (_i32, [_vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),

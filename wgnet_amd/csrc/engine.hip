@@ -74,6 +74,39 @@ void corners(float x, float z, float h, int pad, int nx, int nz, int cell[4], fl
 
 }  // namespace
 
+// Fixed device scratch of wg_loss_noalloc_dev (the per-model loss scale, the
+// observed energy when the caller passes none, and the residual tree's
+// partial sums).  Each device holds its own copy for the life of the process;
+// host code reaches it only through the address hipGetSymbolAddress returns
+// for the current device (scratch_setup below), never through this name.
+constexpr size_t kScratchBytes = (size_t)4 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxDevices] = {};  // device address of `scratch` per device
+
+// Looks up (once per device, with that device current) the device address of
+// `scratch`.
+int scratch_setup(int dev, void** out) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (dev < 0 || dev >= kMaxDevices) return fail(WG_E_DEVICE, "device index out of range for the scratch table");
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        HIPCK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(WG_E_DEVICE, "hipGetSymbolAddress returned a null scratch address");
+        g_scratch_dev[dev] = p;
+    }
+    *out = g_scratch_dev[dev];
+    return WG_OK;
+}
+
+size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+}  // namespace
+
 struct TimedSpan {
     hipEvent_t a, b;
     int64_t n;  // kernel launches between a and b
@@ -119,6 +152,7 @@ struct wg_engine {
     // null for the engine wavelet (set by WavScope under the engine mutex)
     const float* wav_dev = nullptr;
     int tile = 0;  // tile shape of the HBM-streamed kernels (wg::StepParams::tile)
+    void* scratch_ptr = nullptr;  // this device's address of the fixed scratch array
     // per-unit source corners and activity mask of the call in progress
     // (uneven batches), or null (set by UnitScope under the engine mutex)
     const int* unit_src_cell = nullptr;
@@ -797,7 +831,12 @@ int32_t wg_engine_create(const wg_config* cfg, wg_engine** out) {
     if (!resident && (size_t)g.nx * g.nz > ((size_t)1 << 28))
         return fail(WG_E_TOO_LARGE, "grid of " + std::to_string((size_t)g.nx * g.nz) + " cells is too large");
 
+    void* scr = nullptr;
+    int src = scratch_setup(dev, &scr);
+    if (src) return src;
+
     wg_engine* e = new wg_engine();
+    e->scratch_ptr = scr;
     e->streamed = !resident;
     e->cfg = c;
     e->g = g;
@@ -1112,6 +1151,49 @@ int32_t wg_loss_dev(wg_engine* e, int32_t B, const float* vel_dev, const float* 
     if ((rc = run_chunk_forward(e, 0, B, vel_dev, traces, false, st))) return rc;
     LAUNCH(wg::launch_residual(traces, obs_dev, scale, per, B, (float*)e->resid.p, (float*)e->partials.p, st));
     LAUNCH(wg::launch_loss_sum((const float*)e->partials.p, chunks, B, energy, loss_dev, st));
+    return WG_OK;
+}
+
+int32_t wg_loss_noalloc_dev(wg_engine* e, int32_t B, const float* vel_dev, const float* wav_dev,
+                            const float* obs_dev, float* loss_dev, float* traces_dev, double* energy_dev,
+                            void* stream) {
+    if (!e || !vel_dev || !obs_dev || !loss_dev || !traces_dev || B < 0) return fail(WG_E_INVALID, "invalid argument");
+    Guard G(e);
+    WavScope W(e, wav_dev);
+    HIPCK(hipSetDevice(e->device));
+    if (e->streamed)
+        return fail(WG_E_TOO_LARGE, "wg_loss_noalloc_dev runs the resident kernels only (grid too large for one "
+                                    "workgroup)");
+    if (!e->scratch_ptr) return fail(WG_E_DEVICE, "the fixed scratch of this device was not set up");
+    const wg::Grid& g = e->g;
+    const int S = e->cfg.shots;
+    const long long per = (long long)S * g.nt * g.nrec;
+    const long long chunks = (per + 255) / 256;
+    // scratch layout: scale [B] f32 | energy [B] f64 (when the caller passes
+    // none) | partials [B, chunks] f32, each part 256-byte aligned
+    const size_t off_e = align256((size_t)B * sizeof(float));
+    const size_t off_p = off_e + (energy_dev ? 0 : align256((size_t)B * sizeof(double)));
+    const size_t need = off_p + (size_t)B * chunks * sizeof(float);
+    if (need > kScratchBytes) {
+        const size_t per_model = sizeof(float) + (energy_dev ? 0 : sizeof(double)) + chunks * sizeof(float);
+        return fail(WG_E_TOO_LARGE, "wg_loss_noalloc_dev: " + std::to_string(B) + " models need " +
+                                        std::to_string(need) + " bytes of scratch; the fixed scratch holds " +
+                                        std::to_string(kScratchBytes) + " (about " +
+                                        std::to_string(kScratchBytes / per_model - 1) + " models of this engine)");
+    }
+    hipStream_t st = (hipStream_t)stream;
+    int rc;
+    if ((rc = fence(e, st))) return rc;
+    if (B == 0) return WG_OK;
+    char* base = (char*)e->scratch_ptr;
+    float* scale = (float*)base;
+    double* energy = energy_dev ? energy_dev : (double*)(base + off_e);
+    float* partials = (float*)(base + off_p);
+    LAUNCH(wg::launch_energy(obs_dev, per, B, energy, scale, st));
+    wg::FwdParams p = fwd_params(e, vel_dev, traces_dev);
+    LAUNCH(wg::launch_forward(p, B * S, false, false, st));
+    LAUNCH(wg::launch_residual(traces_dev, obs_dev, scale, per, B, nullptr, partials, st));
+    LAUNCH(wg::launch_loss_sum(partials, chunks, B, energy, loss_dev, st));
     return WG_OK;
 }
 

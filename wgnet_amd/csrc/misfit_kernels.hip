@@ -49,7 +49,7 @@ __global__ __launch_bounds__(kEnergyThreads) void k_energy(const float* obs, lon
 
 // RESIDUAL_WGSL (wave_wgpu.rs:28-53): one 256-wide workgroup per chunk of
 // 256 elements of one model's [shots, nt, nrec] gathers; residual (p-d)*scale
-// and the float32 tree sum of 0.5 d^2 scale.
+// (not written when resid is null) and the float32 tree sum of 0.5 d^2 scale.
 __global__ __launch_bounds__(256) void k_residual(const float* pred, const float* obs,
                                                   const float* scale, long long per,
                                                   long long chunks, float* resid,
@@ -64,7 +64,7 @@ __global__ __launch_bounds__(256) void k_residual(const float* pred, const float
     if (i < per) {
         const size_t o = (size_t)b * per + i;
         const float d = pred[o] - obs[o];
-        resid[o] = d * sc;
+        if (resid) resid[o] = d * sc;  // null: the loss alone
         e = 0.5f * d * d * sc;
     }
     wg_sums[threadIdx.x] = e;
