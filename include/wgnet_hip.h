@@ -221,6 +221,20 @@ int32_t wg_set_snapshot_budget(wg_engine *e, uint64_t bytes);
 /* 1 when the engine runs the HBM-streamed kernels (grid too large for the
  * resident ones), 0 otherwise. */
 int32_t wg_engine_streamed(const wg_engine *e);
+/* Row bands (no reference counterpart; the reference's grid is fixed at
+ * 70 x 70, wave_wgpu.rs:714-734): a grid too large for one workgroup whose
+ * units split into 2 to 4 row bands that each fit (the 100 x 100 and
+ * 141 x 141 models) runs forward and adjoint on the band kernels: one
+ * workgroup per band holds its rows in LDS and registers for all nt steps,
+ * with every snapshot stored, and exchanges two edge rows with each
+ * neighbour band per step through global memory.  wg_engine_bands returns
+ * the bands per unit (0: the grid does not split, or fits one workgroup).
+ * Results equal the HBM-streamed kernels' bit for bit; wg_set_band_split(e,
+ * 0) selects those instead.  The band kernels serve simulate and the loss
+ * and gradient when a model's snapshots fit the snapshot budget; the Born
+ * operator, the wavelet gradient and checkpointed adjoints stay streamed. */
+int32_t wg_engine_bands(const wg_engine *e);
+int32_t wg_set_band_split(wg_engine *e, int32_t on);
 /* Tile shapes of the HBM-streamed kernels (no reference counterpart): a
  * 256-thread workgroup computes 64x16 (0, default), 128x8 (1), 32x32 (2) or
  * 64x32 (3) cells.  Results do not depend on the shape; an autotuner times

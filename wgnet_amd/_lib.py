@@ -73,6 +73,8 @@ SIGNATURES = {
     "wg_set_snapshot_budget": (_i32, [_vp, ctypes.c_uint64]),
     "wg_engine_streamed": (_i32, [_vp]),
     "wg_stream_tiles": (_i32, []),
+    "wg_engine_bands": (_i32, [_vp]),
+    "wg_set_band_split": (_i32, [_vp, _i32]),
     "wg_illumination_dev": (_i32, [_vp, _i32, _vp, _vp, _vp]),
     "wg_set_overlap": (_i32, [_vp, _i32]),
     "wg_set_time_block": (_i32, [_vp, _i32]),

@@ -144,6 +144,13 @@ struct wg_engine {
     int time_block = 1;            // adjoint steps per streamed launch (1 or 2)
     int qparts = 4;                // time parts per unit of the resident work queues (0: chunk pipeline)
     int qspin = 1 << 24;           // polls before a work-queue wait gives up
+    // row bands: grids too large for one workgroup whose units split into
+    // bands that fit run the band kernels (one workgroup per band, edge rows
+    // exchanged every step) instead of the per-step HBM-streamed kernels
+    int bands = 0;                 // bands per unit (0: the grid does not split)
+    int band_bounds[wg::kMaxBands + 1] = {};
+    bool band_on = true;
+    DevBuf bxbuf, bqflags;         // exchange rows; [next, err, flags per band]
     DevBuf qstate, qflags;         // work queues: per-unit part state; counters, error flag, done flags
     hipStream_t last = nullptr;    // stream of the last enqueued call
     bool last_valid = false;
@@ -384,8 +391,74 @@ struct Timed {
 int stream_simulate(wg_engine* e, int b0, int nb, const float* vel, const float* wav, long long wav_stride,
                     float* traces, hipStream_t st);
 
+bool use_bands(const wg_engine* e) { return e->streamed && e->bands > 1 && e->band_on; }
+
+// Buffers and counters of a band launch over `units` units: exchange rows,
+// and [next, err, per-band flags].  Clears err when `clear_err`.
+int band_setup(wg_engine* e, int units, bool clear_err, hipStream_t st, wg::BandLaunch* bl) {
+    const int P = e->bands;
+    int rc;
+    if ((rc = ensure(e, e->bxbuf, wg::band_xbuf_floats(e->g, units, P) * sizeof(float)))) return rc;
+    const size_t nq = 2 + (size_t)units * P;
+    if ((rc = ensure(e, e->bqflags, nq * sizeof(int)))) return rc;
+    int* q = (int*)e->bqflags.p;
+    if (clear_err) {
+        LAUNCH(wg::launch_zero((float*)q, nq, st));  // int 0 = float +0 bits
+    } else {
+        LAUNCH(wg::launch_zero((float*)q, 1, st));
+        LAUNCH(wg::launch_zero((float*)(q + 2), nq - 2, st));
+    }
+    bl->P = P;
+    for (int i = 0; i <= wg::kMaxBands; ++i) bl->bounds[i] = e->band_bounds[i];
+    bl->xbuf = (float*)e->bxbuf.p;
+    bl->next = q;
+    bl->err = q + 1;
+    bl->flags = q + 2;
+    bl->spin = e->qspin < 0 ? -1 : std::min(e->qspin, 1 << 22);
+    return WG_OK;
+}
+
+// Forward of models [b0, b0 + nb) on the band kernels: gathers, and with
+// `store` every snapshot (indexed by the unit's position in the chunk).
+int band_forward(wg_engine* e, int b0, int nb, const float* vel, const float* wav, long long wav_stride,
+                 float* traces, bool store, bool clear_err, hipStream_t st) {
+    const int units = nb * e->cfg.shots;
+    wg::BandLaunch bl{};
+    int rc;
+    if ((rc = band_setup(e, units, clear_err, st, &bl))) return rc;
+    wg::FwdParams p = fwd_params(e, vel, traces);
+    p.wavelet = wav;
+    p.wav_stride = wav_stride;
+    p.unit0 = b0 * e->cfg.shots;
+    p.snaps = store ? (float*)e->snaps.p : nullptr;
+    Timed tm(e, &e->ev_fwd, st);
+    LAUNCH(wg::launch_forward_band(p, units, store, bl, st));
+    return WG_OK;
+}
+
+int band_adjoint(wg_engine* e, int b0, int nb, const float* vel, const float* resid, float* gunit, hipStream_t st) {
+    const int units = nb * e->cfg.shots;
+    wg::BandLaunch bl{};
+    int rc;
+    if ((rc = band_setup(e, units, false, st, &bl))) return rc;
+    wg::AdjParams p = adj_params(e, vel, resid, gunit);
+    p.unit0 = b0 * e->cfg.shots;
+    p.snaps = (const float*)e->snaps.p;
+    Timed tm(e, &e->ev_adj, st);
+    LAUNCH(wg::launch_adjoint_band(p, units, bl, st));
+    return WG_OK;
+}
+
+// Non-storing band forward of models [b0, b0 + nb); a wait that gave up
+// leaves NaN gathers in the units concerned.
+int band_simulate(wg_engine* e, int b0, int nb, const float* vel, const float* wav, long long wav_stride,
+                  float* traces, hipStream_t st) {
+    return band_forward(e, b0, nb, vel, wav, wav_stride, traces, false, true, st);
+}
+
 int run_chunk_forward(wg_engine* e, int b0, int nb, const float* vel, float* traces, bool store, hipStream_t st,
                       size_t snap_unit_off = 0) {
+    if (use_bands(e) && !store) return band_simulate(e, b0, nb, vel, wav_ptr(e), wav_stride(e), traces, st);
     if (e->streamed && !store) return stream_simulate(e, b0, nb, vel, wav_ptr(e), wav_stride(e), traces, st);
     wg::FwdParams p = fwd_params(e, vel, traces);
     p.unit0 = b0 * e->cfg.shots;
@@ -656,6 +729,23 @@ int loss_grad_core(wg_engine* e, int B, const float* vel, const float* obs, floa
         traces = (float*)e->traces.p;
     }
     if (compute_energy) LAUNCH(wg::launch_energy(obs, per, B, energy, scale, st));
+    if (use_bands(e) && snap_bytes_per_unit(e) * S <= e->snap_budget) {
+        // every snapshot of a chunk of models in memory, the units of the
+        // chunk as bands: forward, residual, adjoint
+        const int mc = models_per_chunk(e, B);
+        if ((rc = ensure(e, e->snaps, snap_bytes_per_unit(e) * S * mc))) return rc;
+        for (int b0 = 0; b0 < B; b0 += mc) {
+            const int nb = std::min(mc, B - b0);
+            if ((rc = band_forward(e, b0, nb, vel, wav_ptr(e), wav_stride(e), traces, true, b0 == 0, st))) return rc;
+            LAUNCH(wg::launch_residual(traces + b0 * per, obs + b0 * per, scale + b0, per, nb,
+                                       (float*)e->resid.p + b0 * per, (float*)e->partials.p + b0 * chunks, st));
+            if ((rc = band_adjoint(e, b0, nb, vel, (const float*)e->resid.p, (float*)e->gunit.p, st))) return rc;
+        }
+        LAUNCH(wg::launch_loss_sum((const float*)e->partials.p, chunks, B, energy, loss, st));
+        LAUNCH(wg::launch_fold(g, (const float*)e->gunit.p, vel, energy, B, grad, st));
+        LAUNCH(wg::launch_flag_nan((const int*)e->bqflags.p + 1, loss, B, grad, (long long)g.nvx * g.nvz, st));
+        return WG_OK;
+    }
     if (e->streamed) {
         const StreamPlan P = plan_stream(e, B, true);
         if ((rc = stream_alloc(e, P, P.models * S))) return rc;
@@ -851,6 +941,14 @@ int32_t wg_engine_create(const wg_config* cfg, wg_engine** out) {
     }
     build_sources(e);
     build_receivers(e);
+    if (!resident) {
+        wg::Geometry gh{};
+        gh.band_z0 = e->band_z0;
+        gh.band_rows = e->band_rows;
+        gh.n_ent = (int)e->ent_r.size();
+        int P = 0;
+        if (wg::band_plan(g, gh, lds_limit, &P, e->band_bounds)) e->bands = P;
+    }
     e->prof.resize(3 * (g.nx + g.nz));
     wg_cpml_profiles(g.nx, c.h, c.dt, c.pml, c.vmax, c.peak_hz, e->prof.data(), e->prof.data() + g.nx,
                      e->prof.data() + 2 * g.nx);
@@ -904,7 +1002,8 @@ void wg_engine_destroy(wg_engine* e) {
                     (void*)e->d_wavelet})
         if (p) (void)hipFree(p);
     for (DevBuf* b : {&e->snaps, &e->traces, &e->resid, &e->gunit, &e->partials, &e->scale, &e->energy, &e->vel,
-                      &e->obs, &e->loss, &e->grad, &e->fstate, &e->astate, &e->ckpt, &e->qstate, &e->qflags})
+                      &e->obs, &e->loss, &e->grad, &e->fstate, &e->astate, &e->ckpt, &e->qstate, &e->qflags,
+                      &e->bxbuf, &e->bqflags})
         if (b->p) (void)hipFree(b->p);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     if (e->side) {
@@ -1316,6 +1415,7 @@ static int32_t simulate_wavelets_locked(wg_engine* e, int32_t B, const float* ve
     int rc;
     if ((rc = fence(e, st))) return rc;
     if (B == 0) return WG_OK;
+    if (use_bands(e)) return band_simulate(e, 0, B, vel_dev, wav_dev, e->g.nt, traces_dev, st);
     if (e->streamed) return stream_simulate(e, 0, B, vel_dev, wav_dev, e->g.nt, traces_dev, st);
     wg::FwdParams p = fwd_params(e, vel_dev, traces_dev);
     p.wavelet = wav_dev;
@@ -1382,6 +1482,15 @@ int32_t wg_illumination_dev(wg_engine* e, int32_t B, const float* vel_dev, float
 }
 
 int32_t wg_stream_tiles(void) { return wg::kStreamTiles; }
+
+int32_t wg_engine_bands(const wg_engine* e) { return e ? e->bands : -1; }
+
+int32_t wg_set_band_split(wg_engine* e, int32_t on) {
+    if (!e) return fail(WG_E_INVALID, "null engine");
+    Guard G(e);
+    e->band_on = on != 0;
+    return WG_OK;
+}
 
 int32_t wg_set_overlap(wg_engine* e, int32_t on) {
     if (!e) return fail(WG_E_INVALID, "null engine");

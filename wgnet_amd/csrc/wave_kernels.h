@@ -112,6 +112,29 @@ struct StepParams {
 };
 constexpr int kStreamTiles = 4;
 
+// Band launches of the resident kernels: a unit whose grid does not fit one
+// workgroup is cut into P row bands, one workgroup each, which exchange their
+// edge rows after every step (wave_kernels.hip).  A workgroup takes its
+// (unit, band) from a device counter when it starts, so a unit's bands start
+// one after another and bands that wait for each other run together.
+constexpr int kMaxBands = 4;
+struct BandLaunch {
+    int P;
+    int bounds[kMaxBands + 1];  // band p holds the inner rows [bounds[p], bounds[p + 1])
+    float* xbuf;                // band_xbuf_floats(g, units, P) floats
+    int* flags;                 // units * P ints, zero at launch
+    int* next;                  // task counter, zero at launch
+    int* err;                   // set to 1 when a wait gives up
+    int spin;                   // polls before a wait gives up (< 0: at once)
+};
+// The split of a grid too large for one workgroup into bands that fit (P in
+// [2, kMaxBands], boundaries in the interior rows at least one adjoint strip
+// width from either edge, the receiver rows in the top band), or false.
+bool band_plan(const Grid& g, const Geometry& geo, int lds_limit, int* P, int bounds[kMaxBands + 1]);
+size_t band_xbuf_floats(const Grid& g, int units, int P);
+hipError_t launch_forward_band(const FwdParams& p, int units, bool store, const BandLaunch& b, hipStream_t st);
+hipError_t launch_adjoint_band(const AdjParams& p, int units, const BandLaunch& b, hipStream_t st);
+
 hipError_t launch_zero(float* p, size_t n, hipStream_t st);
 hipError_t launch_illum(const Grid& g, const float* snaps, int nb, float* out, hipStream_t st);
 hipError_t launch_causal_fir(const float* x, long long nseries, int nt, int nrec, const float* h, int L, float* y,

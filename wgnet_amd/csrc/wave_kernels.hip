@@ -15,11 +15,27 @@
 //          neighbours read; C-PML profiles; receiver tables.
 //   HBM:   gathers, and snapshots 2 v dt^2 w in inner row-major order.
 //
+// Row bands
+//   A unit too large for one workgroup (the 100 x 100 and 141 x 141 models:
+//   144 x 144 and 185 x 185 padded) is cut into P row bands, one workgroup
+//   each, all P running at once.  A band holds its own rows and two halo rows
+//   above and below in LDS; its cells' state stays in its registers.  The
+//   band boundaries lie in the interior rows, away from the z C-PML strips,
+//   so only the field (forward u, adjoint V = v^2 dt^2 lambda) crosses a
+//   boundary: after each step a band sends its two edge rows to its
+//   neighbours through global memory and takes theirs (see Xchg).  The
+//   arithmetic per cell is the whole-grid kernel's, so results are the same
+//   bits.  Bands run from a device work queue, one workgroup per compute
+//   unit, taking a unit's P bands in a row, so the bands that wait for each
+//   other are always running together.
+//
 // Arithmetic follows the reference operation by operation with no FMA
 // contraction (-ffp-contract=off and the pragma below), so the float32 gathers
 // equal the CPU solver's bit for bit:
 //   forward  crates/geology-core/src/lib.rs:145-220 (== splatForward.wgsl:48-121)
 //   adjoint  crates/wave-gauss-rs/src/shaders/splatAdjoint.wgsl:58-131
+#include <algorithm>
+
 #include "wave_kernels.h"
 
 #pragma clang fp contract(off)
@@ -89,10 +105,29 @@ __host__ __device__ inline void axis_set(int n, int r, bool band, int& len, int&
     }
 }
 
-__host__ __device__ inline Cls make_cls(const Grid& g, int r, bool bx, bool bz) {
+// The rows of a class restricted to the inner rows [lo, hi) of a band: the
+// band rows ([0, r) and [n - r, n)) or the interior run, each cut to [lo, hi).
+// For lo = 0, hi = n it is axis_set.
+__host__ __device__ inline void rows_set(int n, int r, bool band, int lo, int hi, int& len, int& split, int& o0,
+                                         int& o1) {
+    if (band && 2 * r >= n) {
+        const int a = lo > 0 ? lo : 0, b = hi < n ? hi : n;
+        len = b > a ? b - a : 0; split = len; o0 = a; o1 = a;
+    } else if (band) {
+        const int a0 = lo > 0 ? lo : 0, a1 = hi < r ? hi : r;
+        const int b0 = lo > n - r ? lo : n - r, b1 = hi < n ? hi : n;
+        const int la = a1 > a0 ? a1 - a0 : 0, lb = b1 > b0 ? b1 - b0 : 0;
+        len = la + lb; split = la; o0 = a0; o1 = b0 - la;
+    } else {
+        const int a = lo > r ? lo : r, b = hi < n - r ? hi : n - r;
+        len = (2 * r >= n || b <= a) ? 0 : b - a; split = len; o0 = a; o1 = a;
+    }
+}
+
+__host__ __device__ inline Cls make_cls(const Grid& g, int r, bool bx, bool bz, int lo = 0, int hi = -1) {
     Cls c;
     axis_set(g.ni, r, bx, c.w, c.sx, c.ox0, c.ox1);
-    axis_set(g.nzi, r, bz, c.h, c.sz, c.oz0, c.oz1);
+    rows_set(g.nzi, r, bz, lo, hi < 0 ? g.nzi : hi, c.h, c.sz, c.oz0, c.oz1);
     if (c.w == 0) { c.w = 1; c.h = 0; }
     c.dZ = NT / c.w;
     c.dX = NT - c.dZ * c.w;
@@ -101,12 +136,14 @@ __host__ __device__ inline Cls make_cls(const Grid& g, int r, bool bx, bool bz) 
 
 __host__ __device__ inline int cls_count(const Cls& c) { return c.w * c.h; }
 
-// Class and in-class index of inner cell (xi, zi) for region band r.
-__host__ __device__ inline void cell_class(const Grid& g, int r, int xi, int zi, int& cls, int& q) {
+// Class and in-class index of inner cell (xi, zi) for region band r, among
+// the cells of the inner rows [lo, hi) (the cell's own band).
+__host__ __device__ inline void cell_class(const Grid& g, int r, int xi, int zi, int& cls, int& q, int lo = 0,
+                                           int hi = -1) {
     const bool bx = xi < r || xi >= g.ni - r;
     const bool bz = zi < r || zi >= g.nzi - r;
     cls = (bx ? 1 : 0) + (bz ? 2 : 0);  // 0 I, 1 X, 2 Z, 3 C
-    const Cls c = make_cls(g, r, bx, bz);
+    const Cls c = make_cls(g, r, bx, bz, lo, hi);
     const int col = xi - (xi < c.sx + c.ox0 ? c.ox0 : c.ox1);
     const int row = zi - (zi < c.sz + c.oz0 ? c.oz0 : c.oz1);
     q = row * c.w + col;
@@ -125,9 +162,42 @@ __device__ __forceinline__ int cls_xi(const Cls& c, int col) { return col + (col
 __device__ __forceinline__ int cls_zi(const Cls& c, int row) { return row + (row < c.sz ? c.oz0 : c.oz1); }
 
 // ---------------------------------------------------------------------------
+// The rows a workgroup holds.  The whole grid is the band [0, nzi) with both
+// z-strip sides and no neighbours; band p of a split unit holds the inner
+// rows [lo, hi) plus two halo rows on each side (padded rows [lo, hi + 4)),
+// and the z-strip side its rows reach (band boundaries lie in the interior,
+// at least one strip width from either edge).
+// ---------------------------------------------------------------------------
+struct Band {
+    int lo, hi;     // inner rows owned
+    int zb0, rows;  // padded rows held in LDS: [zb0, zb0 + rows)
+    int top, bot;   // z-strip sides held (rows [0, sw), rows [nz - sw, nz))
+    int up, down;   // a neighbour band above / below
+};
+
+__host__ __device__ inline Band whole_band(const Grid& g) {
+    return Band{0, g.nzi, 0, g.nz, 1, 1, 0, 0};
+}
+
+__host__ __device__ inline Band split_band(const Grid& g, const int* bounds, int P, int p) {
+    Band b;
+    b.lo = bounds[p];
+    b.hi = bounds[p + 1];
+    b.zb0 = b.lo;  // padded row of inner row lo - 2
+    b.rows = b.hi - b.lo + 4;
+    b.top = p == 0;
+    b.bot = p == P - 1;
+    b.up = p > 0;
+    b.down = p < P - 1;
+    return b;
+}
+
+// ---------------------------------------------------------------------------
 // LDS layouts (floats).
-//   x-strips: rows [2, nz-2), columns [0, sw) and [nx-sw, nx): 2*sw per row.
-//   z-strips: rows [0, sw) and [nz-sw, nz), columns [2, nx-2): ni per row.
+//   field: the band's padded rows [zb0, zb0 + rows), nx per row.
+//   x-strips: the band's inner rows, columns [0, sw) and [nx-sw, nx): 2*sw per row.
+//   z-strips: rows [0, sw) (top side) and [nz-sw, nz) (bottom side), columns
+//   [2, nx-2): ni per row; a band holds the sides it reaches.
 // A cell lying in both halves (grids narrower than 2*sw) is written to both.
 // ---------------------------------------------------------------------------
 struct FwdLayout {
@@ -137,24 +207,24 @@ struct AdjLayout {
     int V, TX, UX, TZ, UZ, PR, RES, SRC, BOFF, ER, EW, total;
 };
 
-__host__ __device__ inline FwdLayout fwd_layout(const Grid& g) {
+__host__ __device__ inline FwdLayout fwd_layout(const Grid& g, const Band& bd) {
     FwdLayout L;
     L.U = 0;
-    L.AX = L.U + g.nx * g.nz;
-    L.AZ = L.AX + 2 * g.swf * g.nzi;
-    L.PR = L.AZ + 2 * g.swf * g.ni;
+    L.AX = L.U + g.nx * bd.rows;
+    L.AZ = L.AX + 2 * g.swf * (bd.hi - bd.lo);
+    L.PR = L.AZ + (bd.top + bd.bot) * g.swf * g.ni;
     L.total = L.PR + 3 * (g.nx + g.nz);
     return L;
 }
 
-__host__ __device__ inline AdjLayout adj_layout(const Grid& g, const Geometry& geo) {
+__host__ __device__ inline AdjLayout adj_layout(const Grid& g, const Geometry& geo, const Band& bd) {
     AdjLayout L;
     L.V = 0;
-    L.TX = L.V + g.nx * g.nz;
-    L.UX = L.TX + 2 * g.swa * g.nzi;
-    L.TZ = L.UX + 2 * g.swa * g.nzi;
-    L.UZ = L.TZ + 2 * g.swa * g.ni;
-    L.PR = L.UZ + 2 * g.swa * g.ni;
+    L.TX = L.V + g.nx * bd.rows;
+    L.UX = L.TX + 2 * g.swa * (bd.hi - bd.lo);
+    L.TZ = L.UX + 2 * g.swa * (bd.hi - bd.lo);
+    L.UZ = L.TZ + (bd.top + bd.bot) * g.swa * g.ni;
+    L.PR = L.UZ + (bd.top + bd.bot) * g.swa * g.ni;
     L.RES = L.PR + 3 * (g.nx + g.nz);
     L.SRC = L.RES + g.nrec;  // 4 lambdas + 4 coefficients
     L.BOFF = L.SRC + 8;
@@ -164,8 +234,10 @@ __host__ __device__ inline AdjLayout adj_layout(const Grid& g, const Geometry& g
     return L;
 }
 
-size_t forward_lds_bytes(const Grid& g) { return 4ull * fwd_layout(g).total; }
-size_t adjoint_lds_bytes(const Grid& g, const Geometry& geo) { return 4ull * adj_layout(g, geo).total; }
+size_t forward_lds_bytes(const Grid& g) { return 4ull * fwd_layout(g, whole_band(g)).total; }
+size_t adjoint_lds_bytes(const Grid& g, const Geometry& geo) {
+    return 4ull * adj_layout(g, geo, whole_band(g)).total;
+}
 
 // Slot counts per class of a compiled variant.
 struct Tuple {
@@ -194,6 +266,163 @@ int slots_needed(const Grid& g) {
     return (a < 0 || b < 0) ? -1 : (a > b ? a : b);
 }
 
+// Slot counts of the band kernels, the largest that compile without spills:
+// the 100 x 100 model (144 x 144 padded) in two bands, the 141 x 141 model
+// (185 x 185) in four.
+static const Tuple kBandTuples[] = {{9, 5, 5, 3}, {12, 5, 7, 3}};
+constexpr int kNumBandTuples = 2;
+
+// The smallest band tuple that holds every band's classes, forward and
+// adjoint, or -1.
+static int pick_band_tuple(const Grid& g, int P, const int* bounds) {
+    for (int i = 0; i < kNumBandTuples; ++i) {
+        const Tuple& t = kBandTuples[i];
+        const int have[4] = {t.kI, t.kX, t.kZ, t.kC};
+        bool ok = true;
+        for (int p = 0; p < P && ok; ++p)
+            for (int r : {g.region - 2, g.region_adj - 2})
+                for (int c = 0; c < 4 && ok; ++c)
+                    ok = cls_count(make_cls(g, r, c & 1, c & 2, bounds[p], bounds[p + 1])) <= have[c] * NT;
+        if (ok) return i;
+    }
+    return -1;
+}
+
+// Boundaries for P bands whose P - 2 middle bands have m rows each and the
+// top and bottom bands the rest, evenly.
+static void band_bounds(int nzi, int P, int m, int* bounds) {
+    const int edge = nzi - (P - 2) * m, top = edge / 2;
+    bounds[0] = 0;
+    bounds[1] = top;
+    for (int p = 2; p < P; ++p) bounds[p] = bounds[p - 1] + m;
+    bounds[P] = nzi;
+}
+
+bool band_plan(const Grid& g, const Geometry& geo, int lds_limit, int* Pout, int bounds[kMaxBands + 1]) {
+    if (g.nrec > NT || g.ni >= 511) return false;
+    // receiver corners and the residual CSR rows: inner rows of the top band
+    int rec_lo = g.nzi, rec_hi = -1;
+    if (geo.band_rows > 0) {
+        rec_lo = geo.band_z0 - 2;
+        rec_hi = geo.band_z0 + geo.band_rows - 1 - 2;
+    }
+    for (int P = 2; P <= kMaxBands; ++P) {
+        for (int m = (g.nzi + P - 1) / P; m >= 4; --m) {  // middle bands from the even split down
+            band_bounds(g.nzi, P, m, bounds);
+            bool ok = true;
+            for (int p = 1; p < P && ok; ++p)  // boundaries clear of the z-strips, bands of >= 4 rows
+                ok = bounds[p] >= g.swa && bounds[p] <= g.nzi - g.swa && bounds[p] - bounds[p - 1] >= 4;
+            ok = ok && bounds[P] - bounds[P - 1] >= 4 && rec_lo >= 0 && rec_hi < bounds[1];
+            for (int p = 0; p < P && ok; ++p) {
+                const Band bd = split_band(g, bounds, P, p);
+                const size_t f = 4ull * fwd_layout(g, bd).total, a = 4ull * adj_layout(g, geo, bd).total;
+                ok = f + 64 <= (size_t)lds_limit && a + 64 <= (size_t)lds_limit;
+            }
+            if (ok && pick_band_tuple(g, P, bounds) >= 0) {
+                *Pout = P;
+                return true;
+            }
+            if (P == 2) break;  // two bands have no middle band to shrink
+        }
+    }
+    return false;
+}
+
+size_t band_xbuf_floats(const Grid& g, int units, int P) { return (size_t)units * P * 2 * 2 * 2 * g.nx; }
+
+// ---------------------------------------------------------------------------
+// Halo exchange between the bands of a split unit.  After a step each band
+// stores its two edge rows of the field into its slot of `buf` (two slots,
+// by step parity), then its count of published steps into its flag; it waits
+// until each neighbour's count reaches the same step and loads the
+// neighbour's edge rows into its halo rows.  Rows and flags move with
+// relaxed agent-scope atomic stores and loads, which bypass the compute
+// unit's L1 and are coherent across the device; the wait for the row stores
+// to complete (s_waitcnt) before the flag store orders them, so a band that
+// reads a neighbour's count reads the rows stored before it.  (Release and
+// acquire fences order them too, but write back and invalidate the whole L2
+// of the XCD at every step: several microseconds more per step.)  Wave 0
+// does it; the other waves wait at the barrier that follows.  A neighbour
+// overwrites a slot two steps later, after waiting for this band's count of
+// the step in between, which this band publishes only after reading the slot.
+// ---------------------------------------------------------------------------
+struct Xchg {
+    float* buf;  // [launch units][P][2 sides (top rows, bottom rows)][2 parities][2 rows][nx]
+    int* flags;  // [launch units][P], published steps (zero at launch)
+    int* err;    // set by a wait that gives up
+    int P;
+    int spin;    // polls before a wait gives up (< 0: at once)
+};
+
+__device__ __forceinline__ float* xslot(const Xchg& X, int nx, int u, int p, int side, int par) {
+    return X.buf + ((((size_t)u * X.P + p) * 2 + side) * 2 + par) * 2 * nx;
+}
+
+// Exchange of field F (LDS rows of band bd, nx floats each) for unit u (its
+// position in the launch), band p, after step number k >= 1.  Every thread
+// calls it; returns false (in every thread) when a wait gave up.
+__device__ __forceinline__ bool band_exchange(const Xchg& X, float* F, const Band& bd, int nx, int u, int p, int k,
+                                              int* fail_s) {
+    const int par = k & 1, n2 = 2 * nx, lane = threadIdx.x;
+    if (lane < 64) {
+        if (bd.up) {
+            float* d = xslot(X, nx, u, p, 0, par);
+            const float* src = F + 2 * nx;  // the first two own rows
+            for (int j = lane; j < n2; j += 64) __hip_atomic_store(d + j, src[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (bd.down) {
+            float* d = xslot(X, nx, u, p, 1, par);
+            const float* src = F + (bd.hi - bd.lo) * nx;  // the last two own rows
+            for (int j = lane; j < n2; j += 64) __hip_atomic_store(d + j, src[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+        int ok = 1;
+        if (lane == 0) {
+            int* fl = X.flags + (size_t)u * X.P;
+            __hip_atomic_store(fl + p, k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            for (int q = p - 1; q <= p + 1; q += 2) {
+                if ((q < p && !bd.up) || (q > p && !bd.down)) continue;
+                for (int n = 0; __hip_atomic_load(fl + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < k; ++n) {
+                    if (n > X.spin) {
+                        ok = 0;
+                        __hip_atomic_fetch_or(X.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+                if (!ok) break;
+            }
+            *fail_s = !ok;
+        }
+        ok = __builtin_amdgcn_readfirstlane(ok);
+        if (ok && bd.up) {
+            const float* src = xslot(X, nx, u, p - 1, 1, par);
+            for (int j = lane; j < n2; j += 64) F[j] = __hip_atomic_load(src + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (ok && bd.down) {
+            const float* src = xslot(X, nx, u, p + 1, 0, par);
+            float* dst = F + (bd.hi - bd.lo + 2) * nx;
+            for (int j = lane; j < n2; j += 64) dst[j] = __hip_atomic_load(src + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+    __syncthreads();
+    return *fail_s == 0;
+}
+
+// Task of a band workgroup: one (unit, band) per workgroup, numbered in the
+// order the workgroups start (a device counter), so a unit's P bands go to P
+// workgroups that start one after another.  All units before the newest are
+// complete and run to their end, freeing compute units for the bands still
+// to start, so bands that wait for each other are always running together.
+// (A loop over tasks in one workgroup would cost registers: the compiler keeps
+// values live across tasks.  Out of line, the fetch costs the band kernels no
+// registers either: inlined, it cost them 40.)
+__device__ __noinline__ int band_task(int* next, int* ts) {
+    if (threadIdx.x == 0) *ts = __hip_atomic_fetch_add(next, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    return *ts;
+}
+
 // ===========================================================================
 // Forward
 // ===========================================================================
@@ -203,6 +432,7 @@ struct FCtx {
     float* AZ;
     const float *pax, *pbx, *pdbx, *paz, *pbz, *pdbz;
     int nx, nz, ni, reg, swf, sw2;
+    int zb0, lo, azb;  // band: first padded row in LDS, first inner row, bottom z-strip offset
     float rh, rh2, dt2;
 };
 
@@ -251,7 +481,7 @@ __device__ __forceinline__ void fwd_compute(FState<K, PX, PZ, BORN>& S, const Cl
         if (row >= c.h) continue;
         const int xi = cls_xi(c, col), zi = cls_zi(c, row);
         const int x = xi + 2, z = zi + 2;
-        const int i = z * C.nx + x;
+        const int i = (z - C.zb0) * C.nx + x;
         const float* U = C.U;
         const float u = U[i];
         float w = 0.0f;
@@ -262,7 +492,7 @@ __device__ __forceinline__ void fwd_compute(FState<K, PX, PZ, BORN>& S, const Cl
         if constexpr (PX) {
             const float d1x = (C1 * (up1x - um1x) - C2 * (up2x - um2x)) * C.rh;
             const float a = C.pax[x], bb = C.pbx[x];
-            const int base = zi * C.sw2 + (x < C.reg ? x : x - C.nx + C.sw2);
+            const int base = (zi - C.lo) * C.sw2 + (x < C.reg ? x : x - C.nx + C.sw2);
             const float ap1 = C.AX[base + 1], am1 = C.AX[base - 1];
             const float ap2 = C.AX[base + 2], am2 = C.AX[base - 2];
             const float d1_apsi = (C1 * (ap1 - am1) - C2 * (ap2 - am2)) * C.rh;
@@ -282,7 +512,7 @@ __device__ __forceinline__ void fwd_compute(FState<K, PX, PZ, BORN>& S, const Cl
             const float d1z = (C1 * (up1z - um1z) - C2 * (up2z - um2z)) * C.rh;
             const float a = C.paz[z], bb = C.pbz[z];
             const int ni = C.ni;
-            const int base = (z < C.reg ? z : z - C.nz + C.sw2) * ni + xi;
+            const int base = (z < C.reg ? z : z - C.nz + C.swf + C.azb) * ni + xi;
             const float ap1 = C.AZ[base + ni], am1 = C.AZ[base - ni];
             const float ap2 = C.AZ[base + 2 * ni], am2 = C.AZ[base - 2 * ni];
             const float d1_apsi = (C1 * (ap1 - am1) - C2 * (ap2 - am2)) * C.rh;
@@ -314,26 +544,32 @@ __device__ __forceinline__ void fwd_write(FState<K, PX, PZ, BORN>& S, const Cls&
         if (row >= c.h) continue;
         const int xi = cls_xi(c, col), zi = cls_zi(c, row);
         const int x = xi + 2, z = zi + 2;
-        C.U[z * C.nx + x] = S.unext[k];
+        C.U[(z - C.zb0) * C.nx + x] = S.unext[k];
         // a * psi_next, the product neighbours take in lib.rs:167-170, 190-193.
         if constexpr (PX) {
             const float prod = C.pax[x] * S.px[k];
-            if (x < C.swf) C.AX[zi * C.sw2 + x] = prod;
-            if (x >= C.nx - C.swf) C.AX[zi * C.sw2 + x - C.nx + C.sw2] = prod;
+            const int rb = (zi - C.lo) * C.sw2;
+            if (x < C.swf) C.AX[rb + x] = prod;
+            if (x >= C.nx - C.swf) C.AX[rb + x - C.nx + C.sw2] = prod;
         }
         if constexpr (PZ) {
             const float prod = C.paz[z] * S.pz[k];
             if (z < C.swf) C.AZ[z * C.ni + xi] = prod;
-            if (z >= C.nz - C.swf) C.AZ[(z - C.nz + C.sw2) * C.ni + xi] = prod;
+            if (z >= C.nz - C.swf) C.AZ[(z - C.nz + C.swf + C.azb) * C.ni + xi] = prod;
         }
     }
 }
 
-template <int KI, int KX, int KZ, int KC, bool STORE, bool BORN>
-__global__ __launch_bounds__(NT) void k_forward(FwdParams P) {
-    extern __shared__ float lds[];
+// The forward of one unit (all nt steps), or of one band of it (BAND: the
+// inner rows of `bd`, exchanging edge rows with the neighbour bands after
+// each step).  pos is the unit's position in the launch (snapshot and
+// exchange slots).  The band holding the receiver rows (the top one) records
+// the gathers.
+template <int KI, int KX, int KZ, int KC, bool STORE, bool BORN, bool BAND>
+__device__ __forceinline__ void forward_unit(const FwdParams& P, float* lds, int unit, int pos, const Band& bd,
+                                             const Xchg& X, int p, int* fail_s) {
     const Grid& g = P.g;
-    const FwdLayout L = fwd_layout(g);
+    const FwdLayout L = fwd_layout(g, bd);
     float* PR = lds + L.PR;
     FCtx C;
     C.U = lds + L.U;
@@ -351,16 +587,19 @@ __global__ __launch_bounds__(NT) void k_forward(FwdParams P) {
     C.reg = g.region;
     C.swf = g.swf;
     C.sw2 = 2 * g.swf;
+    C.zb0 = bd.zb0;
+    C.lo = bd.lo;
+    C.azb = bd.top ? g.swf : 0;
     C.rh = g.rh;
     C.rh2 = g.rh2;
     C.dt2 = g.dt2;
 
     const int tid = threadIdx.x;
-    const int unit = P.unit0 + blockIdx.x;
     const int b = unit / g.shots;
     const int s = src_slot(P.geo, unit, unit - b * g.shots);
+    const bool rec = bd.top;  // the receiver rows (the host checks) lie in the top band
     if (!unit_active(P.geo, unit)) {  // padding unit of an uneven batch
-        zero_unit_outputs(P.traces + (size_t)unit * g.nt * g.nrec, g.nt * g.nrec, nullptr, 0);
+        if (rec) zero_unit_outputs(P.traces + (size_t)unit * g.nt * g.nrec, g.nt * g.nrec, nullptr, 0);
         return;
     }
 
@@ -371,14 +610,14 @@ __global__ __launch_bounds__(NT) void k_forward(FwdParams P) {
     const float* dvis = BORN ? P.dvel + (size_t)b * g.nvx * g.nvz : nullptr;
     const float* wav = P.wavelet + (size_t)unit * P.wav_stride;
     float* traces = P.traces + (size_t)unit * g.nt * g.nrec;
-    float* snaps = STORE ? P.snaps + (size_t)blockIdx.x * g.nt * g.inner : nullptr;
-    const float* bsnaps = BORN ? P.born_snaps + (size_t)blockIdx.x * g.nt * g.inner : nullptr;
+    float* snaps = STORE ? P.snaps + (size_t)pos * g.nt * g.inner : nullptr;
+    const float* bsnaps = BORN ? P.born_snaps + (size_t)pos * g.nt * g.inner : nullptr;
 
     const int r = g.region - 2;
-    const Cls cI = make_cls(g, r, false, false);
-    const Cls cX = make_cls(g, r, true, false);
-    const Cls cZ = make_cls(g, r, false, true);
-    const Cls cC = make_cls(g, r, true, true);
+    const Cls cI = make_cls(g, r, false, false, bd.lo, bd.hi);
+    const Cls cX = make_cls(g, r, true, false, bd.lo, bd.hi);
+    const Cls cZ = make_cls(g, r, false, true, bd.lo, bd.hi);
+    const Cls cC = make_cls(g, r, true, true, bd.lo, bd.hi);
     const int rI = tid / cI.w, kI0 = tid - rI * cI.w;
     const int rX = tid / cX.w, kX0 = tid - rX * cX.w;
     const int rZ = tid / cZ.w, kZ0 = tid - rZ * cZ.w;
@@ -395,7 +634,7 @@ __global__ __launch_bounds__(NT) void k_forward(FwdParams P) {
 
     // Source corners owned by this thread: the point source is added after the
     // update by the cell's owner, -(v^2 dt^2) w f[t] (lib.rs:209-215); Born
-    // adds the derivative -(2 v dv dt^2) w f[t].
+    // adds the derivative -(2 v dv dt^2) w f[t].  scell is the LDS index.
     int scell[4];
     float scoef[4];
 #pragma unroll
@@ -405,13 +644,13 @@ __global__ __launch_bounds__(NT) void k_forward(FwdParams P) {
         const int cell = P.geo.src_cell[s * 4 + c];
         const float w = P.geo.src_w[s * 4 + c];
         const int cz = cell / g.nx, cx = cell - cz * g.nx;
-        if (w != 0.0f && cx >= 2 && cx < g.nx - 2 && cz >= 2 && cz < g.nz - 2) {
+        if (w != 0.0f && cx >= 2 && cx < g.nx - 2 && cz - 2 >= bd.lo && cz - 2 < bd.hi) {
             int cls, q;
-            cell_class(g, r, cx - 2, cz - 2, cls, q);
+            cell_class(g, r, cx - 2, cz - 2, cls, q, bd.lo, bd.hi);
             if (q % NT == tid) {
                 const float vraw = vis_at(vvis, g, cx, cz);
                 const float v = clampv(vraw, g.vmin, g.vmax);
-                scell[c] = cell;
+                scell[c] = (cz - bd.zb0) * g.nx + cx;
                 if (BORN) {
                     const float m = (vraw > g.vmin && vraw < g.vmax) ? 1.0f : 0.0f;
                     scoef[c] = -(2.0f * v * (vis_at(dvis, g, cx, cz) * m) * g.dt2) * w;
@@ -428,16 +667,17 @@ __global__ __launch_bounds__(NT) void k_forward(FwdParams P) {
     for (int c = 0; c < 4; ++c) {
         rc[c] = 0;
         rw[c] = 0.0f;
-        if (tid < g.nrec) {
-            rc[c] = P.geo.rec_cell[tid * 4 + c];
+        if (rec && tid < g.nrec) {
+            rc[c] = P.geo.rec_cell[tid * 4 + c] - bd.zb0 * g.nx;
             rw[c] = P.geo.rec_w[tid * 4 + c];
         }
     }
     __syncthreads();
 
+    bool failed = false;
     for (int t = 0; t < g.nt; ++t) {
         // Receivers record u^t before the update (lib.rs:146-149, wgsl:61-67).
-        if (tid < g.nrec) {
+        if (rec && tid < g.nrec) {
             float acc = 0.0f;
 #pragma unroll
             for (int c = 0; c < 4; ++c) acc += rw[c] * C.U[rc[c]];
@@ -459,7 +699,36 @@ __global__ __launch_bounds__(NT) void k_forward(FwdParams P) {
         for (int c = 0; c < 4; ++c)
             if (scell[c] >= 0) C.U[scell[c]] += scoef[c] * f_t;
         __syncthreads();
+        if constexpr (BAND) {
+            if (!band_exchange(X, C.U, bd, g.nx, pos, p, t + 1, fail_s)) {
+                failed = true;
+                break;
+            }
+        }
     }
+    if (BAND && failed && rec)  // a neighbour never arrived: the gathers are not valid
+        for (int j = tid; j < g.nt * g.nrec; j += NT) traces[j] = __builtin_nanf("");
+}
+
+template <int KI, int KX, int KZ, int KC, bool STORE, bool BORN>
+__global__ __launch_bounds__(NT) void k_forward(FwdParams P) {
+    extern __shared__ float lds[];
+    const Xchg X{};
+    forward_unit<KI, KX, KZ, KC, STORE, BORN, false>(P, lds, P.unit0 + blockIdx.x, blockIdx.x, whole_band(P.g), X,
+                                                     0, nullptr);
+}
+
+// Band launch: `units` units of P bands each, one workgroup per band.
+template <int KI, int KX, int KZ, int KC, bool STORE>
+__global__ __launch_bounds__(NT) void k_forward_band(FwdParams P, BandLaunch B, int units) {
+    extern __shared__ float lds[];
+    __shared__ int task_s, fail_s;
+    const Xchg X{B.xbuf, B.flags, B.err, B.P, B.spin};
+    const int task = __builtin_amdgcn_readfirstlane(band_task(B.next, &task_s));
+    if (task >= units * B.P) return;
+    const int u = task / B.P, p = task - u * B.P;
+    forward_unit<KI, KX, KZ, KC, STORE, false, true>(P, lds, P.unit0 + u, u, split_band(P.g, B.bounds, B.P, p), X, p,
+                                                     &fail_s);
 }
 
 // ===========================================================================
@@ -481,6 +750,7 @@ struct ACtx {
     const int* ER;
     const float* EW;
     int nx, nz, ni, ra, swa, sw2, bz0, bz1;
+    int zb0, lo, azb;  // band: first padded row in LDS, first inner row, bottom z-strip offset
     float rh, dt2;
 };
 
@@ -526,23 +796,25 @@ __device__ __forceinline__ void adj_phase1(AState<K, PX, PZ>& S, const Cls& c, i
         const int xi = cls_xi(c, col), zi = cls_zi(c, row);
         const int x = xi + 2, z = zi + 2;
         const float Vc = S.v2[k] * S.lam[k];
-        C.V[z * C.nx + x] = Vc;
+        C.V[(z - C.zb0) * C.nx + x] = Vc;
+        const int rb = (zi - C.lo) * C.sw2;
         if constexpr (PX) {
             const float bb = C.pbx[x];
             const float T = (1.0f + bb) * Vc + bb * S.Zx[k];
             const float U1 = C.pdbx[x] * T + bb * S.Px[k];
             if (x < C.swa) {
-                C.TX[zi * C.sw2 + x] = T;
-                C.UX[zi * C.sw2 + x] = U1;
+                C.TX[rb + x] = T;
+                C.UX[rb + x] = U1;
             }
             if (x >= C.nx - C.swa) {
-                C.TX[zi * C.sw2 + x - C.nx + C.sw2] = T;
-                C.UX[zi * C.sw2 + x - C.nx + C.sw2] = U1;
+                C.TX[rb + x - C.nx + C.sw2] = T;
+                C.UX[rb + x - C.nx + C.sw2] = U1;
             }
         } else {
-            if (x < C.swa) C.TX[zi * C.sw2 + x] = Vc;
-            if (x >= C.nx - C.swa) C.TX[zi * C.sw2 + x - C.nx + C.sw2] = Vc;
+            if (x < C.swa) C.TX[rb + x] = Vc;
+            if (x >= C.nx - C.swa) C.TX[rb + x - C.nx + C.sw2] = Vc;
         }
+        const int zb = z - C.nz + C.swa + C.azb;  // row of the bottom z-strip side
         if constexpr (PZ) {
             const float bb = C.pbz[z];
             const float T = (1.0f + bb) * Vc + bb * S.Zz[k];
@@ -552,12 +824,12 @@ __device__ __forceinline__ void adj_phase1(AState<K, PX, PZ>& S, const Cls& c, i
                 C.UZ[z * C.ni + xi] = U1;
             }
             if (z >= C.nz - C.swa) {
-                C.TZ[(z - C.nz + C.sw2) * C.ni + xi] = T;
-                C.UZ[(z - C.nz + C.sw2) * C.ni + xi] = U1;
+                C.TZ[zb * C.ni + xi] = T;
+                C.UZ[zb * C.ni + xi] = U1;
             }
         } else {
             if (z < C.swa) C.TZ[z * C.ni + xi] = Vc;
-            if (z >= C.nz - C.swa) C.TZ[(z - C.nz + C.sw2) * C.ni + xi] = Vc;
+            if (z >= C.nz - C.swa) C.TZ[zb * C.ni + xi] = Vc;
         }
         if (k == sk) C.SRC[so.corner] = S.lam[k];
     }
@@ -576,12 +848,12 @@ __device__ __forceinline__ void adj_phase2(AState<K, PX, PZ>& S, const Cls& c, i
         const int xi = cls_xi(c, col), zi = cls_zi(c, row);
         const int x = xi + 2, z = zi + 2;
         const int nx = C.nx, ni = C.ni;
-        const int i = z * nx + x;
+        const int i = (z - C.zb0) * nx + x;
         const float Sn = snap_t[zi * ni + xi];
         const float v0 = C.V[i];
         float w = 0.0f;
         if constexpr (PX) {
-            const int base = zi * C.sw2 + (x < C.ra ? x : x - nx + C.sw2);
+            const int base = (zi - C.lo) * C.sw2 + (x < C.ra ? x : x - nx + C.sw2);
             const float tm2 = C.TX[base - 2], tm1 = C.TX[base - 1], t00 = C.TX[base];
             const float tp1 = C.TX[base + 1], tp2 = C.TX[base + 2];
             const float um2 = C.UX[base - 2], um1 = C.UX[base - 1];
@@ -603,7 +875,7 @@ __device__ __forceinline__ void adj_phase2(AState<K, PX, PZ>& S, const Cls& c, i
             w += (D2A * v0 + D2B * (vp1 + vm1) + D2C * (vp2 + vm2)) * rh * rh;
         }
         if constexpr (PZ) {
-            const int base = (z < C.ra ? z : z - C.nz + C.sw2) * ni + xi;
+            const int base = (z < C.ra ? z : z - C.nz + C.swa + C.azb) * ni + xi;
             const float tm2 = C.TZ[base - 2 * ni], tm1 = C.TZ[base - ni], t00 = C.TZ[base];
             const float tp1 = C.TZ[base + ni], tp2 = C.TZ[base + 2 * ni];
             const float um2 = C.UZ[base - 2 * ni], um1 = C.UZ[base - ni];
@@ -649,11 +921,16 @@ __device__ __forceinline__ void adj_store(AState<K, PX, PZ>& S, const Cls& c, in
     }
 }
 
-template <int KI, int KX, int KZ, int KC, bool DWAV>
-__global__ __launch_bounds__(NT) void k_adjoint(AdjParams P) {
-    extern __shared__ float lds[];
+// The adjoint of one unit (t = nt-1 .. 0), or of one band of it (BAND: the
+// inner rows of `bd`, exchanging the edge rows of V with the neighbour bands
+// after phase 1 of every step).  pos is the unit's position in the launch.
+// A band writes the padded gradient of the padded rows it owns (the top and
+// bottom bands also the outer FD_PAD rows).
+template <int KI, int KX, int KZ, int KC, bool DWAV, bool BAND>
+__device__ __forceinline__ void adjoint_unit(const AdjParams& P, float* lds, int unit, int pos, const Band& bd,
+                                             const Xchg& X, int p, int* fail_s) {
     const Grid& g = P.g;
-    const AdjLayout L = adj_layout(g, P.geo);
+    const AdjLayout L = adj_layout(g, P.geo, bd);
     float* PR = lds + L.PR;
     float* RES = lds + L.RES;
     float* SRC = lds + L.SRC;  // [0..4) lambda at corners, [4..8) -v^2 dt^2 w
@@ -685,15 +962,20 @@ __global__ __launch_bounds__(NT) void k_adjoint(AdjParams P) {
     C.sw2 = 2 * g.swa;
     C.bz0 = P.geo.band_z0;
     C.bz1 = P.geo.band_z0 + P.geo.band_rows;
+    C.zb0 = bd.zb0;
+    C.lo = bd.lo;
+    C.azb = bd.top ? g.swa : 0;
     C.rh = g.rh;
     C.dt2 = g.dt2;
 
     const int tid = threadIdx.x;
-    const int unit = P.unit0 + blockIdx.x;
     const int b = unit / g.shots;
     const int s = src_slot(P.geo, unit, unit - b * g.shots);
+    const bool rec = bd.top;  // the receiver rows (the host checks) lie in the top band
+    const int own_z0 = bd.top ? 0 : bd.lo + 2, own_z1 = bd.bot ? g.nz : bd.hi + 2;  // padded rows owned
+    float* gout = P.grad + (size_t)unit * g.nx * g.nz;
     if (!unit_active(P.geo, unit)) {  // padding unit of an uneven batch
-        zero_unit_outputs(P.grad + (size_t)unit * g.nx * g.nz, g.nx * g.nz,
+        zero_unit_outputs(gout + (size_t)own_z0 * g.nx, (own_z1 - own_z0) * g.nx,
                           DWAV ? P.dwav + (size_t)unit * g.nt : nullptr, DWAV ? g.nt : 0);
         return;
     }
@@ -709,7 +991,7 @@ __global__ __launch_bounds__(NT) void k_adjoint(AdjParams P) {
     const float* vvis = P.vel + (size_t)b * g.nvx * g.nvz;
     const float* wav = P.wavelet + (size_t)unit * P.wav_stride;
     const float* resid = P.resid + (size_t)unit * g.nt * g.nrec;
-    const float* snaps = P.snaps + (size_t)blockIdx.x * g.nt * g.inner;
+    const float* snaps = P.snaps + (size_t)pos * g.nt * g.inner;
 
     if (tid < 4) {
         const int cell = P.geo.src_cell[s * 4 + tid];
@@ -720,10 +1002,10 @@ __global__ __launch_bounds__(NT) void k_adjoint(AdjParams P) {
     }
 
     const int r = g.region_adj - 2;
-    const Cls cI = make_cls(g, r, false, false);
-    const Cls cX = make_cls(g, r, true, false);
-    const Cls cZ = make_cls(g, r, false, true);
-    const Cls cC = make_cls(g, r, true, true);
+    const Cls cI = make_cls(g, r, false, false, bd.lo, bd.hi);
+    const Cls cX = make_cls(g, r, true, false, bd.lo, bd.hi);
+    const Cls cZ = make_cls(g, r, false, true, bd.lo, bd.hi);
+    const Cls cC = make_cls(g, r, true, true, bd.lo, bd.hi);
     const int rI = tid / cI.w, kI0 = tid - rI * cI.w;
     const int rX = tid / cX.w, kX0 = tid - rX * cX.w;
     const int rZ = tid / cZ.w, kZ0 = tid - rZ * cZ.w;
@@ -731,7 +1013,8 @@ __global__ __launch_bounds__(NT) void k_adjoint(AdjParams P) {
 
     // Source corners (wgsl:245-249): at most one per class per thread (corner
     // cells differ by 1, w or w +- 1 in a class's numbering; the host checks
-    // that no class is 511 or more columns wide).
+    // that no class is 511 or more columns wide).  A band owns the corners in
+    // its rows.
     SrcOwn so[4];
 #pragma unroll
     for (int cl = 0; cl < 4; ++cl) so[cl] = SrcOwn{-1, 0, 0.0f};
@@ -740,9 +1023,9 @@ __global__ __launch_bounds__(NT) void k_adjoint(AdjParams P) {
         const int cell = P.geo.src_cell[s * 4 + c];
         const float w = P.geo.src_w[s * 4 + c];
         const int cz = cell / g.nx, cx = cell - cz * g.nx;
-        if (cx >= 2 && cx < g.nx - 2 && cz >= 2 && cz < g.nz - 2) {
+        if (cx >= 2 && cx < g.nx - 2 && cz - 2 >= bd.lo && cz - 2 < bd.hi) {
             int cls, q;
-            cell_class(g, r, cx - 2, cz - 2, cls, q);
+            cell_class(g, r, cx - 2, cz - 2, cls, q, bd.lo, bd.hi);
             if (q % NT == tid) {
                 const float v = clampv(vis_at(vvis, g, cx, cz), g.vmin, g.vmax);
                 const SrcOwn o{q / NT, c, -2.0f * v * g.dt2 * w};
@@ -763,13 +1046,21 @@ __global__ __launch_bounds__(NT) void k_adjoint(AdjParams P) {
     adj_init(SC, cC, kC0, rC, g, vvis);
     __syncthreads();
 
+    bool failed = false;
     for (int t = g.nt - 1; t >= 0; --t) {
-        for (int rr = tid; rr < g.nrec; rr += NT) RES[rr] = resid[(size_t)t * g.nrec + rr];
+        if (rec)
+            for (int rr = tid; rr < g.nrec; rr += NT) RES[rr] = resid[(size_t)t * g.nrec + rr];
         adj_phase1(SI, cI, kI0, rI, C, so[0]);
         adj_phase1(SX, cX, kX0, rX, C, so[1]);
         adj_phase1(SZ, cZ, kZ0, rZ, C, so[2]);
         adj_phase1(SC, cC, kC0, rC, C, so[3]);
         __syncthreads();
+        if constexpr (BAND) {
+            if (!band_exchange(X, C.V, bd, g.nx, pos, p, g.nt - t, fail_s)) {
+                failed = true;
+                break;
+            }
+        }
         if (DWAV && tid == 0) {
             float acc = 0.0f;
 #pragma unroll
@@ -784,16 +1075,39 @@ __global__ __launch_bounds__(NT) void k_adjoint(AdjParams P) {
         adj_phase2(SC, cC, kC0, rC, C, so[3], snap_t, f_t);
         __syncthreads();
     }
-    // Padded-grid gradient of this unit; the FD_PAD ring is zero.
-    float* gout = P.grad + (size_t)unit * g.nx * g.nz;
-    for (int j = tid; j < g.nx * g.nz; j += NT) {
+    // Padded-grid gradient of the owned rows; the FD_PAD ring is zero.
+    for (int j = own_z0 * g.nx + tid; j < own_z1 * g.nx; j += NT) {
         const int z = j / g.nx, x = j - z * g.nx;
         if (x < 2 || x >= g.nx - 2 || z < 2 || z >= g.nz - 2) gout[j] = 0.0f;
+    }
+    if (BAND && failed) {  // a neighbour never arrived: the gradient is not valid
+        for (int j = own_z0 * g.nx + tid; j < own_z1 * g.nx; j += NT) gout[j] = __builtin_nanf("");
+        return;
     }
     adj_store(SI, cI, kI0, rI, g.nx, gout);
     adj_store(SX, cX, kX0, rX, g.nx, gout);
     adj_store(SZ, cZ, kZ0, rZ, g.nx, gout);
     adj_store(SC, cC, kC0, rC, g.nx, gout);
+}
+
+template <int KI, int KX, int KZ, int KC, bool DWAV>
+__global__ __launch_bounds__(NT) void k_adjoint(AdjParams P) {
+    extern __shared__ float lds[];
+    const Xchg X{};
+    adjoint_unit<KI, KX, KZ, KC, DWAV, false>(P, lds, P.unit0 + blockIdx.x, blockIdx.x, whole_band(P.g), X, 0,
+                                              nullptr);
+}
+
+template <int KI, int KX, int KZ, int KC>
+__global__ __launch_bounds__(NT) void k_adjoint_band(AdjParams P, BandLaunch B, int units) {
+    extern __shared__ float lds[];
+    __shared__ int task_s, fail_s;
+    const Xchg X{B.xbuf, B.flags, B.err, B.P, B.spin};
+    const int task = __builtin_amdgcn_readfirstlane(band_task(B.next, &task_s));
+    if (task >= units * B.P) return;
+    const int u = task / B.P, p = task - u * B.P;
+    adjoint_unit<KI, KX, KZ, KC, false, true>(P, lds, P.unit0 + u, u, split_band(P.g, B.bounds, B.P, p), X, p,
+                                              &fail_s);
 }
 
 // ---------------------------------------------------------------------------
@@ -819,6 +1133,50 @@ static hipError_t adj_t(const AdjParams& p, int units, hipStream_t st) {
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(fn, dim3(units), dim3(NT), lds, st, p);
     return hipGetLastError();
+}
+
+template <int KI, int KX, int KZ, int KC>
+static hipError_t fwd_band_t(const FwdParams& p, int units, bool store, const BandLaunch& b, hipStream_t st) {
+    size_t lds = 0;
+    for (int q = 0; q < b.P; ++q) lds = std::max<size_t>(lds, 4ull * fwd_layout(p.g, split_band(p.g, b.bounds, b.P, q)).total);
+    void (*fn)(FwdParams, BandLaunch, int) = store ? k_forward_band<KI, KX, KZ, KC, true>
+                                                   : k_forward_band<KI, KX, KZ, KC, false>;
+    hipError_t e = hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(fn, dim3(units * b.P), dim3(NT), lds, st, p, b, units);
+    return hipGetLastError();
+}
+
+template <int KI, int KX, int KZ, int KC>
+static hipError_t adj_band_t(const AdjParams& p, int units, const BandLaunch& b, hipStream_t st) {
+    size_t lds = 0;
+    for (int q = 0; q < b.P; ++q)
+        lds = std::max<size_t>(lds, 4ull * adj_layout(p.g, p.geo, split_band(p.g, b.bounds, b.P, q)).total);
+    void (*fn)(AdjParams, BandLaunch, int) = k_adjoint_band<KI, KX, KZ, KC>;
+    hipError_t e = hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(fn, dim3(units * b.P), dim3(NT), lds, st, p, b, units);
+    return hipGetLastError();
+}
+
+hipError_t launch_forward_band(const FwdParams& p, int units, bool store, const BandLaunch& b, hipStream_t st) {
+    if (units <= 0) return hipSuccess;
+    if (b.P < 2 || b.P > kMaxBands) return hipErrorInvalidValue;
+    switch (pick_band_tuple(p.g, b.P, b.bounds)) {
+        case 0: return fwd_band_t<9, 5, 5, 3>(p, units, store, b, st);
+        case 1: return fwd_band_t<12, 5, 7, 3>(p, units, store, b, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_adjoint_band(const AdjParams& p, int units, const BandLaunch& b, hipStream_t st) {
+    if (units <= 0) return hipSuccess;
+    if (b.P < 2 || b.P > kMaxBands || p.dwav) return hipErrorInvalidValue;
+    switch (pick_band_tuple(p.g, b.P, b.bounds)) {
+        case 0: return adj_band_t<9, 5, 5, 3>(p, units, b, st);
+        case 1: return adj_band_t<12, 5, 7, 3>(p, units, b, st);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t launch_forward(const FwdParams& p, int units, bool store, bool born, hipStream_t st) {
