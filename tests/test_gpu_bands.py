@@ -37,8 +37,10 @@ def test_band_kernels_match_oracle_and_streamed(orc, n, bands, nt):
     src = np.array([[55.0, 10.0], [0.5 * (n - 1) * 10.0 + 3.3, 17.5], [(n - 1) * 10.0 - 40.0, 0.62 * n * 10.0]],
                    np.float32)
     E.set_sources(src)
-    start, true = synthetic.batch(1, seed=910 + n, nvz=n, nvx=n)
-    start, true = start[0], true[0]
+    _, true = synthetic.batch(1, seed=910 + n, nvz=n, nvx=n)
+    true = true[0]
+    start = true.copy()
+    start[8:24, n // 3:n // 3 + 25] += 130.0  # shallow: a misfit the gathers see within nt
     obs = E.simulate(true.reshape(-1))
     loss, grad = E.evaluate_adjoint(start.reshape(-1), obs.reshape(-1))
     assert lib.wg_set_band_split(h, 0) == 0
@@ -66,7 +68,7 @@ def test_band_batch_larger_than_the_device():
     from wgnet_amd import synthetic
     from wgnet_amd.physics import WavePhysics
 
-    n, nt, B, S = 141, 120, 24, 5  # 120 units x 4 bands = 480 workgroups
+    n, nt, B, S = 141, 300, 24, 5  # 120 units x 4 bands = 480 workgroups
     ph = WavePhysics(shots=S, nt=nt, dt=0.001, nvx=n, nvz=n)
     lib, h = ph.engine._lib, ph.engine.handle
     assert lib.wg_engine_bands(h) == 4
@@ -80,7 +82,7 @@ def test_band_batch_larger_than_the_device():
     l_s, g_s = ph.loss_and_grad_torch(f_km, obs_s)
     lib.wg_set_band_split(h, 1)
     torch.cuda.synchronize()
-    assert bool(torch.isfinite(l).all()) and bool((l > 0).all())
+    assert bool(torch.isfinite(l).all()) and bool((l >= 0).all()) and float(l.max()) > 0.0
     assert torch.equal(obs, obs_s) and torch.equal(l, l_s) and torch.equal(g, g_s)
 
 
