@@ -615,8 +615,13 @@ def bench_uneven(args) -> int:
     shot_solves = ub.active_shot_solves
     cell_steps = sum((s.nvx + 40) * (s.nvz + 40) * s.nt * s.shots for s in solves)
     value = shot_solves * args.steps / el
-    groups = [{"grid": f"{g.key[0]}x{g.key[1]}", "nt": g.key[2], "solves": len(g.ids),
-               "path": "streamed" if g.engine.streamed else "resident"} for g in ub.groups]
+    def path(g):
+        bands = g.engine._lib.wg_engine_bands(g.engine.handle)
+        return f"resident, {bands} row bands per unit" if bands > 0 else (
+            "streamed" if g.engine.streamed else "resident")
+
+    groups = [{"grid": f"{g.key[0]}x{g.key[1]}", "nt": g.key[2], "solves": len(g.ids), "path": path(g)}
+              for g in ub.groups]
     out = {
         "metric": "uneven batch fwd+adjoint shot-solves/s (grids 70/100/141, nt 1000-2000, 1-5 shots, bands)",
         "value": round(value, 2),

@@ -172,8 +172,8 @@ class UnevenBatch:
 
         chosen = {}
         for g in self.groups:
-            if not g.engine.streamed:
-                continue
+            if not g.engine.streamed or lib.wg_engine_bands(g.engine.handle) > 0:
+                continue  # resident kernels: whole units or row bands
             best, best_t = 0, math.inf
             for tile in range(lib.wg_stream_tiles()):
                 _lib.check(lib.wg_set_stream_tile(g.engine.handle, tile))
