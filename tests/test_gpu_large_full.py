@@ -6,11 +6,16 @@ snapshots, 113 GB, so the engine must checkpoint).
 
 Checked:
   - the FP64-accumulated dot-product test of the wavelet operator F (source
-    wavelet -> gathers) through the checkpointed adjoint, with x a random
-    wavelet low-passed to the 15 Hz Ricker's band and y = F x:
-    |<F x, y> - <x, F^T y>| <= 1e-6 |<F x, y>|.  White-noise x (energy up to
-    the 625 Hz Nyquist frequency, far outside what the grid propagates) is
-    printed beside it;
+    wavelet -> gathers) through the checkpointed adjoint, y = F x, for x a
+    random wavelet low-passed to the 15 Hz Ricker's band and for white-noise
+    x: |<F x, y> - <x, F^T y>| <= 1e-5 |<F x, y>|.  The gate is float32
+    roundoff over 4000 steps, not transposition: the float32 oracle pair (the
+    reference's float32 forward restated, and the WGSL adjoint) shows the
+    same growth with nt on a small grid, 4e-7 at nt = 500 to 9e-6 at nt =
+    4000, where its float64 version stays at 1e-15
+    (tests/test_oracle.py::test_dot_product_float32_level_grows_with_nt); the
+    GPU measures 1.9e-6 here with the band-limited x, and 2.6e-7 at
+    70 x 70 x 1000;
   - the loss and gradient of the checkpointed path: finite, gradient nonzero;
   - checkpointed = every snapshot stored, bit for bit, on the same grid and
     shots with nt shortened to 400 (four segments)."""
@@ -51,7 +56,7 @@ def test_large_dot_product_checkpointed():
     P = lambda t: ctypes.c_void_p(t.data_ptr())
     rng = np.random.default_rng(31)
     taps = lowpass_kernel(15.0, DT)
-    for label, smooth, gate in (("15 Hz band-limited x", True, 1e-6), ("white-noise x", False, None)):
+    for label, smooth in (("15 Hz band-limited x", True), ("white-noise x", False)):
         x = rng.standard_normal((1, S, nt)).astype(np.float32)
         if smooth:
             x = np.stack([[np.convolve(x[0, k], taps, mode="same") for k in range(S)]]).astype(np.float32)
@@ -64,8 +69,7 @@ def test_large_dot_product_checkpointed():
         rel = abs(lhs - rhs) / abs(lhs)
         print(f"1000x300x4000, 20 shots, checkpointed adjoint, {label}: <Fx,Fx>={lhs:.12e} <x,F^T F x>={rhs:.12e} "
               f"rel {rel:.3e}", flush=True)
-        if gate is not None:
-            assert rel <= gate
+        assert rel <= 1e-5
         del fx
 
 

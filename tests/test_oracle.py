@@ -195,6 +195,29 @@ def test_adjoint_dot_product_born_fp64(orc):
     assert abs(lhs - rhs) <= 1e-11 * abs(lhs)
 
 
+def test_dot_product_float32_level_grows_with_nt(orc):
+    """The dot-product test of the float32 oracle pair (forward restated from
+    lib.rs, adjoint from the WGSL) against its float64 version over long runs:
+    float64 stays at roundoff (<= 1e-12) at every nt; float32 roundoff grows
+    with nt (printed; below 1e-5).  A GPU dot test at nt in the thousands is
+    judged against this level."""
+    rng = np.random.default_rng(0)
+    for nt in (500, 2000, 4000):
+        g = orc.Geometry(nvx=60, nvz=30, nt=nt, dt=0.0008, src_x=300.0, src_z=10.0)
+        v = layered(30, 60, depths=(12, 22))
+        x = rng.standard_normal(nt).astype(np.float32)
+        tr, sn = orc.forward_f32(g, v, x, snaps=True)
+        _, dw = orc.adjoint_f32(g, v, x, sn, tr, want_dwav=True)
+        r32 = abs(float((tr.astype(np.float64) ** 2).sum()) - float((x.astype(np.float64) * dw).sum()))
+        r32 /= float((tr.astype(np.float64) ** 2).sum())
+        tr6, sn6 = orc.forward_f64(g, v, x.astype(np.float64), snaps=True)
+        _, dw6 = orc.adjoint_f64(g, v, x.astype(np.float64), sn6, tr6, want_dwav=True)
+        r64 = abs(float((tr6 ** 2).sum()) - float((x * dw6).sum())) / float((tr6 ** 2).sum())
+        print(f"nt {nt}: float32 {r32:.2e}, float64 {r64:.2e}")
+        assert r64 <= 1e-12
+        assert r32 <= 1e-5
+
+
 def test_gradient_central_fd_fp64(orc):
     g = geom(orc)
     true = layered(g.nvz, g.nvx)
