@@ -150,7 +150,7 @@ struct wg_engine {
     int bands = 0;                 // bands per unit (0: the grid does not split)
     int band_bounds[wg::kMaxBands + 1] = {};
     bool band_on = true;
-    DevBuf bxbuf, bqflags;         // exchange rows; [next, err, flags per band]
+    DevBuf bxbuf, bqflags;         // exchange words; [next, err]
     DevBuf qstate, qflags;         // work queues: per-unit part state; counters, error flag, done flags
     hipStream_t last = nullptr;    // stream of the last enqueued call
     bool last_valid = false;
@@ -393,27 +393,22 @@ int stream_simulate(wg_engine* e, int b0, int nb, const float* vel, const float*
 
 bool use_bands(const wg_engine* e) { return e->streamed && e->bands > 1 && e->band_on; }
 
-// Buffers and counters of a band launch over `units` units: exchange rows,
-// and [next, err, per-band flags].  Clears err when `clear_err`.
+// Buffers and counters of a band launch over `units` units: exchange words
+// (zeroed: no step has tag 0) and [next, err].  Clears err when `clear_err`.
 int band_setup(wg_engine* e, int units, bool clear_err, hipStream_t st, wg::BandLaunch* bl) {
     const int P = e->bands;
+    const size_t words = wg::band_xbuf_words(e->g, units, P);
     int rc;
-    if ((rc = ensure(e, e->bxbuf, wg::band_xbuf_floats(e->g, units, P) * sizeof(float)))) return rc;
-    const size_t nq = 2 + (size_t)units * P;
-    if ((rc = ensure(e, e->bqflags, nq * sizeof(int)))) return rc;
+    if ((rc = ensure(e, e->bxbuf, words * sizeof(unsigned long long)))) return rc;
+    if ((rc = ensure(e, e->bqflags, 2 * sizeof(int)))) return rc;
     int* q = (int*)e->bqflags.p;
-    if (clear_err) {
-        LAUNCH(wg::launch_zero((float*)q, nq, st));  // int 0 = float +0 bits
-    } else {
-        LAUNCH(wg::launch_zero((float*)q, 1, st));
-        LAUNCH(wg::launch_zero((float*)(q + 2), nq - 2, st));
-    }
+    LAUNCH(wg::launch_zero((float*)e->bxbuf.p, 2 * words, st));  // 0 bits
+    LAUNCH(wg::launch_zero((float*)q, clear_err ? 2 : 1, st));
     bl->P = P;
     for (int i = 0; i <= wg::kMaxBands; ++i) bl->bounds[i] = e->band_bounds[i];
-    bl->xbuf = (float*)e->bxbuf.p;
+    bl->xbuf = (unsigned long long*)e->bxbuf.p;
     bl->next = q;
     bl->err = q + 1;
-    bl->flags = q + 2;
     bl->spin = e->qspin < 0 ? -1 : std::min(e->qspin, 1 << 22);
     return WG_OK;
 }

@@ -121,8 +121,7 @@ constexpr int kMaxBands = 4;
 struct BandLaunch {
     int P;
     int bounds[kMaxBands + 1];  // band p holds the inner rows [bounds[p], bounds[p + 1])
-    float* xbuf;                // band_xbuf_floats(g, units, P) floats
-    int* flags;                 // units * P ints, zero at launch
+    unsigned long long* xbuf;   // band_xbuf_words(g, units, P) words, zero at launch
     int* next;                  // task counter, zero at launch
     int* err;                   // set to 1 when a wait gives up
     int spin;                   // polls before a wait gives up (< 0: at once)
@@ -131,7 +130,7 @@ struct BandLaunch {
 // [2, kMaxBands], boundaries in the interior rows at least one adjoint strip
 // width from either edge, the receiver rows in the top band), or false.
 bool band_plan(const Grid& g, const Geometry& geo, int lds_limit, int* P, int bounds[kMaxBands + 1]);
-size_t band_xbuf_floats(const Grid& g, int units, int P);
+size_t band_xbuf_words(const Grid& g, int units, int P);
 hipError_t launch_forward_band(const FwdParams& p, int units, bool store, const BandLaunch& b, hipStream_t st);
 hipError_t launch_adjoint_band(const AdjParams& p, int units, const BandLaunch& b, hipStream_t st);
 

@@ -268,21 +268,26 @@ int slots_needed(const Grid& g) {
 
 // Slot counts of the band kernels, the largest that compile without spills:
 // the 100 x 100 model (144 x 144 padded) in two bands, the 141 x 141 model
-// (185 x 185) in four.
-static const Tuple kBandTuples[] = {{9, 5, 5, 3}, {12, 5, 7, 3}};
-constexpr int kNumBandTuples = 2;
+// (185 x 185) in three.  The forward and the adjoint pick their own (the
+// adjoint's C-PML region is two cells wider, and its slots cost more
+// registers).
+static const Tuple kBandTuples[] = {{9, 5, 5, 3}, {12, 5, 7, 3}, {13, 4, 6, 2}};
+constexpr int kNumBandTuples = 3;
+static const bool kBandTupleFwd[] = {true, true, true};
+static const bool kBandTupleAdj[] = {true, true, false};
 
-// The smallest band tuple that holds every band's classes, forward and
-// adjoint, or -1.
-static int pick_band_tuple(const Grid& g, int P, const int* bounds) {
+// The first band tuple that holds every band's classes of region band r
+// (forward: region - 2, adjoint: region_adj - 2), or -1.
+static int pick_band_tuple(const Grid& g, int P, const int* bounds, bool adjoint) {
+    const int r = adjoint ? g.region_adj - 2 : g.region - 2;
     for (int i = 0; i < kNumBandTuples; ++i) {
+        if (!(adjoint ? kBandTupleAdj[i] : kBandTupleFwd[i])) continue;
         const Tuple& t = kBandTuples[i];
         const int have[4] = {t.kI, t.kX, t.kZ, t.kC};
         bool ok = true;
         for (int p = 0; p < P && ok; ++p)
-            for (int r : {g.region - 2, g.region_adj - 2})
-                for (int c = 0; c < 4 && ok; ++c)
-                    ok = cls_count(make_cls(g, r, c & 1, c & 2, bounds[p], bounds[p + 1])) <= have[c] * NT;
+            for (int c = 0; c < 4 && ok; ++c)
+                ok = cls_count(make_cls(g, r, c & 1, c & 2, bounds[p], bounds[p + 1])) <= have[c] * NT;
         if (ok) return i;
     }
     return -1;
@@ -318,7 +323,7 @@ bool band_plan(const Grid& g, const Geometry& geo, int lds_limit, int* Pout, int
                 const size_t f = 4ull * fwd_layout(g, bd).total, a = 4ull * adj_layout(g, geo, bd).total;
                 ok = f + 64 <= (size_t)lds_limit && a + 64 <= (size_t)lds_limit;
             }
-            if (ok && pick_band_tuple(g, P, bounds) >= 0) {
+            if (ok && pick_band_tuple(g, P, bounds, false) >= 0 && pick_band_tuple(g, P, bounds, true) >= 0) {
                 *Pout = P;
                 return true;
             }
@@ -328,34 +333,55 @@ bool band_plan(const Grid& g, const Geometry& geo, int lds_limit, int* Pout, int
     return false;
 }
 
-size_t band_xbuf_floats(const Grid& g, int units, int P) { return (size_t)units * P * 2 * 2 * 2 * g.nx; }
+size_t band_xbuf_words(const Grid& g, int units, int P) { return (size_t)units * P * 2 * 2 * 2 * g.nx; }
 
 // ---------------------------------------------------------------------------
 // Halo exchange between the bands of a split unit.  After a step each band
 // stores its two edge rows of the field into its slot of `buf` (two slots,
-// by step parity), then its count of published steps into its flag; it waits
-// until each neighbour's count reaches the same step and loads the
-// neighbour's edge rows into its halo rows.  Rows and flags move with
-// relaxed agent-scope atomic stores and loads, which bypass the compute
-// unit's L1 and are coherent across the device; the wait for the row stores
-// to complete (s_waitcnt) before the flag store orders them, so a band that
-// reads a neighbour's count reads the rows stored before it.  (Release and
-// acquire fences order them too, but write back and invalidate the whole L2
-// of the XCD at every step: several microseconds more per step.)  Wave 0
-// does it; the other waves wait at the barrier that follows.  A neighbour
-// overwrites a slot two steps later, after waiting for this band's count of
-// the step in between, which this band publishes only after reading the slot.
+// by step parity), each float in a 64-bit word whose high half is the step
+// number; then it loads each neighbour's edge rows into its halo rows,
+// polling every word until it carries the step number.  Words move with
+// relaxed agent-scope 64-bit atomic stores and loads: single-copy atomic,
+// past the compute unit's L1, coherent across the device.  The step number
+// in every word is what orders the data, so there is no flag to publish after
+// the rows and no wait for their stores to complete, and a stale word (an
+// older step of the same parity, or the zeros the slots start with) is never
+// taken.  (Release and acquire fences would order the rows too, but they
+// write back and invalidate the whole L2 of the XCD at every step: several
+// microseconds more per step.)  A neighbour overwrites a slot two steps
+// later, after loading this band's rows of the step in between, which this
+// band stores only after it has loaded the slot.  Wave 0 does the exchange;
+// the other waves wait at the barrier that follows.
 // ---------------------------------------------------------------------------
 struct Xchg {
-    float* buf;  // [launch units][P][2 sides (top rows, bottom rows)][2 parities][2 rows][nx]
-    int* flags;  // [launch units][P], published steps (zero at launch)
-    int* err;    // set by a wait that gives up
+    unsigned long long* buf;  // [launch units][P][2 sides (top rows, bottom rows)][2 parities][2 rows][nx]
+    int* err;                 // set by a wait that gives up
     int P;
-    int spin;    // polls before a wait gives up (< 0: at once)
+    int spin;                 // polls before a wait gives up (< 0: at once)
 };
 
-__device__ __forceinline__ float* xslot(const Xchg& X, int nx, int u, int p, int side, int par) {
+__device__ __forceinline__ unsigned long long* xslot(const Xchg& X, int nx, int u, int p, int side, int par) {
     return X.buf + ((((size_t)u * X.P + p) * 2 + side) * 2 + par) * 2 * nx;
+}
+
+__device__ __forceinline__ void xput(unsigned long long* d, const float* src, int n, unsigned k, int lane) {
+    for (int j = lane; j < n; j += 64)
+        __hip_atomic_store(d + j, ((unsigned long long)k << 32) | __float_as_uint(src[j]), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Loads n words of step k into dst; false when a word never arrived.
+__device__ __forceinline__ bool xget(float* dst, const unsigned long long* s, int n, unsigned k, int lane, int spin) {
+    for (int j = lane; j < n; j += 64) {
+        unsigned long long w = __hip_atomic_load(s + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int c = 0; (unsigned)(w >> 32) != k; ++c) {
+            if (c > spin) return false;
+            __builtin_amdgcn_s_sleep(1);
+            w = __hip_atomic_load(s + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        dst[j] = __uint_as_float((unsigned)w);
+    }
+    return true;
 }
 
 // Exchange of field F (LDS rows of band bd, nx floats each) for unit u (its
@@ -365,44 +391,15 @@ __device__ __forceinline__ bool band_exchange(const Xchg& X, float* F, const Ban
                                               int* fail_s) {
     const int par = k & 1, n2 = 2 * nx, lane = threadIdx.x;
     if (lane < 64) {
-        if (bd.up) {
-            float* d = xslot(X, nx, u, p, 0, par);
-            const float* src = F + 2 * nx;  // the first two own rows
-            for (int j = lane; j < n2; j += 64) __hip_atomic_store(d + j, src[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        if (bd.down) {
-            float* d = xslot(X, nx, u, p, 1, par);
-            const float* src = F + (bd.hi - bd.lo) * nx;  // the last two own rows
-            for (int j = lane; j < n2; j += 64) __hip_atomic_store(d + j, src[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        __builtin_amdgcn_s_waitcnt(0);
-        int ok = 1;
+        if (bd.up) xput(xslot(X, nx, u, p, 0, par), F + 2 * nx, n2, k, lane);                // first two own rows
+        if (bd.down) xput(xslot(X, nx, u, p, 1, par), F + (bd.hi - bd.lo) * nx, n2, k, lane);  // last two
+        bool ok = true;
+        if (bd.up) ok = xget(F, xslot(X, nx, u, p - 1, 1, par), n2, k, lane, X.spin);
+        if (ok && bd.down) ok = xget(F + (bd.hi - bd.lo + 2) * nx, xslot(X, nx, u, p + 1, 0, par), n2, k, lane, X.spin);
+        const bool all = __builtin_amdgcn_ballot_w64(!ok) == 0;
         if (lane == 0) {
-            int* fl = X.flags + (size_t)u * X.P;
-            __hip_atomic_store(fl + p, k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            for (int q = p - 1; q <= p + 1; q += 2) {
-                if ((q < p && !bd.up) || (q > p && !bd.down)) continue;
-                for (int n = 0; __hip_atomic_load(fl + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < k; ++n) {
-                    if (n > X.spin) {
-                        ok = 0;
-                        __hip_atomic_fetch_or(X.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(1);
-                }
-                if (!ok) break;
-            }
-            *fail_s = !ok;
-        }
-        ok = __builtin_amdgcn_readfirstlane(ok);
-        if (ok && bd.up) {
-            const float* src = xslot(X, nx, u, p - 1, 1, par);
-            for (int j = lane; j < n2; j += 64) F[j] = __hip_atomic_load(src + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        if (ok && bd.down) {
-            const float* src = xslot(X, nx, u, p + 1, 0, par);
-            float* dst = F + (bd.hi - bd.lo + 2) * nx;
-            for (int j = lane; j < n2; j += 64) dst[j] = __hip_atomic_load(src + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            *fail_s = !all;
+            if (!all) __hip_atomic_fetch_or(X.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
     __syncthreads();
@@ -723,7 +720,7 @@ template <int KI, int KX, int KZ, int KC, bool STORE>
 __global__ __launch_bounds__(NT) void k_forward_band(FwdParams P, BandLaunch B, int units) {
     extern __shared__ float lds[];
     __shared__ int task_s, fail_s;
-    const Xchg X{B.xbuf, B.flags, B.err, B.P, B.spin};
+    const Xchg X{B.xbuf, B.err, B.P, B.spin};
     const int task = __builtin_amdgcn_readfirstlane(band_task(B.next, &task_s));
     if (task >= units * B.P) return;
     const int u = task / B.P, p = task - u * B.P;
@@ -1102,7 +1099,7 @@ template <int KI, int KX, int KZ, int KC>
 __global__ __launch_bounds__(NT) void k_adjoint_band(AdjParams P, BandLaunch B, int units) {
     extern __shared__ float lds[];
     __shared__ int task_s, fail_s;
-    const Xchg X{B.xbuf, B.flags, B.err, B.P, B.spin};
+    const Xchg X{B.xbuf, B.err, B.P, B.spin};
     const int task = __builtin_amdgcn_readfirstlane(band_task(B.next, &task_s));
     if (task >= units * B.P) return;
     const int u = task / B.P, p = task - u * B.P;
@@ -1162,9 +1159,10 @@ static hipError_t adj_band_t(const AdjParams& p, int units, const BandLaunch& b,
 hipError_t launch_forward_band(const FwdParams& p, int units, bool store, const BandLaunch& b, hipStream_t st) {
     if (units <= 0) return hipSuccess;
     if (b.P < 2 || b.P > kMaxBands) return hipErrorInvalidValue;
-    switch (pick_band_tuple(p.g, b.P, b.bounds)) {
+    switch (pick_band_tuple(p.g, b.P, b.bounds, false)) {
         case 0: return fwd_band_t<9, 5, 5, 3>(p, units, store, b, st);
         case 1: return fwd_band_t<12, 5, 7, 3>(p, units, store, b, st);
+        case 2: return fwd_band_t<13, 4, 6, 2>(p, units, store, b, st);
         default: return hipErrorInvalidValue;
     }
 }
@@ -1172,7 +1170,7 @@ hipError_t launch_forward_band(const FwdParams& p, int units, bool store, const 
 hipError_t launch_adjoint_band(const AdjParams& p, int units, const BandLaunch& b, hipStream_t st) {
     if (units <= 0) return hipSuccess;
     if (b.P < 2 || b.P > kMaxBands || p.dwav) return hipErrorInvalidValue;
-    switch (pick_band_tuple(p.g, b.P, b.bounds)) {
+    switch (pick_band_tuple(p.g, b.P, b.bounds, true)) {
         case 0: return adj_band_t<9, 5, 5, 3>(p, units, b, st);
         case 1: return adj_band_t<12, 5, 7, 3>(p, units, b, st);
         default: return hipErrorInvalidValue;
