@@ -1,5 +1,5 @@
 """Row bands: the 100 x 100 and 141 x 141 models (144 x 144 and 185 x 185
-padded), too large for one workgroup, run as 2 and 3 row bands, one workgroup
+padded), too large for one workgroup, run as 2 and 4 row bands, one workgroup
 each, exchanging edge rows every step.
 
 Checked: gathers, loss and gradient against the float32 oracle (gathers rel-L2
@@ -24,7 +24,7 @@ def relerr(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
-@pytest.mark.parametrize("n,bands,nt", [(100, 2, 400), (141, 3, 400)])
+@pytest.mark.parametrize("n,bands,nt", [(100, 2, 400), (141, 4, 400)])
 def test_band_kernels_match_oracle_and_streamed(orc, n, bands, nt):
     from wgnet_amd import synthetic
     from wgnet_amd.wave_gauss_rs import PyWaveEngine
@@ -68,10 +68,10 @@ def test_band_batch_larger_than_the_device():
     from wgnet_amd import synthetic
     from wgnet_amd.physics import WavePhysics
 
-    n, nt, B, S = 141, 300, 24, 5  # 120 units x 3 bands = 360 workgroups
+    n, nt, B, S = 141, 300, 24, 5  # 120 units x 4 bands = 480 workgroups
     ph = WavePhysics(shots=S, nt=nt, dt=0.001, nvx=n, nvz=n)
     lib, h = ph.engine._lib, ph.engine.handle
-    assert lib.wg_engine_bands(h) == 3
+    assert lib.wg_engine_bands(h) == 4
     start, true = synthetic.batch(B, seed=77, nvz=n, nvx=n)
     t_km = torch.from_numpy(true / 1000.0).cuda()
     f_km = torch.from_numpy(start / 1000.0).cuda()

@@ -303,6 +303,22 @@ static void band_bounds(int nzi, int P, int m, int* bounds) {
     bounds[P] = nzi;
 }
 
+// Work of one band per step in interior-cell equivalents: C-PML cells cost
+// more than interior ones (about 1.6 times on one axis, 2.2 on both), and each
+// exchanged side about 1,500 cells' worth.  A unit's bands wait for each
+// other, so a split costs P times its most expensive band.
+static double band_cost(const Grid& g, const int* bounds, int P) {
+    const double wcls[4] = {1.0, 1.6, 1.6, 2.2};
+    double worst = 0.0;
+    for (int p = 0; p < P; ++p) {
+        double c = 1500.0 * ((p > 0) + (p < P - 1));
+        for (int k = 0; k < 4; ++k)
+            c += wcls[k] * cls_count(make_cls(g, g.region - 2, k & 1, k & 2, bounds[p], bounds[p + 1]));
+        worst = c > worst ? c : worst;
+    }
+    return worst * P;
+}
+
 bool band_plan(const Grid& g, const Geometry& geo, int lds_limit, int* Pout, int bounds[kMaxBands + 1]) {
     if (g.nrec > NT || g.ni >= 511) return false;
     // receiver corners and the residual CSR rows: inner rows of the top band
@@ -311,26 +327,34 @@ bool band_plan(const Grid& g, const Geometry& geo, int lds_limit, int* Pout, int
         rec_lo = geo.band_z0 - 2;
         rec_hi = geo.band_z0 + geo.band_rows - 1 - 2;
     }
+    // every split that fits (LDS, registers, boundaries clear of the z-strips),
+    // the cheapest kept
+    double best = -1.0;
+    int b[kMaxBands + 1];
     for (int P = 2; P <= kMaxBands; ++P) {
         for (int m = (g.nzi + P - 1) / P; m >= 4; --m) {  // middle bands from the even split down
-            band_bounds(g.nzi, P, m, bounds);
+            band_bounds(g.nzi, P, m, b);
             bool ok = true;
             for (int p = 1; p < P && ok; ++p)  // boundaries clear of the z-strips, bands of >= 4 rows
-                ok = bounds[p] >= g.swa && bounds[p] <= g.nzi - g.swa && bounds[p] - bounds[p - 1] >= 4;
-            ok = ok && bounds[P] - bounds[P - 1] >= 4 && rec_lo >= 0 && rec_hi < bounds[1];
+                ok = b[p] >= g.swa && b[p] <= g.nzi - g.swa && b[p] - b[p - 1] >= 4;
+            ok = ok && b[P] - b[P - 1] >= 4 && rec_lo >= 0 && rec_hi < b[1];
             for (int p = 0; p < P && ok; ++p) {
-                const Band bd = split_band(g, bounds, P, p);
+                const Band bd = split_band(g, b, P, p);
                 const size_t f = 4ull * fwd_layout(g, bd).total, a = 4ull * adj_layout(g, geo, bd).total;
                 ok = f + 64 <= (size_t)lds_limit && a + 64 <= (size_t)lds_limit;
             }
-            if (ok && pick_band_tuple(g, P, bounds, false) >= 0 && pick_band_tuple(g, P, bounds, true) >= 0) {
-                *Pout = P;
-                return true;
+            if (ok && pick_band_tuple(g, P, b, false) >= 0 && pick_band_tuple(g, P, b, true) >= 0) {
+                const double c = band_cost(g, b, P);
+                if (best < 0.0 || c < best) {
+                    best = c;
+                    *Pout = P;
+                    for (int i = 0; i <= P; ++i) bounds[i] = b[i];
+                }
             }
             if (P == 2) break;  // two bands have no middle band to shrink
         }
     }
-    return false;
+    return best >= 0.0;
 }
 
 size_t band_xbuf_words(const Grid& g, int units, int P) { return (size_t)units * P * 2 * 2 * 2 * g.nx; }
