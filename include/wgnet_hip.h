@@ -229,9 +229,13 @@ int32_t wg_engine_streamed(const wg_engine *e);
  * with every snapshot stored, and exchanges two edge rows with each
  * neighbour band per step through global memory.  wg_engine_bands returns
  * the bands per unit (0: the grid does not split, or fits one workgroup).
- * Results equal the HBM-streamed kernels' bit for bit; wg_set_band_split(e,
- * 0) selects those instead.  The band kernels serve simulate and the loss
- * and gradient when a model's snapshots fit the snapshot budget; the Born
+ * The 100 x 100 and 141 x 141 grids (pml 20) have band kernels compiled for
+ * their geometry and split (2 and 4 bands); other grids run runtime-geometry
+ * band kernels.  Results equal the HBM-streamed kernels' bit for bit either
+ * way.  wg_set_band_split(e, on): 1 (default) the band kernels, compiled
+ * where the grid has them; 2 the runtime-geometry band kernels only; 0 the
+ * HBM-streamed kernels.  The band kernels serve simulate and the loss and
+ * gradient when a model's snapshots fit the snapshot budget; the Born
  * operator, the wavelet gradient and checkpointed adjoints stay streamed. */
 int32_t wg_engine_bands(const wg_engine *e);
 int32_t wg_set_band_split(wg_engine *e, int32_t on);
@@ -255,6 +259,13 @@ int32_t wg_set_stream_tile(wg_engine *e, int32_t tile);
  * wavelet-gradient adjoint (wg_adjoint_apply_dev with dwav) always runs
  * single steps. */
 int32_t wg_set_time_block(wg_engine *e, int32_t steps);
+/* Temporal blocking of the HBM-streamed forward (no reference counterpart):
+ * time steps per launch, 1 (default) or 2.  A two-step launch recomputes a
+ * 2-cell ring around each tile and reads the wavefield, velocity and memory
+ * variables and writes them once per two steps (snapshots every step).
+ * Gathers, snapshots, checkpoints and everything computed from them are
+ * identical either way; the Born forward always runs single steps. */
+int32_t wg_set_forward_time_block(wg_engine *e, int32_t steps);
 /* Resident work queues (no reference counterpart; grids of the compiled
  * 70 x 70 kernels): the loss and gradient of a whole batch run as one
  * forward launch and one adjoint launch, in which every (model, shot) unit's

@@ -1,13 +1,16 @@
 """Row bands: the 100 x 100 and 141 x 141 models (144 x 144 and 185 x 185
 padded), too large for one workgroup, run as 2 and 4 row bands, one workgroup
-each, exchanging edge rows every step.
+each, exchanging edge rows every step, on band kernels compiled for these
+grids (the default) or on the runtime-geometry band kernels.
 
 Checked: gathers, loss and gradient against the float32 oracle (gathers rel-L2
 <= 1e-5, loss |rel| <= 1e-6, gradient rel-L2 <= 1e-6; the per-cell arithmetic
 is the whole-grid kernels', so bit-equality is expected), and bit for bit
-against the HBM-streamed kernels on the same engine; off-node sources in
-several bands; a batch with more bands than compute units; and a wait that
-gives up (forced from the host) turns every result of the call into NaN."""
+between the compiled band kernels, the runtime-geometry band kernels and the
+HBM-streamed kernels on the same engine; off-node sources in several bands; a
+batch with more bands than compute units; and a wait that gives up (forced
+from the host) turns every result of the call into NaN, on both band
+kernels."""
 
 from __future__ import annotations
 
@@ -43,12 +46,16 @@ def test_band_kernels_match_oracle_and_streamed(orc, n, bands, nt):
     start[8:24, n // 3:n // 3 + 25] += 130.0  # shallow: a misfit the gathers see within nt
     obs = E.simulate(true.reshape(-1))
     loss, grad = E.evaluate_adjoint(start.reshape(-1), obs.reshape(-1))
-    assert lib.wg_set_band_split(h, 0) == 0
+    assert lib.wg_set_band_split(h, 2) == 0  # runtime-geometry band kernels
+    obs_r = E.simulate(true.reshape(-1))
+    loss_r, grad_r = E.evaluate_adjoint(start.reshape(-1), obs.reshape(-1))
+    assert lib.wg_set_band_split(h, 0) == 0  # HBM-streamed kernels
     obs_s = E.simulate(true.reshape(-1))
     loss_s, grad_s = E.evaluate_adjoint(start.reshape(-1), obs.reshape(-1))
     assert lib.wg_set_band_split(h, 1) == 0
-    print(f"{n}x{n}, {bands} bands: gathers equal to streamed {np.array_equal(obs, obs_s)}, loss "
-          f"{loss == loss_s}, gradient {np.array_equal(grad, grad_s)}", flush=True)
+    print(f"{n}x{n}, {bands} bands: gathers equal to runtime bands {np.array_equal(obs, obs_r)} and streamed "
+          f"{np.array_equal(obs, obs_s)}, loss {loss == loss_r} {loss == loss_s}, gradient "
+          f"{np.array_equal(grad, grad_r)} {np.array_equal(grad, grad_s)}", flush=True)
     for s, (x, z) in enumerate(src):
         g = orc.Geometry(nvx=n, nvz=n, nt=nt, dt=0.001, src_x=float(x), src_z=float(z))
         err = relerr(obs[s], orc.forward_f32(g, true, E.wavelet()))
@@ -59,6 +66,7 @@ def test_band_kernels_match_oracle_and_streamed(orc, n, bands, nt):
     print(f"  loss {loss:.9e} vs {rl:.9e}; gradient rel-L2 {relerr(grad, rg):.2e}", flush=True)
     assert abs(loss - rl) <= 1e-6 * rl
     assert relerr(grad, rg) <= 1e-6
+    assert np.array_equal(obs, obs_r) and loss == loss_r and np.array_equal(grad, grad_r)
     assert np.array_equal(obs, obs_s) and loss == loss_s and np.array_equal(grad, grad_s)
 
 
@@ -77,16 +85,20 @@ def test_band_batch_larger_than_the_device():
     f_km = torch.from_numpy(start / 1000.0).cuda()
     obs = ph.simulate_torch(t_km)
     l, g = ph.loss_and_grad_torch(f_km, obs)
-    lib.wg_set_band_split(h, 0)
-    obs_s = ph.simulate_torch(t_km)
-    l_s, g_s = ph.loss_and_grad_torch(f_km, obs_s)
+    results = {}
+    for mode in (2, 0):  # runtime-geometry band kernels, HBM-streamed kernels
+        assert lib.wg_set_band_split(h, mode) == 0
+        o = ph.simulate_torch(t_km)
+        results[mode] = (o,) + tuple(ph.loss_and_grad_torch(f_km, o))
     lib.wg_set_band_split(h, 1)
     torch.cuda.synchronize()
     assert bool(torch.isfinite(l).all()) and bool((l >= 0).all()) and float(l.max()) > 0.0
-    assert torch.equal(obs, obs_s) and torch.equal(l, l_s) and torch.equal(g, g_s)
+    for mode, (o, l_m, g_m) in results.items():
+        assert torch.equal(obs, o) and torch.equal(l, l_m) and torch.equal(g, g_m), mode
 
 
-def test_band_wait_that_gives_up_gives_nan():
+@pytest.mark.parametrize("mode", [1, 2])
+def test_band_wait_that_gives_up_gives_nan(mode):
     """A negative wait limit makes every band's first exchange give up (all
     kernels end normally): the gathers, losses and gradients of the call are
     NaN; the default limit gives the results back."""
@@ -96,6 +108,7 @@ def test_band_wait_that_gives_up_gives_nan():
     n, nt = 100, 60
     ph = WavePhysics(shots=2, nt=nt, dt=0.001, nvx=n, nvz=n)
     lib, h = ph.engine._lib, ph.engine.handle
+    assert lib.wg_set_band_split(h, mode) == 0
     start, true = synthetic.batch(2, seed=5, nvz=n, nvx=n)
     obs = ph.simulate_torch(torch.from_numpy(true / 1000.0).cuda())
     f = torch.from_numpy(start / 1000.0).cuda()

@@ -75,13 +75,17 @@ def test_sources_on_and_beyond_the_edges(orc):
         assert np.array_equal(gpu[s], ref), f"source {s} at ({x}, {z})"
 
 
-def test_single_step_streamed(orc):
-    """nt = 1 and 3 on the HBM-streamed kernels (100 x 100)."""
+@pytest.mark.parametrize("bands,fwd_block", [(True, 1), (False, 1), (False, 2)])
+def test_single_step_streamed(orc, bands, fwd_block):
+    """nt = 1, 2 and 3 on the 100 x 100 grid: row bands, or the HBM-streamed
+    kernels with one or two forward steps per launch."""
     from wgnet_amd.wave_gauss_rs import PyWaveEngine
 
-    for nt in (1, 3):
+    for nt in (1, 2, 3):
         E = PyWaveEngine(1, nt, 0.001, 15.0, 20, 1000.0, 5000.0, nvx=100, nvz=100)
         assert E.streamed
+        E.set_band_split(bands)
+        E.set_forward_time_block(fwd_block)
         E.set_sources(np.array([[120.0, 10.0]], np.float32))
         rng = np.random.default_rng(10 + nt)
         obs = rng.standard_normal(nt * 100).astype(np.float32)

@@ -23,13 +23,18 @@ def layered(nvz, nvx, seed):
     return m
 
 
-def test_streamed_100x100_matches_oracle(orc):
-    """Config 2's 100 x 100 grid (144 x 144 padded)."""
+@pytest.mark.parametrize("fwd_block", [1, 2])
+def test_streamed_100x100_matches_oracle(orc, fwd_block):
+    """Config 2's 100 x 100 grid (144 x 144 padded) on the HBM-streamed
+    kernels (the row bands it runs on by default are tested in
+    test_gpu_bands.py), with one and two forward steps per launch."""
     from wgnet_amd.wave_gauss_rs import PyWaveEngine
 
     nt = 300
     E = PyWaveEngine(2, nt, 0.001, 15.0, 20, 1000.0, 5000.0, nvx=100, nvz=100)
-    assert E.streamed
+    assert E.streamed and E.bands == 2
+    E.set_band_split(False)
+    E.set_forward_time_block(fwd_block)
     E.set_sources(np.array([[150.0, 10.0], [777.7, 10.0]], np.float32))
     true = layered(100, 100, 1)
     cur = true.copy()
@@ -56,7 +61,7 @@ def test_streamed_checkpoint_recompute_is_exact(orc):
 
     nt, nvx, nvz = 450, 300, 100
     E = PyWaveEngine(3, nt, 0.0008, 15.0, 20, 1000.0, 5000.0, nvx=nvx, nvz=nvz)
-    assert E.streamed
+    assert E.streamed and E.bands == 0
     E.set_sources(np.array([[0.0, 10.0], [1495.0, 10.0], [2990.0, 10.0]], np.float32))
     true = layered(nvz, nvx, 2)
     yy, xx = np.mgrid[0:nvz, 0:nvx]
@@ -75,8 +80,10 @@ def test_streamed_checkpoint_recompute_is_exact(orc):
     assert relerr(grad_b, rg) <= 1e-6
 
 
-def test_streamed_dot_products():
-    """Wavelet- and Born-operator dot tests on the streamed kernels (y = F x)."""
+@pytest.mark.parametrize("bands", [False, True])
+def test_streamed_dot_products(bands):
+    """Wavelet- and Born-operator dot tests on the streamed kernels (y = F x);
+    with the row bands on, the wavelet forward runs on them."""
     import ctypes
 
     import torch
@@ -87,6 +94,7 @@ def test_streamed_dot_products():
 
     B, S, nt = 1, 2, 300
     E = PyWaveEngine(S, nt, 0.001, 15.0, 20, 1000.0, 5000.0, nvx=100, nvz=100)
+    E.set_band_split(bands)
     rng = np.random.default_rng(3)
     P = lambda t: ctypes.c_void_p(t.data_ptr())
     vel = torch.from_numpy(layered(100, 100, 5)[None].copy()).cuda()
@@ -119,6 +127,7 @@ def test_time_blocked_adjoint_is_bit_identical(nvx, nvz, nt, budget):
 
     E = PyWaveEngine(3, nt, 0.001, 15.0, 20, 1000.0, 5000.0, nvx=nvx, nvz=nvz)
     assert E.streamed
+    E.set_band_split(False)
     if budget:
         E.set_snapshot_budget(budget)
     w = (nvx - 1) * 10.0
@@ -140,11 +149,50 @@ def test_time_blocked_adjoint_is_bit_identical(nvx, nvz, nt, budget):
         assert np.array_equal(grad, ref_grad), key
 
 
+@pytest.mark.parametrize("nvx,nvz,nt,budget", [(100, 100, 301, None), (100, 100, 300, None),
+                                               (150, 90, 251, 40 << 20), (150, 90, 250, 40 << 20)])
+def test_time_blocked_forward_is_bit_identical(nvx, nvz, nt, budget):
+    """Two forward steps per launch (tile + recomputed 2-cell ring, the
+    second step's gathers recorded by the next launch) equal single steps bit
+    for bit, on every tile shape and with both adjoint blockings: gathers
+    (odd and even step counts, so the last gathers come from a two-step
+    launch's successor or from a record launch before a final single step),
+    and loss and gradient, through stored snapshots or checkpoint segments
+    (100, 100, 51 / 50 steps: a segment of odd length ends with a single
+    step); receivers, sources and the C-PML strips cross tile edges."""
+    from wgnet_amd.wave_gauss_rs import PyWaveEngine
+
+    E = PyWaveEngine(3, nt, 0.001, 15.0, 20, 1000.0, 5000.0, nvx=nvx, nvz=nvz)
+    assert E.streamed
+    E.set_band_split(False)
+    if budget:
+        E.set_snapshot_budget(budget)
+    w = (nvx - 1) * 10.0
+    E.set_sources(np.array([[0.0, 10.0], [0.37 * w, 35.0], [w, 150.0]], np.float32))
+    true = layered(nvz, nvx, 6)
+    cur = true.copy()
+    cur[nvz // 4:nvz // 2, nvx // 3:nvx // 2] -= 150.0
+    obs_ref = E.simulate(true.reshape(-1))
+    assert np.abs(obs_ref[:, -1]).max() > 0.0  # the last step's gathers are not trivially zero
+    ref = E.evaluate_adjoint(cur.reshape(-1), obs_ref.reshape(-1))
+    assert np.abs(ref[1]).max() > 0.0
+    E.set_forward_time_block(2)
+    for tile in range(4):
+        E.set_stream_tile(tile)
+        obs = E.simulate(true.reshape(-1))
+        assert np.array_equal(obs, obs_ref), tile
+        for tb in (1, 2):
+            E.set_time_block(tb)
+            loss, grad = E.evaluate_adjoint(cur.reshape(-1), obs_ref.reshape(-1))
+            assert loss == ref[0], (tile, tb)
+            assert np.array_equal(grad, ref[1]), (tile, tb)
+
+
 def test_streamed_xcd_tile_order_matches_oracle(orc):
     """A 600 x 200 grid (644 x 244 padded): 10 x 15 tiles of 64 x 16 a unit,
     enough for the launchers' XCD-aware tile order (from 128 tiles a unit).
-    Gathers and gradient equal the oracle; the two-step adjoint equals the
-    single-step one bit for bit."""
+    Gathers and gradient equal the oracle; the two-step adjoint and the
+    two-step forward equal the single-step ones bit for bit."""
     from wgnet_amd.wave_gauss_rs import PyWaveEngine
 
     nt, nvx, nvz = 200, 600, 200
@@ -172,3 +220,7 @@ def test_streamed_xcd_tile_order_matches_oracle(orc):
     E.set_time_block(2)
     loss2, grad2 = E.evaluate_adjoint(cur.reshape(-1), obs.reshape(-1))
     assert loss2 == loss and np.array_equal(grad2, grad)
+    E.set_forward_time_block(2)
+    assert np.array_equal(E.simulate(true.reshape(-1)), obs)
+    loss3, grad3 = E.evaluate_adjoint(cur.reshape(-1), obs.reshape(-1))
+    assert loss3 == loss and np.array_equal(grad3, grad)

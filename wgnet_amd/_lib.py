@@ -78,6 +78,7 @@ SIGNATURES = {
     "wg_illumination_dev": (_i32, [_vp, _i32, _vp, _vp, _vp]),
     "wg_set_overlap": (_i32, [_vp, _i32]),
     "wg_set_time_block": (_i32, [_vp, _i32]),
+    "wg_set_forward_time_block": (_i32, [_vp, _i32]),
     "wg_set_time_split": (_i32, [_vp, _i32]),
     "wg_set_queue_wait_limit": (_i32, [_vp, _i32]),
     "wg_causal_filter_dev": (_i32, [_vp, ctypes.c_int64, _i32, _i32, _vp, _i32, _vp, _vp]),

@@ -19,8 +19,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libwgnet_hip.so")
-SOURCES = ["wave_kernels.hip", "wave_kernels_fixed.hip", "stream_kernels.hip", "misfit_kernels.hip", "precond.hip",
-           "engine.hip"]
+SOURCES = ["wave_kernels.hip", "wave_kernels_fixed.hip", "wave_kernels_band.hip", "stream_kernels.hip",
+           "misfit_kernels.hip", "precond.hip", "engine.hip"]
 ARCH = os.environ.get("WGNET_OFFLOAD_ARCH", "gfx950")
 
 

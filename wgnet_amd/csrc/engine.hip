@@ -142,6 +142,7 @@ struct wg_engine {
     int cus = 256;                 // compute units of the device
     bool overlap = true;           // overlap chunk forwards with the previous chunk's adjoint
     int time_block = 1;            // adjoint steps per streamed launch (1 or 2)
+    int fwd_block = 1;             // forward steps per streamed launch (1 or 2)
     int qparts = 4;                // time parts per unit of the resident work queues (0: chunk pipeline)
     int qspin = 1 << 24;           // polls before a work-queue wait gives up
     // row bands: grids too large for one workgroup whose units split into
@@ -149,7 +150,7 @@ struct wg_engine {
     // exchanged every step) instead of the per-step HBM-streamed kernels
     int bands = 0;                 // bands per unit (0: the grid does not split)
     int band_bounds[wg::kMaxBands + 1] = {};
-    bool band_on = true;
+    int band_on = 1;               // 0: streamed kernels, 1: band kernels (compiled where the grid has them), 2: runtime-geometry band kernels
     DevBuf bxbuf, bqflags;         // exchange words; [next, err]
     DevBuf qstate, qflags;         // work queues: per-unit part state; counters, error flag, done flags
     hipStream_t last = nullptr;    // stream of the last enqueued call
@@ -410,6 +411,7 @@ int band_setup(wg_engine* e, int units, bool clear_err, hipStream_t st, wg::Band
     bl->next = q;
     bl->err = q + 1;
     bl->spin = e->qspin < 0 ? -1 : std::min(e->qspin, 1 << 22);
+    bl->fixed = e->band_on == 1 ? 1 : 0;
     return WG_OK;
 }
 
@@ -486,12 +488,13 @@ struct StreamPlan {
 };
 
 constexpr int kCheckpointEvery = 100;  // steps between forward checkpoints when snapshots do not fit
+constexpr int kCkptArrays = 6;         // a checkpoint: u^t, u^{t-1}, psi_x, psi_z, zeta_x, zeta_z
 constexpr int kMaxStepUnits = 65535;   // units of one step launch (grid z dimension)
 
 StreamPlan plan_stream(wg_engine* e, int B, bool need_snaps) {
     const wg::Grid& g = e->g;
     const size_t S = e->cfg.shots, cells = (size_t)g.nx * g.nz, inner = g.inner;
-    const size_t state = 20 * cells * sizeof(float);
+    const size_t state = (wg::kFwdArrays + 12) * cells * sizeof(float);
     StreamPlan P{1, g.nt, 1, need_snaps};
     const size_t full = state + (need_snaps ? (size_t)g.nt * inner * sizeof(float) : 0);
     if (!need_snaps || full * S <= e->snap_budget) {
@@ -501,7 +504,7 @@ StreamPlan plan_stream(wg_engine* e, int B, bool need_snaps) {
     }
     P.seg_len = std::min(g.nt, kCheckpointEvery);
     P.nseg = (g.nt + P.seg_len - 1) / P.seg_len;
-    const size_t per = state + (size_t)P.seg_len * inner * sizeof(float) + (size_t)P.nseg * 8 * cells * sizeof(float);
+    const size_t per = state + (size_t)P.seg_len * inner * sizeof(float) + (size_t)P.nseg * kCkptArrays * cells * sizeof(float);
     P.models = (int)std::max<size_t>(1, std::min<size_t>(B, e->snap_budget / (per * S)));
     P.models = std::min(P.models, std::max(1, kMaxStepUnits / (int)S));
     return P;
@@ -523,11 +526,90 @@ wg::StepParams step_params(wg_engine* e, const float* vel, int units, int unit0)
 int stream_alloc(wg_engine* e, const StreamPlan& P, int units) {
     const size_t cells = (size_t)e->g.nx * e->g.nz;
     int rc;
-    if ((rc = ensure(e, e->fstate, 8 * units * cells * sizeof(float)))) return rc;
+    if ((rc = ensure(e, e->fstate, wg::kFwdArrays * units * cells * sizeof(float)))) return rc;
     if (!P.snaps) return WG_OK;
     if ((rc = ensure(e, e->astate, 12 * units * cells * sizeof(float)))) return rc;
     if ((rc = ensure(e, e->snaps, (size_t)units * P.seg_len * e->g.inner * sizeof(float)))) return rc;
-    if (P.nseg > 1 && (rc = ensure(e, e->ckpt, (size_t)P.nseg * 8 * units * cells * sizeof(float)))) return rc;
+    if (P.nseg > 1 && (rc = ensure(e, e->ckpt, (size_t)P.nseg * kCkptArrays * units * cells * sizeof(float))))
+        return rc;
+    return WG_OK;
+}
+
+// Where the streamed forward state is: the wavefield buffers holding u^t
+// and u^{t-1}, the half psi is read from and the half holding zeta.
+struct FwdRot {
+    int cur = 0, prev = 1, ps = 0, ze = 0;
+};
+
+// Forward steps [t0, t1) of `units` units from the state `rot` describes
+// (updated), two per launch where the engine's forward block is 2; gathers
+// into p.traces when set.  A two-step launch cannot record its second step's
+// gathers (u^{t+1} is computed, not read), so the next two-step launch
+// records them, or a record launch before a single step or at the end.
+int fwd_steps(wg_engine* e, wg::StepParams& p, int units, int t0, int t1, bool store, FwdRot& rot, hipStream_t st) {
+    bool pending = false;
+    auto record_prev = [&](int t) {
+        wg::StepParams q = p;
+        q.t = t - 1;
+        q.u_cur = rot.prev;
+        return wg::launch_fwd_record(q, units, st);
+    };
+    for (int t = t0; t < t1;) {
+        p.t = t;
+        p.u_cur = rot.cur;
+        p.u_prev = rot.prev;
+        p.ps_half = rot.ps;
+        p.ze_half = rot.ze;
+        if (e->fwd_block == 2 && t + 1 < t1) {
+            int o[2], n = 0;
+            for (int k = 0; k < 4; ++k)
+                if (k != rot.cur && k != rot.prev) o[n++] = k;
+            p.u_out0 = o[0];
+            p.u_out1 = o[1];
+            p.trace_prev = pending ? 1 : 0;
+            LAUNCH(wg::launch_fwd_step2(p, units, store, st));
+            rot.prev = o[0];
+            rot.cur = o[1];
+            rot.ps ^= 1;
+            rot.ze ^= 1;
+            pending = p.traces != nullptr;
+            t += 2;
+        } else {
+            if (pending) LAUNCH(record_prev(t));
+            pending = false;
+            p.trace_prev = 0;
+            LAUNCH(wg::launch_fwd_step(p, units, store, false, st));
+            std::swap(rot.cur, rot.prev);
+            rot.ps ^= 1;
+            t += 1;
+        }
+    }
+    if (pending) LAUNCH(record_prev(t1));
+    p.trace_prev = 0;
+    return WG_OK;
+}
+
+// Launches of fwd_steps over [t0, t1) (for the timing average).
+int fwd_launches(const wg_engine* e, int n) { return e->fwd_block == 2 ? n / 2 + n % 2 : n; }
+
+// The live arrays of the forward state, in checkpoint order.
+void ckpt_arrays(const FwdRot& rot, int out[kCkptArrays]) {
+    const int a[kCkptArrays] = {wg::fwd_u_arr(rot.cur), wg::fwd_u_arr(rot.prev), 2 + rot.ps, 4 + rot.ps,
+                                wg::fwd_zx_arr(rot.ze), wg::fwd_zz_arr(rot.ze)};
+    for (int k = 0; k < kCkptArrays; ++k) out[k] = a[k];
+}
+
+// Copies the live forward state into checkpoint `slot` (save) or checkpoint
+// `slot` into the arrays of the state FwdRot{} describes (restore).
+int ckpt_copy(wg_engine* e, int units, int slot, const FwdRot& rot, bool save, hipStream_t st) {
+    const size_t abytes = (size_t)units * e->g.nx * e->g.nz * sizeof(float);
+    int arrs[kCkptArrays];
+    ckpt_arrays(save ? rot : FwdRot{}, arrs);
+    for (int k = 0; k < kCkptArrays; ++k) {
+        char* state = (char*)e->fstate.p + (size_t)arrs[k] * abytes;
+        char* ck = (char*)e->ckpt.p + ((size_t)slot * kCkptArrays + k) * abytes;
+        HIPCK(hipMemcpyAsync(save ? ck : state, save ? state : ck, abytes, hipMemcpyDeviceToDevice, st));
+    }
     return WG_OK;
 }
 
@@ -537,8 +619,7 @@ int stream_alloc(wg_engine* e, const StreamPlan& P, int units) {
 int stream_forward(wg_engine* e, const StreamPlan& P, int units, int unit0, const float* vel, const float* wav,
                    long long wav_stride, float* traces, bool store, hipStream_t st) {
     const size_t cells = (size_t)e->g.nx * e->g.nz;
-    const size_t sbytes = 8 * units * cells * sizeof(float);
-    LAUNCH(wg::launch_zero((float*)e->fstate.p, sbytes / sizeof(float), st));
+    LAUNCH(wg::launch_zero((float*)e->fstate.p, (size_t)wg::kFwdArrays * units * cells, st));
     wg::StepParams p = step_params(e, vel, units, unit0);
     p.wavelet = wav;
     p.wav_stride = wav_stride;
@@ -548,13 +629,16 @@ int stream_forward(wg_engine* e, const StreamPlan& P, int units, int unit0, cons
     p.seg_len = P.seg_len;
     p.state = (float*)e->fstate.p;
     const bool keep = store && P.nseg == 1;
-    Timed tm(e, &e->ev_fwd, st, e->g.nt);
-    for (int t = 0; t < e->g.nt; ++t) {
-        if (store && P.nseg > 1 && t % P.seg_len == 0)
-            HIPCK(hipMemcpyAsync((char*)e->ckpt.p + (size_t)(t / P.seg_len) * sbytes, e->fstate.p, sbytes,
-                                 hipMemcpyDeviceToDevice, st));
-        p.t = t;
-        LAUNCH(wg::launch_fwd_step(p, units, keep, false, st));
+    const bool ckpt = store && P.nseg > 1;
+    const int nt = e->g.nt, seg = ckpt ? P.seg_len : nt;
+    int launches = 0;
+    for (int t0 = 0; t0 < nt; t0 += seg) launches += fwd_launches(e, std::min(nt, t0 + seg) - t0);
+    Timed tm(e, &e->ev_fwd, st, launches);
+    FwdRot rot;
+    int rc;
+    for (int t0 = 0; t0 < nt; t0 += seg) {
+        if (ckpt && (rc = ckpt_copy(e, units, t0 / seg, rot, true, st))) return rc;
+        if ((rc = fwd_steps(e, p, units, t0, std::min(nt, t0 + seg), keep, rot, st))) return rc;
     }
     return WG_OK;
 }
@@ -566,7 +650,6 @@ int stream_adjoint(wg_engine* e, const StreamPlan& P, int units, int unit0, cons
                    float* gunit, float* dwav, hipStream_t st) {
     const wg::Grid& g = e->g;
     const size_t cells = (size_t)g.nx * g.nz;
-    const size_t sbytes = 8 * units * cells * sizeof(float);
     LAUNCH(wg::launch_zero((float*)e->astate.p, 12 * units * cells, st));
     LAUNCH(wg::launch_zero(gunit + (size_t)unit0 * cells, units * cells, st));
     wg::StepParams pf = step_params(e, vel, units, unit0);
@@ -588,14 +671,12 @@ int stream_adjoint(wg_engine* e, const StreamPlan& P, int units, int unit0, cons
     for (int seg = P.nseg - 1; seg >= 0; --seg) {
         const int t0 = seg * P.seg_len, t1 = std::min(g.nt, t0 + P.seg_len);
         if (P.nseg > 1) {
-            HIPCK(hipMemcpyAsync(e->fstate.p, (char*)e->ckpt.p + (size_t)seg * sbytes, sbytes,
-                                 hipMemcpyDeviceToDevice, st));
+            int rc;
+            FwdRot rot;
+            if ((rc = ckpt_copy(e, units, seg, rot, false, st))) return rc;
             pf.snap_t0 = t0;
-            Timed tm(e, &e->ev_fwd, st, t1 - t0);
-            for (int t = t0; t < t1; ++t) {
-                pf.t = t;
-                LAUNCH(wg::launch_fwd_step(pf, units, true, false, st));
-            }
+            Timed tm(e, &e->ev_fwd, st, fwd_launches(e, t1 - t0));
+            if ((rc = fwd_steps(e, pf, units, t0, t1, true, rot, st))) return rc;
         }
         pa.snap_t0 = P.nseg > 1 ? t0 : 0;
         const int launches = two ? (t1 - t0 + 1) / 2 : t1 - t0;
@@ -942,7 +1023,8 @@ int32_t wg_engine_create(const wg_config* cfg, wg_engine** out) {
         gh.band_rows = e->band_rows;
         gh.n_ent = (int)e->ent_r.size();
         int P = 0;
-        if (wg::band_plan(g, gh, lds_limit, &P, e->band_bounds)) e->bands = P;
+        if (wg::band_split_fx(g, gh, &P, e->band_bounds) || wg::band_plan(g, gh, lds_limit, &P, e->band_bounds))
+            e->bands = P;
     }
     e->prof.resize(3 * (g.nx + g.nz));
     wg_cpml_profiles(g.nx, c.h, c.dt, c.pml, c.vmax, c.peak_hz, e->prof.data(), e->prof.data() + g.nx,
@@ -1381,8 +1463,12 @@ int32_t wg_born_dev(wg_engine* e, int32_t B, const float* vel_dev, const float* 
             p.state = (float*)e->astate.p;
             p.dvel = dvel_dev;
             p.born_snaps = (const float*)e->snaps.p;
-            for (int t = 0; t < g.nt; ++t) {
+            for (int t = 0; t < g.nt; ++t) {  // single steps: wavefield buffers 0 and 1, zeta half 0
                 p.t = t;
+                p.u_cur = t & 1;
+                p.u_prev = 1 - (t & 1);
+                p.ps_half = t & 1;
+                p.ze_half = 0;
                 LAUNCH(wg::launch_fwd_step(p, nb * S, false, true, st));
             }
         }
@@ -1483,7 +1569,8 @@ int32_t wg_engine_bands(const wg_engine* e) { return e ? e->bands : -1; }
 int32_t wg_set_band_split(wg_engine* e, int32_t on) {
     if (!e) return fail(WG_E_INVALID, "null engine");
     Guard G(e);
-    e->band_on = on != 0;
+    if (on < 0 || on > 2) return fail(WG_E_INVALID, "band split must be 0, 1 or 2");
+    e->band_on = on;
     return WG_OK;
 }
 
@@ -1512,6 +1599,13 @@ int32_t wg_set_time_block(wg_engine* e, int32_t steps) {
     if (!e || steps < 1 || steps > 2) return fail(WG_E_INVALID, "steps must be 1 or 2");
     Guard G(e);
     e->time_block = steps;
+    return WG_OK;
+}
+
+int32_t wg_set_forward_time_block(wg_engine* e, int32_t steps) {
+    if (!e || steps < 1 || steps > 2) return fail(WG_E_INVALID, "steps must be 1 or 2");
+    Guard G(e);
+    e->fwd_block = steps;
     return WG_OK;
 }
 

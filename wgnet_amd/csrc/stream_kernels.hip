@@ -91,15 +91,16 @@ __global__ __launch_bounds__(NT) void k_fwd_step(StepParams P) {
     const int unit = P.unit0 + ul;
     const int b = unit / g.shots, s = src_slot(P.geo, unit, unit - (unit / g.shots) * g.shots);
     const size_t cells = (size_t)g.nx * g.nz;
-    const int t = P.t, par = t & 1;
-    float* Ucur = P.state + ((size_t)(0 + par) * P.units + ul) * cells;
-    float* Unxt = P.state + ((size_t)(1 - par) * P.units + ul) * cells;
-    float* PXc = P.state + ((size_t)(2 + par) * P.units + ul) * cells;
-    float* PXn = P.state + ((size_t)(3 - par) * P.units + ul) * cells;
-    float* PZc = P.state + ((size_t)(4 + par) * P.units + ul) * cells;
-    float* PZn = P.state + ((size_t)(5 - par) * P.units + ul) * cells;
-    float* ZX = P.state + ((size_t)6 * P.units + ul) * cells;
-    float* ZZ = P.state + ((size_t)7 * P.units + ul) * cells;
+    const int t = P.t, h = P.ps_half;
+    auto arr = [&](int a) { return P.state + ((size_t)a * P.units + ul) * cells; };
+    float* Ucur = arr(fwd_u_arr(P.u_cur));
+    float* Unxt = arr(fwd_u_arr(P.u_prev));  // u^{t-1}, overwritten with u^{t+1}
+    float* PXc = arr(2 + h);
+    float* PXn = arr(3 - h);
+    float* PZc = arr(4 + h);
+    float* PZn = arr(5 - h);
+    float* ZX = arr(fwd_zx_arr(P.ze_half));
+    float* ZZ = arr(fwd_zz_arr(P.ze_half));
     const float* vvis = P.vel + (size_t)b * g.nvx * g.nvz;
     const float* pax = P.geo.prof;
     const float* pbx = pax + g.nx;
@@ -269,6 +270,362 @@ __global__ __launch_bounds__(NT) void k_fwd_step(StepParams P) {
             PZn[i] = psz;
             ZZ[i] = zez;
         }
+    }
+}
+
+// --------------------------------------------------------------------------
+// Two forward steps per launch (temporal blocking; lib.rs:145-220 for t and
+// t + 1).  Step A advances the tile and the 2-cell ring around it (the cells
+// step B's stencils read; the ring's corners are not needed) from u^t on the
+// tile plus a 4-cell halo, u^{t-1} and the memory variables, which are read
+// from HBM once per two steps; step B advances the tile from step A's values,
+// kept in registers and LDS.  Ring cells are recomputed by the neighbouring
+// tiles with the same operations, so the result is bit-identical to two
+// single steps.  u^{t+1} and u^{t+2} go to the two wavefield buffers not
+// read, psi and zeta to the other halves, so no tile reads what another
+// writes in the same launch.  Gathers: step t's from u^t, and with
+// P.trace_prev step t - 1's from u^{t-1}; step t + 1's are recorded by the
+// next launch.
+// --------------------------------------------------------------------------
+template <class TL, bool STORE>
+__global__ __launch_bounds__(NT) void k_fwd_step2(StepParams P) {
+    WG_TILE_CONSTANTS
+    constexpr int AX = TX + 8, AZ = TZ + 8;  // tile + 4-cell halo
+    constexpr int BX = TX + 4, BZ = TZ + 4;  // tile + 2-cell halo
+    constexpr int RING = 4 * TX + 4 * TZ;    // 2-cell ring without its corners
+    constexpr int RPT = (RING + NT - 1) / NT;
+    // step A: u^t on tile + 4 (pitch AX); a psi_x^t on rows tile + 2, columns
+    // tile + 4; a psi_z^t on rows tile + 4, columns tile + 2.  Step B reuses
+    // them for u^{t+1} on tile + 2 (pitch BX), a psi_x^{t+1} on rows tile,
+    // columns tile + 2, and a psi_z^{t+1} on rows tile + 2, columns tile.
+    __shared__ float sU[AZ * AX];
+    __shared__ float sAX[BZ * AX];
+    __shared__ float sAZ[AZ * BX];
+    const Grid& g = P.g;
+    const TileId tile = tile_id(P);
+    const int ul = tile.unit;
+    const int unit = P.unit0 + ul;
+    const int b = unit / g.shots, s = src_slot(P.geo, unit, unit - (unit / g.shots) * g.shots);
+    const size_t cells = (size_t)g.nx * g.nz;
+    const int t = P.t, h = P.ps_half;
+    auto arr = [&](int a) { return P.state + ((size_t)a * P.units + ul) * cells; };
+    const float* Ucur = arr(fwd_u_arr(P.u_cur));    // u^t
+    const float* Uprev = arr(fwd_u_arr(P.u_prev));  // u^{t-1}
+    float* O0 = arr(fwd_u_arr(P.u_out0));           // u^{t+1}
+    float* O1 = arr(fwd_u_arr(P.u_out1));           // u^{t+2}
+    const float* PXr = arr(2 + h);
+    float* PXw = arr(3 - h);
+    const float* PZr = arr(4 + h);
+    float* PZw = arr(5 - h);
+    const float* ZXr = arr(fwd_zx_arr(P.ze_half));
+    float* ZXw = arr(fwd_zx_arr(1 - P.ze_half));
+    const float* ZZr = arr(fwd_zz_arr(P.ze_half));
+    float* ZZw = arr(fwd_zz_arr(1 - P.ze_half));
+    const float* vvis = P.vel + (size_t)b * g.nvx * g.nvz;
+    const float* pax = P.geo.prof;
+    const float* pbx = pax + g.nx;
+    const float* pdbx = pbx + g.nx;
+    const float* paz = P.geo.prof + 3 * g.nx;
+    const float* pbz = paz + g.nz;
+    const float* pdbz = pbz + g.nz;
+    const int tid = threadIdx.x;
+    const int xi0 = tile.x * TX, zi0 = tile.z * TZ;  // tile origin; cell (rx, rz) is padded (xi0+2+rx, zi0+2+rz)
+
+    if (!unit_active(P.geo, unit)) {  // padding unit of an uneven batch
+        if (P.traces && tile.x == 0 && tile.z == 0)
+            for (int r = tid; r < g.nrec; r += NT) {
+                P.traces[((size_t)unit * g.nt + t) * g.nrec + r] = 0.0f;
+                if (P.trace_prev) P.traces[((size_t)unit * g.nt + t - 1) * g.nrec + r] = 0.0f;
+            }
+        return;
+    }
+    // Receivers record u^t (and u^{t-1}) before the update (lib.rs:146-149).
+    if (P.traces && tile.x == 0 && tile.z == 0) {
+        for (int r = tid; r < g.nrec; r += NT) {
+            float acc = 0.0f;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) acc += P.geo.rec_w[r * 4 + c] * Ucur[P.geo.rec_cell[r * 4 + c]];
+            P.traces[((size_t)unit * g.nt + t) * g.nrec + r] = acc;
+            if (P.trace_prev) {
+                float ap = 0.0f;
+#pragma unroll
+                for (int c = 0; c < 4; ++c) ap += P.geo.rec_w[r * 4 + c] * Uprev[P.geo.rec_cell[r * 4 + c]];
+                P.traces[((size_t)unit * g.nt + t - 1) * g.nrec + r] = ap;
+            }
+        }
+    }
+    auto inner_cell = [&](int rx, int rz) {
+        const int x = xi0 + 2 + rx, z = zi0 + 2 + rz;
+        return x >= 2 && z >= 2 && x < g.nx - 2 && z < g.nz - 2;
+    };
+    auto pml_x = [&](int x) { return x < g.region || x >= g.nx - g.region; };
+    auto pml_z = [&](int z) { return z < g.region || z >= g.nz - g.region; };
+    int scell[4];
+    float sw[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        scell[c] = P.geo.src_cell[s * 4 + c];
+        sw[c] = P.geo.src_w[s * 4 + c];
+    }
+    // own state of this thread's cells, loaded before the staging so that its
+    // latency overlaps it: CPT tile cells (rows tid / TX + k RZ of column
+    // tid % TX) and up to RPT ring cells (top and bottom rows, then the left
+    // and right columns): u^{t-1}, the velocity, psi and zeta
+    const int lx = tid % TX;
+    float pu[CPT], pv[CPT], ppx[CPT], pzx[CPT], ppz[CPT], pzz[CPT];
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+        const int lz = tid / TX + RZ * k;
+        pu[k] = pv[k] = ppx[k] = pzx[k] = ppz[k] = pzz[k] = 0.0f;
+        if (inner_cell(lx, lz)) {
+            const int x = xi0 + 2 + lx, z = zi0 + 2 + lz;
+            const int i = z * g.nx + x;
+            pu[k] = Uprev[i];
+            pv[k] = vis_raw(vvis, g, x, z);
+            if (pml_x(x)) {
+                ppx[k] = PXr[i];
+                pzx[k] = ZXr[i];
+            }
+            if (pml_z(z)) {
+                ppz[k] = PZr[i];
+                pzz[k] = ZZr[i];
+            }
+        }
+    }
+    int rrx[RPT], rrz[RPT];
+    float ru[RPT], rv[RPT], rpx[RPT], rzx[RPT], rpz[RPT], rzz[RPT];
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+        const int j = tid + NT * r;
+        int rx = 0, rz = -100;  // rz = -100: no ring cell for this thread
+        if (j < 2 * TX) {
+            rz = -2 + j / TX;
+            rx = j % TX;
+        } else if (j < 4 * TX) {
+            rz = TZ + (j - 2 * TX) / TX;
+            rx = (j - 2 * TX) % TX;
+        } else if (j < 4 * TX + 2 * TZ) {
+            rz = (j - 4 * TX) / 2;
+            rx = -2 + ((j - 4 * TX) & 1);
+        } else if (j < RING) {
+            rz = (j - 4 * TX - 2 * TZ) / 2;
+            rx = TX + ((j - 4 * TX - 2 * TZ) & 1);
+        }
+        rrx[r] = rx;
+        rrz[r] = rz;
+        ru[r] = rv[r] = rpx[r] = rzx[r] = rpz[r] = rzz[r] = 0.0f;
+        if (rz != -100 && inner_cell(rx, rz)) {
+            const int x = xi0 + 2 + rx, z = zi0 + 2 + rz;
+            const int i = z * g.nx + x;
+            ru[r] = Uprev[i];
+            rv[r] = vis_raw(vvis, g, x, z);
+            if (pml_x(x)) {
+                rpx[r] = PXr[i];
+                rzx[r] = ZXr[i];
+            }
+            if (pml_z(z)) {
+                rpz[r] = PZr[i];
+                rzz[r] = ZZr[i];
+            }
+        }
+    }
+    // Stage u^t (tile + 4), a*psi_x^t and a*psi_z^t, each thread's loads
+    // issued together before its LDS writes; 0 outside the grid.
+    {
+        constexpr int NS = (AZ * AX + NT - 1) / NT;
+        float tu[NS];
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            const int j = tid + k * NT;
+            const int x = xi0 - 2 + (j % AX), z = zi0 - 2 + (j / AX);
+            tu[k] = (j < AZ * AX && x >= 0 && z >= 0 && x < g.nx && z < g.nz) ? Ucur[z * g.nx + x] : 0.0f;
+        }
+#pragma unroll
+        for (int k = 0; k < NS; ++k)
+            if (tid + k * NT < AZ * AX) sU[tid + k * NT] = tu[k];
+    }
+    // does step A's region (the tile and its ring, padded [xi0, xi0 + TX + 4))
+    // hold C-PML cells along x / z?
+    const bool tpx2 = xi0 < g.region || xi0 + TX + 4 > g.nx - g.region;
+    const bool tpz2 = zi0 < g.region || zi0 + TZ + 4 > g.nz - g.region;
+    if (tpx2) {
+        constexpr int NS = (BZ * AX + NT - 1) / NT;
+        float ta[NS];
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            const int j = tid + k * NT;
+            const int x = xi0 - 2 + (j % AX), z = zi0 + (j / AX);
+            ta[k] = (j < BZ * AX && x >= 0 && x < g.nx && z < g.nz) ? pax[x] * PXr[z * g.nx + x] : 0.0f;
+        }
+#pragma unroll
+        for (int k = 0; k < NS; ++k)
+            if (tid + k * NT < BZ * AX) sAX[tid + k * NT] = ta[k];
+    }
+    if (tpz2) {
+        constexpr int NS = (AZ * BX + NT - 1) / NT;
+        float ta[NS];
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            const int j = tid + k * NT;
+            const int x = xi0 + (j % BX), z = zi0 - 2 + (j / BX);
+            ta[k] = (j < AZ * BX && z >= 0 && x < g.nx && z < g.nz) ? paz[z] * PZr[z * g.nx + x] : 0.0f;
+        }
+#pragma unroll
+        for (int k = 0; k < NS; ++k)
+            if (tid + k * NT < AZ * BX) sAZ[tid + k * NT] = ta[k];
+    }
+    __syncthreads();
+
+    // Step A at inner cell (rx, rz): returns u^{t+1}, updates psi / zeta,
+    // gives u^t in uc and stores step t's snapshot when `snap`.
+    const float f_a = P.wavelet[(size_t)unit * P.wav_stride + t];
+    auto step_a = [&](int rx, int rz, float up, float vr, float& psx, float& zex, float& psz, float& zez, float& uc,
+                      bool snap) -> float {
+        const int x = xi0 + 2 + rx, z = zi0 + 2 + rz;
+        const int i = z * g.nx + x;
+        const int li = (rz + 4) * AX + (rx + 4);
+        const float u = sU[li];
+        uc = u;
+        const bool px = pml_x(x), pz = pml_z(z);
+        Pml mx{0, 0, 0, 0}, mz{0, 0, 0, 0};
+        if (px) {
+            const float bb = pbx[x];
+            mx = Pml{pax[x], 1.0f + bb, bb, pdbx[x]};
+        }
+        if (pz) {
+            const float bb = pbz[z];
+            mz = Pml{paz[z], 1.0f + bb, bb, pdbz[z]};
+        }
+        const float* AXc = sAX + (rz + 2) * AX + (rx + 4);
+        const float* AZc = sAZ + (rz + 4) * BX + (rx + 2);
+        float w;
+        if (px && pz)
+            w = fwd_w<true, true>(sU, li, AX, u, g.rh, g.rh2, AXc, mx, psx, zex, AZc, BX, mz, psz, zez);
+        else if (px)
+            w = fwd_w<true, false>(sU, li, AX, u, g.rh, g.rh2, AXc, mx, psx, zex, AZc, BX, mz, psz, zez);
+        else if (pz)
+            w = fwd_w<false, true>(sU, li, AX, u, g.rh, g.rh2, AXc, mx, psx, zex, AZc, BX, mz, psz, zez);
+        else
+            w = fwd_w<false, false>(sU, li, AX, u, g.rh, g.rh2, AXc, mx, psx, zex, AZc, BX, mz, psz, zez);
+        const float v = clampv(vr, g.vmin, g.vmax);
+        if (STORE && snap) {
+            const size_t q = (size_t)(z - 2) * g.ni + (x - 2);
+            P.snaps[((size_t)(t - P.snap_t0) * P.units + ul) * g.inner + q] = 2.0f * v * g.dt2 * w;
+        }
+        float next = v * v * g.dt2 * w + 2.0f * u - up;
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+            if (scell[c] == i && sw[c] != 0.0f) next += -(v * v * g.dt2) * sw[c] * f_a;
+        return next;
+    };
+    float ut[CPT], un[CPT];  // u^t and u^{t+1} of the tile cells
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+        const int lz = tid / TX + RZ * k;
+        ut[k] = un[k] = 0.0f;
+        if (inner_cell(lx, lz)) un[k] = step_a(lx, lz, pu[k], pv[k], ppx[k], pzx[k], ppz[k], pzz[k], ut[k], true);
+    }
+    float rn[RPT];
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+        rn[r] = 0.0f;
+        float uc;
+        if (rrz[r] != -100 && inner_cell(rrx[r], rrz[r]))
+            rn[r] = step_a(rrx[r], rrz[r], ru[r], rv[r], rpx[r], rzx[r], rpz[r], rzz[r], uc, false);
+    }
+    __syncthreads();  // step A's reads of the LDS are done
+
+    // u^{t+1} on tile + 2 and the step B strips a*psi^{t+1}, from step A's
+    // registers (psi is 0 where a cell is outside the region or the grid)
+    const bool tpx = xi0 + 2 < g.region || xi0 + TX + 2 > g.nx - g.region;
+    const bool tpz = zi0 + 2 < g.region || zi0 + TZ + 2 > g.nz - g.region;
+    auto put = [&](int rx, int rz, float u1, float psx, float psz) {
+        const int x = xi0 + 2 + rx, z = zi0 + 2 + rz;
+        const bool ing = x < g.nx && z < g.nz;
+        sU[(rz + 2) * BX + (rx + 2)] = u1;
+        if (tpx && rz >= 0 && rz < TZ) sAX[rz * BX + (rx + 2)] = ing ? pax[x] * psx : 0.0f;
+        if (tpz && rx >= 0 && rx < TX) sAZ[(rz + 2) * TX + rx] = ing ? paz[z] * psz : 0.0f;
+    };
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) put(lx, tid / TX + RZ * k, un[k], ppx[k], ppz[k]);
+#pragma unroll
+    for (int r = 0; r < RPT; ++r)
+        if (rrz[r] != -100) put(rrx[r], rrz[r], rn[r], rpx[r], rpz[r]);
+    __syncthreads();
+
+    // Step B (time t + 1) on the tile
+    const int t1 = t + 1;
+    const float f_b = P.wavelet[(size_t)unit * P.wav_stride + t1];
+    const int xo = xi0 + 2 + lx;
+    const bool pxo = pml_x(xo);
+    Pml mx{0, 0, 0, 0};
+    if (pxo && xo < g.nx - 2) {
+        const float bb = pbx[xo];
+        mx = Pml{pax[xo], 1.0f + bb, bb, pdbx[xo]};
+    }
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+        const int lz = tid / TX + RZ * k;
+        if (!inner_cell(lx, lz)) continue;
+        const int x = xo, z = zi0 + 2 + lz;
+        const int i = z * g.nx + x;
+        const int li = (lz + 2) * BX + (lx + 2);
+        const float u = sU[li];
+        const bool pz = pml_z(z);
+        Pml mz{0, 0, 0, 0};
+        if (pz) {
+            const float bb = pbz[z];
+            mz = Pml{paz[z], 1.0f + bb, bb, pdbz[z]};
+        }
+        float psx = ppx[k], zex = pzx[k], psz = ppz[k], zez = pzz[k];
+        const float* AXc = sAX + lz * BX + (lx + 2);
+        const float* AZc = sAZ + (lz + 2) * TX + lx;
+        float w;
+        if (pxo && pz)
+            w = fwd_w<true, true>(sU, li, BX, u, g.rh, g.rh2, AXc, mx, psx, zex, AZc, TX, mz, psz, zez);
+        else if (pxo)
+            w = fwd_w<true, false>(sU, li, BX, u, g.rh, g.rh2, AXc, mx, psx, zex, AZc, TX, mz, psz, zez);
+        else if (pz)
+            w = fwd_w<false, true>(sU, li, BX, u, g.rh, g.rh2, AXc, mx, psx, zex, AZc, TX, mz, psz, zez);
+        else
+            w = fwd_w<false, false>(sU, li, BX, u, g.rh, g.rh2, AXc, mx, psx, zex, AZc, TX, mz, psz, zez);
+        const float v = clampv(pv[k], g.vmin, g.vmax);
+        if (STORE) {
+            const size_t q = (size_t)(z - 2) * g.ni + (x - 2);
+            P.snaps[((size_t)(t1 - P.snap_t0) * P.units + ul) * g.inner + q] = 2.0f * v * g.dt2 * w;
+        }
+        float next = v * v * g.dt2 * w + 2.0f * u - ut[k];
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+            if (scell[c] == i && sw[c] != 0.0f) next += -(v * v * g.dt2) * sw[c] * f_b;
+        O0[i] = u;
+        O1[i] = next;
+        if (pxo) {
+            PXw[i] = psx;
+            ZXw[i] = zex;
+        }
+        if (pz) {
+            PZw[i] = psz;
+            ZZw[i] = zez;
+        }
+    }
+}
+
+// Gathers of step P.t from wavefield buffer P.u_cur, one workgroup a unit.
+__global__ __launch_bounds__(NT) void k_fwd_record(StepParams P) {
+    const Grid& g = P.g;
+    const int ul = blockIdx.x, unit = P.unit0 + ul;
+    const size_t cells = (size_t)g.nx * g.nz;
+    const float* U = P.state + ((size_t)fwd_u_arr(P.u_cur) * P.units + ul) * cells;
+    float* tr = P.traces + ((size_t)unit * g.nt + P.t) * g.nrec;
+    const bool on = unit_active(P.geo, unit);
+    for (int r = threadIdx.x; r < g.nrec; r += NT) {
+        float acc = 0.0f;
+        if (on) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) acc += P.geo.rec_w[r * 4 + c] * U[P.geo.rec_cell[r * 4 + c]];
+        }
+        tr[r] = acc;
     }
 }
 
@@ -872,6 +1229,14 @@ hipError_t fwd_step_t(const StepParams& p, int units, bool store, bool born, hip
 }
 
 template <class T>
+hipError_t fwd_step2_t(const StepParams& p, int units, bool store, hipStream_t st) {
+    const dim3 grid((p.g.ni + T::TX - 1) / T::TX, (p.g.nzi + T::TZ - 1) / T::TZ, units);
+    void (*fn)(StepParams) = store ? st::k_fwd_step2<T, true> : st::k_fwd_step2<T, false>;
+    hipLaunchKernelGGL(fn, grid, dim3(st::NT), 0, st, ordered(p, grid));
+    return hipGetLastError();
+}
+
+template <class T>
 hipError_t adj_step_t(const StepParams& p, int units, hipStream_t st) {
     const dim3 grid((p.g.ni + T::TX - 1) / T::TX, (p.g.nzi + T::TZ - 1) / T::TZ, units);
     void (*fn)(StepParams) = p.dwav ? st::k_adj_step<T, true> : st::k_adj_step<T, false>;
@@ -901,6 +1266,23 @@ hipError_t launch_fwd_step(const StepParams& p, int units, bool store, bool born
         case 3: return fwd_step_t<T3>(p, units, store, born, st);
         default: return fwd_step_t<T0>(p, units, store, born, st);
     }
+}
+
+hipError_t launch_fwd_step2(const StepParams& p, int units, bool store, hipStream_t st) {
+    if (units < 1 || units > 65535 || p.t < 0 || p.t + 1 >= p.g.nt || (p.trace_prev && p.t < 1)) return hipErrorInvalidValue;
+    if (store && (p.t < p.snap_t0 || p.t + 1 >= p.snap_t0 + p.seg_len)) return hipErrorInvalidValue;
+    switch (p.tile) {
+        case 1: return fwd_step2_t<T1>(p, units, store, st);
+        case 2: return fwd_step2_t<T2>(p, units, store, st);
+        case 3: return fwd_step2_t<T3>(p, units, store, st);
+        default: return fwd_step2_t<T0>(p, units, store, st);
+    }
+}
+
+hipError_t launch_fwd_record(const StepParams& p, int units, hipStream_t st) {
+    if (units < 1 || p.t < 0 || p.t >= p.g.nt || !p.traces) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(st::k_fwd_record, dim3(units), dim3(st::NT), 0, st, p);
+    return hipGetLastError();
 }
 
 hipError_t launch_adj_step2(const StepParams& p, int units, hipStream_t st) {

@@ -108,9 +108,25 @@ struct StepParams {
     // for two-step launches the two written ones) and the half (0 / 1) the
     // C-PML memory variables are read from (the other half is written)
     int lam_cur, lam_prev, lam_out0, lam_out1, half;
+    // forward state buffers: wavefield buffers 0..3 (u^t, u^{t-1}, and for
+    // two-step launches the two written ones, u^{t+1} and u^{t+2}), the half
+    // psi is read from (the other half is written) and the half holding zeta
+    // (a single step updates it in place, a two-step launch writes the other
+    // half); trace_prev: a two-step launch also records the gathers of step
+    // t - 1 (u^{t-1}, which the launch before it computed but could not record)
+    int u_cur, u_prev, u_out0, u_out1, ps_half, ze_half, trace_prev;
     int xcd_order;           // set by the launchers: tiles renumbered per XCD (large grids)
 };
 constexpr int kStreamTiles = 4;
+
+// Arrays of the streamed forward state, each [units, nx * nz]: wavefield
+// buffers 0..3, psi_x halves (2, 3), psi_z halves (4, 5), zeta_x and zeta_z
+// halves.  Single steps use wavefield buffers 0 and 1, the psi halves by
+// turns and one zeta half: arrays 0..7.
+constexpr int kFwdArrays = 12;
+__host__ __device__ inline int fwd_u_arr(int k) { return k < 2 ? k : k + 6; }
+__host__ __device__ inline int fwd_zx_arr(int h) { return h ? 10 : 6; }
+__host__ __device__ inline int fwd_zz_arr(int h) { return h ? 11 : 7; }
 
 // Band launches of the resident kernels: a unit whose grid does not fit one
 // workgroup is cut into P row bands, one workgroup each, which exchange their
@@ -125,6 +141,7 @@ struct BandLaunch {
     int* next;                  // task counter, zero at launch
     int* err;                   // set to 1 when a wait gives up
     int spin;                   // polls before a wait gives up (< 0: at once)
+    int fixed;                  // 1: the compiled-grid band kernels where the grid and split have them
 };
 // The split of a grid too large for one workgroup into bands that fit (P in
 // [2, kMaxBands], boundaries in the interior rows at least one adjoint strip
@@ -139,6 +156,11 @@ hipError_t launch_illum(const Grid& g, const float* snaps, int nb, float* out, h
 hipError_t launch_causal_fir(const float* x, long long nseries, int nt, int nrec, const float* h, int L, float* y,
                              hipStream_t st);
 hipError_t launch_fwd_step(const StepParams& p, int units, bool store, bool born, hipStream_t st);
+// Two forward steps (t, t + 1) per launch; no Born variant.
+hipError_t launch_fwd_step2(const StepParams& p, int units, bool store, hipStream_t st);
+// Gathers of step p.t from wavefield buffer p.u_cur (after a two-step launch,
+// whose second step's input no launch of its own records).
+hipError_t launch_fwd_record(const StepParams& p, int units, hipStream_t st);
 hipError_t launch_adj_step(const StepParams& p, int units, hipStream_t st);
 // Two adjoint steps (t, t - 1) per launch; p.dwav must be null.
 hipError_t launch_adj_step2(const StepParams& p, int units, hipStream_t st);
@@ -182,6 +204,14 @@ bool launch_queue_fx(const QueueParams& q, bool adjoint, int workgroups, hipStre
 hipError_t launch_flag_nan(const int* err, float* loss, int B, float* grad, long long nvis, hipStream_t st);
 size_t forward_lds_bytes_fx(const Grid& g);
 size_t adjoint_lds_bytes_fx(const Grid& g, const Geometry& geo);
+// Row-band kernels of the compiled grids (the 100 x 100 and 141 x 141
+// models, pml 20): band_split_fx gives the compiled split of such a grid
+// (false for another grid, or receivers outside its top band); the launchers
+// return false when the grid or the split b is not a compiled one.
+bool band_split_fx(const Grid& g, const Geometry& geo, int* P, int bounds[kMaxBands + 1]);
+bool launch_forward_band_fx(const FwdParams& p, int units, bool store, const BandLaunch& b, hipStream_t st,
+                            hipError_t* err);
+bool launch_adjoint_band_fx(const AdjParams& p, int units, const BandLaunch& b, hipStream_t st, hipError_t* err);
 hipError_t launch_adjoint(const AdjParams& p, int units, hipStream_t st);
 
 // Misfit: energy (f64), residual (p-d)*scale and the reference's 256-chunk

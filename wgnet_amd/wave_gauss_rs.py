@@ -133,6 +133,23 @@ class PyWaveEngine:
         results are identical either way."""
         _lib.check(self._lib.wg_set_time_block(self._h, int(steps)))
 
+    def set_forward_time_block(self, steps: int) -> None:
+        """Forward time steps per HBM-streamed launch (1, the default, or 2);
+        results are identical either way."""
+        _lib.check(self._lib.wg_set_forward_time_block(self._h, int(steps)))
+
+    @property
+    def bands(self) -> int:
+        """Row bands per (model, shot) unit on the band kernels (0: none)."""
+        return int(self._lib.wg_engine_bands(self._h))
+
+    def set_band_split(self, on) -> None:
+        """Run a grid that splits into row bands on the band kernels (True or
+        1, the default: compiled for the grid where it has them; 2: the
+        runtime-geometry band kernels) or on the HBM-streamed kernels (False
+        or 0); results are identical."""
+        _lib.check(self._lib.wg_set_band_split(self._h, int(on)))
+
     def set_stream_tile(self, tile: int) -> None:
         """Tile shape of the HBM-streamed kernels (0 .. wg_stream_tiles() - 1)."""
         _lib.check(self._lib.wg_set_stream_tile(self._h, int(tile)))
