@@ -232,11 +232,17 @@ int32_t wg_engine_streamed(const wg_engine *e);
  * The 100 x 100 and 141 x 141 grids (pml 20) have band kernels compiled for
  * their geometry and split (2 and 4 bands); other grids run runtime-geometry
  * band kernels.  Results equal the HBM-streamed kernels' bit for bit either
- * way.  wg_set_band_split(e, on): 1 (default) the band kernels, compiled
- * where the grid has them; 2 the runtime-geometry band kernels only; 0 the
- * HBM-streamed kernels.  The band kernels serve simulate and the loss and
- * gradient when a model's snapshots fit the snapshot budget; the Born
- * operator, the wavelet gradient and checkpointed adjoints stay streamed. */
+ * way.  The band kernels serve simulate and the loss and gradient when a
+ * model's snapshots fit the snapshot budget; the Born operator, the wavelet
+ * gradient and checkpointed adjoints stay streamed.
+ * The reference's 70 x 70 grid fits one workgroup, but a call with at most a
+ * quarter (half) as many (model, shot) units as the device has compute units
+ * runs it as 4 (2) compiled bands, so that a small batch is not bound by one
+ * workgroup's step latency; results equal the whole-unit kernels' bit for
+ * bit.  The Born forward and the wavelet-gradient adjoint stay whole-unit.
+ * wg_set_band_split(e, on): 1 (default) the band kernels, compiled where the
+ * grid has them; 2 the runtime-geometry band kernels only; 0 no bands (the
+ * HBM-streamed kernels, or whole-unit workgroups on the 70 x 70 grid). */
 int32_t wg_engine_bands(const wg_engine *e);
 int32_t wg_set_band_split(wg_engine *e, int32_t on);
 /* Tile shapes of the HBM-streamed kernels (no reference counterpart): a

@@ -151,7 +151,7 @@ struct wg_engine {
     int bands = 0;                 // bands per unit (0: the grid does not split)
     int band_bounds[wg::kMaxBands + 1] = {};
     int band_on = 1;               // 0: streamed kernels, 1: band kernels (compiled where the grid has them), 2: runtime-geometry band kernels
-    DevBuf bxbuf, bqflags;         // exchange words; [next, err]
+    DevBuf bxbuf[2], bqflags[2];   // per band-launch slot (one per stream in flight): exchange words; [next, err]
     DevBuf qstate, qflags;         // work queues: per-unit part state; counters, error flag, done flags
     hipStream_t last = nullptr;    // stream of the last enqueued call
     bool last_valid = false;
@@ -394,20 +394,40 @@ int stream_simulate(wg_engine* e, int b0, int nb, const float* vel, const float*
 
 bool use_bands(const wg_engine* e) { return e->streamed && e->bands > 1 && e->band_on; }
 
-// Buffers and counters of a band launch over `units` units: exchange words
+// Band split for a call over `units` units of a grid that fits one workgroup
+// (the compiled 70 x 70 one): 4 or 2 bands when so few units would leave
+// most compute units idle, else 0 (whole-unit kernels).
+int small_bands(const wg_engine* e, int units, int bounds[wg::kMaxBands + 1]) {
+    if (e->streamed || e->band_on == 0) return 0;
+    wg::Geometry gh{};
+    gh.band_z0 = e->band_z0;
+    gh.band_rows = e->band_rows;
+    int P = 0;
+    return wg::band_split_small_fx(e->g, gh, units, e->cus, &P, bounds) ? P : 0;
+}
+
+// Buffers and counters of a band launch over `units` units in split (P,
+// bounds) (P = 0: the engine's split), in buffer slot `slot` (band launches
+// that may run at the same time use different slots): exchange words
 // (zeroed: no step has tag 0) and [next, err].  Clears err when `clear_err`.
-int band_setup(wg_engine* e, int units, bool clear_err, hipStream_t st, wg::BandLaunch* bl) {
-    const int P = e->bands;
+int band_setup(wg_engine* e, int units, bool clear_err, hipStream_t st, wg::BandLaunch* bl, int P = 0,
+               const int* bounds = nullptr, int slot = 0) {
+    if (P == 0) {
+        P = e->bands;
+        bounds = e->band_bounds;
+    }
+    DevBuf& xb = e->bxbuf[slot];
+    DevBuf& qf = e->bqflags[slot];
     const size_t words = wg::band_xbuf_words(e->g, units, P);
     int rc;
-    if ((rc = ensure(e, e->bxbuf, words * sizeof(unsigned long long)))) return rc;
-    if ((rc = ensure(e, e->bqflags, 2 * sizeof(int)))) return rc;
-    int* q = (int*)e->bqflags.p;
-    LAUNCH(wg::launch_zero((float*)e->bxbuf.p, 2 * words, st));  // 0 bits
+    if ((rc = ensure(e, xb, words * sizeof(unsigned long long)))) return rc;
+    if ((rc = ensure(e, qf, 2 * sizeof(int)))) return rc;
+    int* q = (int*)qf.p;
+    LAUNCH(wg::launch_zero((float*)xb.p, 2 * words, st));  // 0 bits
     LAUNCH(wg::launch_zero((float*)q, clear_err ? 2 : 1, st));
     bl->P = P;
-    for (int i = 0; i <= wg::kMaxBands; ++i) bl->bounds[i] = e->band_bounds[i];
-    bl->xbuf = (unsigned long long*)e->bxbuf.p;
+    for (int i = 0; i <= wg::kMaxBands; ++i) bl->bounds[i] = i <= P ? bounds[i] : 0;
+    bl->xbuf = (unsigned long long*)xb.p;
     bl->next = q;
     bl->err = q + 1;
     bl->spin = e->qspin < 0 ? -1 : std::min(e->qspin, 1 << 22);
@@ -416,31 +436,34 @@ int band_setup(wg_engine* e, int units, bool clear_err, hipStream_t st, wg::Band
 }
 
 // Forward of models [b0, b0 + nb) on the band kernels: gathers, and with
-// `store` every snapshot (indexed by the unit's position in the chunk).
+// `store` every snapshot (indexed by the unit's position in the chunk, from
+// unit snap_unit_off of the snapshot buffer).  P, bounds, slot: band_setup's.
 int band_forward(wg_engine* e, int b0, int nb, const float* vel, const float* wav, long long wav_stride,
-                 float* traces, bool store, bool clear_err, hipStream_t st) {
+                 float* traces, bool store, bool clear_err, hipStream_t st, int P = 0, const int* bounds = nullptr,
+                 int slot = 0, size_t snap_unit_off = 0) {
     const int units = nb * e->cfg.shots;
     wg::BandLaunch bl{};
     int rc;
-    if ((rc = band_setup(e, units, clear_err, st, &bl))) return rc;
+    if ((rc = band_setup(e, units, clear_err, st, &bl, P, bounds, slot))) return rc;
     wg::FwdParams p = fwd_params(e, vel, traces);
     p.wavelet = wav;
     p.wav_stride = wav_stride;
     p.unit0 = b0 * e->cfg.shots;
-    p.snaps = store ? (float*)e->snaps.p : nullptr;
+    p.snaps = store ? (float*)e->snaps.p + snap_unit_off * e->g.nt * e->g.inner : nullptr;
     Timed tm(e, &e->ev_fwd, st);
     LAUNCH(wg::launch_forward_band(p, units, store, bl, st));
     return WG_OK;
 }
 
-int band_adjoint(wg_engine* e, int b0, int nb, const float* vel, const float* resid, float* gunit, hipStream_t st) {
+int band_adjoint(wg_engine* e, int b0, int nb, const float* vel, const float* resid, float* gunit, hipStream_t st,
+                 int P = 0, const int* bounds = nullptr, int slot = 0, size_t snap_unit_off = 0) {
     const int units = nb * e->cfg.shots;
     wg::BandLaunch bl{};
     int rc;
-    if ((rc = band_setup(e, units, false, st, &bl))) return rc;
+    if ((rc = band_setup(e, units, false, st, &bl, P, bounds, slot))) return rc;
     wg::AdjParams p = adj_params(e, vel, resid, gunit);
     p.unit0 = b0 * e->cfg.shots;
-    p.snaps = (const float*)e->snaps.p;
+    p.snaps = (const float*)e->snaps.p + snap_unit_off * e->g.nt * e->g.inner;
     Timed tm(e, &e->ev_adj, st);
     LAUNCH(wg::launch_adjoint_band(p, units, bl, st));
     return WG_OK;
@@ -453,10 +476,17 @@ int band_simulate(wg_engine* e, int b0, int nb, const float* vel, const float* w
     return band_forward(e, b0, nb, vel, wav, wav_stride, traces, false, true, st);
 }
 
+// Forward of models [b0, b0 + nb) of a grid that fits one workgroup (and
+// the non-storing forward of any grid); few units run as bands (buffer slot
+// `slot`: callers running two at once pass different slots).
 int run_chunk_forward(wg_engine* e, int b0, int nb, const float* vel, float* traces, bool store, hipStream_t st,
-                      size_t snap_unit_off = 0) {
+                      size_t snap_unit_off = 0, int slot = 0) {
     if (use_bands(e) && !store) return band_simulate(e, b0, nb, vel, wav_ptr(e), wav_stride(e), traces, st);
     if (e->streamed && !store) return stream_simulate(e, b0, nb, vel, wav_ptr(e), wav_stride(e), traces, st);
+    int bounds[wg::kMaxBands + 1];
+    if (const int P = small_bands(e, nb * e->cfg.shots, bounds))
+        return band_forward(e, b0, nb, vel, wav_ptr(e), wav_stride(e), traces, store, true, st, P, bounds, slot,
+                            snap_unit_off);
     wg::FwdParams p = fwd_params(e, vel, traces);
     p.unit0 = b0 * e->cfg.shots;
     p.snaps = store ? (float*)e->snaps.p + snap_unit_off * e->g.nt * e->g.inner : nullptr;
@@ -466,7 +496,11 @@ int run_chunk_forward(wg_engine* e, int b0, int nb, const float* vel, float* tra
 }
 
 int run_chunk_adjoint(wg_engine* e, int b0, int nb, const float* vel, const float* resid, float* gunit, float* dwav,
-                      hipStream_t st, size_t snap_unit_off = 0) {
+                      hipStream_t st, size_t snap_unit_off = 0, int slot = 0) {
+    int bounds[wg::kMaxBands + 1];
+    if (!dwav)
+        if (const int P = small_bands(e, nb * e->cfg.shots, bounds))
+            return band_adjoint(e, b0, nb, vel, resid, gunit, st, P, bounds, slot, snap_unit_off);
     wg::AdjParams p = adj_params(e, vel, resid, gunit);
     p.unit0 = b0 * e->cfg.shots;
     p.snaps = (const float*)e->snaps.p + snap_unit_off * e->g.nt * e->g.inner;
@@ -819,7 +853,7 @@ int loss_grad_core(wg_engine* e, int B, const float* vel, const float* obs, floa
         }
         LAUNCH(wg::launch_loss_sum((const float*)e->partials.p, chunks, B, energy, loss, st));
         LAUNCH(wg::launch_fold(g, (const float*)e->gunit.p, vel, energy, B, grad, st));
-        LAUNCH(wg::launch_flag_nan((const int*)e->bqflags.p + 1, loss, B, grad, (long long)g.nvx * g.nvz, st));
+        LAUNCH(wg::launch_flag_nan((const int*)e->bqflags[0].p + 1, loss, B, grad, (long long)g.nvx * g.nvz, st));
         return WG_OK;
     }
     if (e->streamed) {
@@ -863,12 +897,12 @@ int loss_grad_core(wg_engine* e, int B, const float* vel, const float* obs, floa
             hipStream_t cs = sts[i & 1];
             const size_t off = (size_t)(i & 1) * wave * S;
             if (i > 0) HIPCK(hipStreamWaitEvent(cs, e->ev_pipe[(i - 1) & 1], 0));
-            if ((rc = run_chunk_forward(e, b0, nb, vel, traces, true, cs, off))) return rc;
+            if ((rc = run_chunk_forward(e, b0, nb, vel, traces, true, cs, off, i & 1))) return rc;
             HIPCK(hipEventRecord(e->ev_pipe[i & 1], cs));
             LAUNCH(wg::launch_residual(traces + b0 * per, obs + b0 * per, scale + b0, per, nb,
                                        (float*)e->resid.p + b0 * per, (float*)e->partials.p + b0 * chunks, cs));
             if ((rc = run_chunk_adjoint(e, b0, nb, vel, (const float*)e->resid.p, (float*)e->gunit.p, nullptr, cs,
-                                        off)))
+                                        off, i & 1)))
                 return rc;
         }
         // join: the caller's stream waits for everything queued on the side stream
@@ -1080,7 +1114,7 @@ void wg_engine_destroy(wg_engine* e) {
         if (p) (void)hipFree(p);
     for (DevBuf* b : {&e->snaps, &e->traces, &e->resid, &e->gunit, &e->partials, &e->scale, &e->energy, &e->vel,
                       &e->obs, &e->loss, &e->grad, &e->fstate, &e->astate, &e->ckpt, &e->qstate, &e->qflags,
-                      &e->bxbuf, &e->bqflags})
+                      &e->bxbuf[0], &e->bqflags[0], &e->bxbuf[1], &e->bqflags[1]})
         if (b->p) (void)hipFree(b->p);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     if (e->side) {
@@ -1273,7 +1307,7 @@ int32_t wg_loss_grad_true_dev(wg_engine* e, int32_t B, const float* vel_dev, con
     if ((rc = ensure_pipeline(e))) return rc;
     HIPCK(hipEventRecord(e->ev_pipe[0], st));
     HIPCK(hipStreamWaitEvent(e->side, e->ev_pipe[0], 0));
-    if ((rc = run_chunk_forward(e, 0, B, true_dev, obs_dev, false, e->side))) return rc;
+    if ((rc = run_chunk_forward(e, 0, B, true_dev, obs_dev, false, e->side, 0, 1))) return rc;
     HIPCK(hipEventRecord(e->ev_pipe[1], e->side));
     if ((rc = run_chunk_forward(e, 0, B, vel_dev, traces, true, st))) return rc;
     HIPCK(hipStreamWaitEvent(st, e->ev_pipe[1], 0));

@@ -205,10 +205,16 @@ hipError_t launch_flag_nan(const int* err, float* loss, int B, float* grad, long
 size_t forward_lds_bytes_fx(const Grid& g);
 size_t adjoint_lds_bytes_fx(const Grid& g, const Geometry& geo);
 // Row-band kernels of the compiled grids (the 100 x 100 and 141 x 141
-// models, pml 20): band_split_fx gives the compiled split of such a grid
+// models, pml 20, and small batches of the 70 x 70 one): band_split_fx gives
+// the compiled split of a grid too large for one workgroup
 // (false for another grid, or receivers outside its top band); the launchers
 // return false when the grid or the split b is not a compiled one.
 bool band_split_fx(const Grid& g, const Geometry& geo, int* P, int bounds[kMaxBands + 1]);
+// The 70 x 70 model (pml 20) as 4 or 2 bands when `units` units of that many
+// bands fit `cus` compute units (one band workgroup per compute unit): the
+// split, or false (another grid, or too many units).
+bool band_split_small_fx(const Grid& g, const Geometry& geo, int units, int cus, int* P,
+                         int bounds[kMaxBands + 1]);
 bool launch_forward_band_fx(const FwdParams& p, int units, bool store, const BandLaunch& b, hipStream_t st,
                             hipError_t* err);
 bool launch_adjoint_band_fx(const AdjParams& p, int units, const BandLaunch& b, hipStream_t st, hipError_t* err);

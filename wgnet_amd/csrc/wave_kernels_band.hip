@@ -1,6 +1,9 @@
-// Row-band kernels of the compiled grids: the 100 x 100 and 141 x 141 models
-// (20-cell C-PML: 144 x 144 and 185 x 185 padded) as 2 and 4 row bands, one
-// workgroup each, on the device templates of fixed_kernels.h.
+// Row-band kernels of the compiled grids, on the device templates of
+// fixed_kernels.h: the 100 x 100 and 141 x 141 models (20-cell C-PML:
+// 144 x 144 and 185 x 185 padded), too large for one workgroup, as 2 and 4
+// row bands; and the reference's 70 x 70 model (114 x 114) as 2 or 4 bands
+// when a batch has so few units that whole-unit workgroups would leave most
+// compute units idle.
 #include "fixed_kernels.h"
 
 namespace wg {
@@ -91,8 +94,11 @@ constexpr int band_adj_floats() {
     }
 }
 
+using G70 = Geo<70, 70, 20>;
 using G100 = Geo<100, 100, 20>;
 using G141 = Geo<141, 141, 20>;
+using S70x2 = Split<G70, 55>;           // 114 x 114 padded: two bands of 55 rows
+using S70x4 = Split<G70, 28, 55, 82>;   // four bands of 28, 27, 27 and 28 rows
 using S100 = Split<G100, 70>;           // 144 x 144 padded: two bands of 70 rows
 using S141 = Split<G141, 44, 90, 136>;  // 185 x 185 padded: 44, 46, 46 and 45 rows
 
@@ -143,14 +149,32 @@ hipError_t adj_band_fx_t(const AdjParams& p, int units, const BandLaunch& b, hip
 
 }  // namespace
 
+// One receiver per thread, every receiver row and residual CSR row in the
+// top band [0, top).
+static bool receivers_fit(const Grid& g, const Geometry& geo, int top) {
+    if (g.nrec > fx::NT) return false;
+    return geo.band_rows <= 0 || (geo.band_z0 - 2 >= 0 && geo.band_z0 + geo.band_rows - 1 - 2 < top);
+}
+
 bool band_split_fx(const Grid& g, const Geometry& geo, int* P, int bounds[kMaxBands + 1]) {
     int q = 0, b[kMaxBands + 1] = {};
     if (is_grid(g, 100)) split_of<fx::S100>(&q, b);
     else if (is_grid(g, 141)) split_of<fx::S141>(&q, b);
     else return false;
-    // one receiver per thread, every receiver row and residual CSR row in the top band
-    if (g.nrec > fx::NT) return false;
-    if (geo.band_rows > 0 && (geo.band_z0 - 2 < 0 || geo.band_z0 + geo.band_rows - 1 - 2 >= b[1])) return false;
+    if (!receivers_fit(g, geo, b[1])) return false;
+    *P = q;
+    for (int p = 0; p <= q; ++p) bounds[p] = b[p];
+    return true;
+}
+
+bool band_split_small_fx(const Grid& g, const Geometry& geo, int units, int cus, int* P,
+                         int bounds[kMaxBands + 1]) {
+    if (!is_grid(g, 70) || units < 1) return false;
+    int q = 0, b[kMaxBands + 1] = {};
+    if ((long long)units * fx::S70x4::P <= cus) split_of<fx::S70x4>(&q, b);
+    else if ((long long)units * fx::S70x2::P <= cus) split_of<fx::S70x2>(&q, b);
+    else return false;
+    if (!receivers_fit(g, geo, b[1])) return false;
     *P = q;
     for (int p = 0; p <= q; ++p) bounds[p] = b[p];
     return true;
@@ -160,6 +184,8 @@ bool launch_forward_band_fx(const FwdParams& p, int units, bool store, const Ban
                             hipError_t* err) {
     if (is_grid(p.g, 100) && same_split<fx::S100>(b)) *err = fwd_band_fx_t<fx::S100>(p, units, store, b, st);
     else if (is_grid(p.g, 141) && same_split<fx::S141>(b)) *err = fwd_band_fx_t<fx::S141>(p, units, store, b, st);
+    else if (is_grid(p.g, 70) && same_split<fx::S70x4>(b)) *err = fwd_band_fx_t<fx::S70x4>(p, units, store, b, st);
+    else if (is_grid(p.g, 70) && same_split<fx::S70x2>(b)) *err = fwd_band_fx_t<fx::S70x2>(p, units, store, b, st);
     else return false;
     return true;
 }
@@ -168,6 +194,8 @@ bool launch_adjoint_band_fx(const AdjParams& p, int units, const BandLaunch& b, 
     if (p.dwav) return false;
     if (is_grid(p.g, 100) && same_split<fx::S100>(b)) *err = adj_band_fx_t<fx::S100>(p, units, b, st);
     else if (is_grid(p.g, 141) && same_split<fx::S141>(b)) *err = adj_band_fx_t<fx::S141>(p, units, b, st);
+    else if (is_grid(p.g, 70) && same_split<fx::S70x4>(b)) *err = adj_band_fx_t<fx::S70x4>(p, units, b, st);
+    else if (is_grid(p.g, 70) && same_split<fx::S70x2>(b)) *err = adj_band_fx_t<fx::S70x2>(p, units, b, st);
     else return false;
     return true;
 }
