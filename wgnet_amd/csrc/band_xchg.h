@@ -55,8 +55,9 @@ __host__ __device__ inline Band split_band(const Grid& g, const int* bounds, int
 // write back and invalidate the whole L2 of the XCD at every step: several
 // microseconds more per step.)  A neighbour overwrites a slot two steps
 // later, after loading this band's rows of the step in between, which this
-// band stores only after it has loaded the slot.  Wave 0 does the exchange;
-// the other waves wait at the barrier that follows.
+// band stores only after it has loaded the slot (every thread's loads end
+// before the barrier at the end of the exchange, and its next stores come
+// after the next step's barriers).
 // ---------------------------------------------------------------------------
 struct Xchg {
     unsigned long long* buf;  // [launch units][P][2 sides (top rows, bottom rows)][2 parities][2 rows][nx]
@@ -69,46 +70,62 @@ __device__ __forceinline__ unsigned long long* xslot(const Xchg& X, int nx, int 
     return X.buf + ((((size_t)u * X.P + p) * 2 + side) * 2 + par) * 2 * nx;
 }
 
-__device__ __forceinline__ void xput(unsigned long long* d, const float* src, int n, unsigned k, int lane) {
-    for (int j = lane; j < n; j += 64)
-        __hip_atomic_store(d + j, ((unsigned long long)k << 32) | __float_as_uint(src[j]), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Loads n words of step k into dst; false when a word never arrived.
-__device__ __forceinline__ bool xget(float* dst, const unsigned long long* s, int n, unsigned k, int lane, int spin) {
-    for (int j = lane; j < n; j += 64) {
-        unsigned long long w = __hip_atomic_load(s + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        for (int c = 0; (unsigned)(w >> 32) != k; ++c) {
-            if (c > spin) return false;
-            __builtin_amdgcn_s_sleep(1);
-            w = __hip_atomic_load(s + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        dst[j] = __uint_as_float((unsigned)w);
-    }
-    return true;
-}
-
 // Exchange of field F (LDS rows of band bd, nx floats each) for unit u (its
 // position in the launch), band p, after step number k >= 1.  Every thread
-// calls it; returns false (in every thread) when a wait gave up.
+// of the workgroup calls it and moves at most one word of each neighbour's
+// rows per round (2 nx words a side, so one round for nx <= 256 with 512
+// threads): its own edge words go out first, then its loads from both
+// neighbours are issued together and each is polled until it carries step k.
+// So the exchange costs about one round trip through the memory system, not
+// one per word a thread handles.  Returns false (in every thread) when a
+// wait gave up.
 __device__ __forceinline__ bool band_exchange(const Xchg& X, float* F, const Band& bd, int nx, int u, int p, int k,
                                               int* fail_s) {
-    const int par = k & 1, n2 = 2 * nx, lane = threadIdx.x;
-    if (lane < 64) {
-        if (bd.up) xput(xslot(X, nx, u, p, 0, par), F + 2 * nx, n2, k, lane);                // first two own rows
-        if (bd.down) xput(xslot(X, nx, u, p, 1, par), F + (bd.hi - bd.lo) * nx, n2, k, lane);  // last two
-        bool ok = true;
-        if (bd.up) ok = xget(F, xslot(X, nx, u, p - 1, 1, par), n2, k, lane, X.spin);
-        if (ok && bd.down) ok = xget(F + (bd.hi - bd.lo + 2) * nx, xslot(X, nx, u, p + 1, 0, par), n2, k, lane, X.spin);
-        const bool all = __builtin_amdgcn_ballot_w64(!ok) == 0;
-        if (lane == 0) {
-            *fail_s = !all;
-            if (!all) __hip_atomic_fetch_or(X.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
+    const int par = k & 1, n2 = 2 * nx, nth = blockDim.x;
+    const unsigned long long tag = (unsigned long long)(unsigned)k << 32;
+    unsigned long long* up_out = xslot(X, nx, u, p, 0, par);    // first two own rows
+    unsigned long long* down_out = xslot(X, nx, u, p, 1, par);  // last two own rows
+    const unsigned long long* up_in = bd.up ? xslot(X, nx, u, p - 1, 1, par) : nullptr;
+    const unsigned long long* down_in = bd.down ? xslot(X, nx, u, p + 1, 0, par) : nullptr;
+    float* own_first = F + 2 * nx;
+    float* own_last = F + (bd.hi - bd.lo) * nx;
+    float* halo_up = F;
+    float* halo_down = F + (bd.hi - bd.lo + 2) * nx;
+    for (int j = threadIdx.x; j < n2; j += nth) {
+        if (bd.up)
+            __hip_atomic_store(up_out + j, tag | __float_as_uint(own_first[j]), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        if (bd.down)
+            __hip_atomic_store(down_out + j, tag | __float_as_uint(own_last[j]), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
     }
-    __syncthreads();
-    return *fail_s == 0;
+    bool ok = true;
+    for (int j = threadIdx.x; j < n2; j += nth) {
+        unsigned long long wu = 0, wd = 0;
+        if (bd.up) wu = __hip_atomic_load(up_in + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (bd.down) wd = __hip_atomic_load(down_in + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int c = 0; ok && bd.up && (unsigned)(wu >> 32) != (unsigned)k; ++c) {
+            if (c > X.spin) ok = false;
+            else {
+                __builtin_amdgcn_s_sleep(1);
+                wu = __hip_atomic_load(up_in + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        for (int c = 0; ok && bd.down && (unsigned)(wd >> 32) != (unsigned)k; ++c) {
+            if (c > X.spin) ok = false;
+            else {
+                __builtin_amdgcn_s_sleep(1);
+                wd = __hip_atomic_load(down_in + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        if (!ok) break;
+        if (bd.up) halo_up[j] = __uint_as_float((unsigned)wu);
+        if (bd.down) halo_down[j] = __uint_as_float((unsigned)wd);
+    }
+    const int failed = __syncthreads_or(ok ? 0 : 1);
+    if (failed && threadIdx.x == 0) __hip_atomic_fetch_or(X.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    (void)fail_s;
+    return failed == 0;
 }
 
 // Task of a band workgroup: one (unit, band) per workgroup, numbered in the
