@@ -181,6 +181,35 @@ int32_t wg_loss_grad_units_dev(wg_engine *e, int32_t B,
                                float *loss_dev, float *grad_dev,
                                float *traces_dev, double *energy_dev,
                                void *stream);
+/* The loss and gradient of several engines' batches in one call (no
+ * reference counterpart: an uneven batch, one engine per (grid, nt) group;
+ * the reference solves one model per engine call, physics.py:62-75).  Each
+ * group carries wg_loss_grad_units_dev's arguments for its engine, and the
+ * results are the same bits as n such calls.  When every group's grid has
+ * compiled kernels (the 70 x 70, 100 x 100 and 141 x 141 models, pml 20, on
+ * the default band split), all groups run in ONE launch: a workgroup takes a
+ * task (a unit, or one row band of a unit) from a device counter, longest
+ * tasks first, and runs its forward, residual and adjoint, with the
+ * snapshots in a slot it holds while the task runs (about one slot per
+ * compute unit, each the largest group's nt x inner floats, held by the
+ * first engine; the engines' snapshot budgets do not apply).  Otherwise the
+ * groups run one after another on `stream`.  Each engine may appear once; at
+ * most 64 groups. */
+typedef struct wg_group_args {
+    wg_engine *engine;
+    int32_t B;
+    const int32_t *src_cell_dev;
+    const float *src_w_dev;
+    const uint8_t *active_dev;
+    const float *vel_dev;
+    const float *wav_dev;
+    const float *obs_dev;
+    float *loss_dev;
+    float *grad_dev;
+    float *traces_dev;
+    double *energy_dev;
+} wg_group_args;
+int32_t wg_loss_grad_groups_dev(int32_t n, const wg_group_args *groups, void *stream);
 int32_t wg_simulate_units_dev(wg_engine *e, int32_t B,
                               const int32_t *src_cell_dev,
                               const float *src_w_dev, const uint8_t *active_dev,

@@ -66,6 +66,7 @@ SIGNATURES = {
     "wg_loss_noalloc_dev": (_i32, [_vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "wg_source_corners": (_i32, [_vp, _i32, _vp, _vp, _vp]),
     "wg_loss_grad_units_dev": (_i32, [_vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "wg_loss_grad_groups_dev": (_i32, [_i32, _vp, _vp]),
     "wg_simulate_units_dev": (_i32, [_vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "wg_adjoint_apply_dev": (_i32, [_vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp]),
     "wg_born_dev": (_i32, [_vp, _i32, _vp, _vp, _vp, _vp]),
@@ -106,6 +107,14 @@ _LIB: ctypes.CDLL | None = None
 
 class WaveEngineError(RuntimeError):
     """HIP runtime failure inside the wave engine."""
+
+
+class GroupArgs(ctypes.Structure):
+    """wg_group_args of include/wgnet_hip.h: one group of wg_loss_grad_groups_dev."""
+
+    _fields_ = [("engine", _vp), ("B", _i32), ("src_cell_dev", _vp), ("src_w_dev", _vp), ("active_dev", _vp),
+                ("vel_dev", _vp), ("wav_dev", _vp), ("obs_dev", _vp), ("loss_dev", _vp), ("grad_dev", _vp),
+                ("traces_dev", _vp), ("energy_dev", _vp)]
 
 
 def load() -> ctypes.CDLL:

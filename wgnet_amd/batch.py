@@ -7,14 +7,19 @@ crates/wave-gauss-rs/src/wave_wgpu.rs:761-988).  Here a batch of solves is
 grouped by (grid, nt); each group is one engine whose units are (solve, shot
 slot) with up to `max_shots` slots per solve.  Every unit carries its own
 source position and its own band-limited wavelet; slots beyond a solve's shot
-count are inactive padding (zero traces, zero gradient, no time steps).  A
-group runs as one launch of the resident kernels (one workgroup per unit, all
-nt steps, LDS-resident wavefield, snapshots to HBM) when its grid fits a
-workgroup, else as the HBM-streamed per-step kernels with the tile shape and
-adjoint steps per launch (one, or two with temporal blocking) that the
-autotuner picked for that grid.  Groups run side by side on a few streams
-(longest first onto the least loaded), and the whole batch (every group's
-forward, misfit and adjoint) is captured once as a HIP graph and replayed.
+count are inactive padding (zero traces, zero gradient, no time steps).
+
+By default (`fused=True`) the whole batch is one engine call
+(wg_loss_grad_groups_dev): every group's units (70 x 70) or row bands
+(100 x 100, 141 x 141) are the tasks of ONE forward launch and ONE adjoint
+launch, longest first, with each group's energy, residual, loss and fold
+around them.  With `fused=False` (or a grid without compiled kernels) each
+group is its own engine call: one launch of the resident or band kernels, or
+the HBM-streamed per-step kernels with the tile shape and adjoint steps per
+launch that the autotuner picked for that grid, the groups side by side on a
+few streams (longest first onto the least loaded).  Either way the batch is
+captured once as a HIP graph and replayed, and the results are the same
+bits.
 
 Each solve applies the causal low-pass of its passband (physics.py:18-57) to
 both its wavelet and its observed data.
@@ -69,8 +74,9 @@ class UnevenBatch:
     """Fixed batch geometry; models and observed data are set per call."""
 
     def __init__(self, solves: list[Solve], dt: float = 0.001, peak_hz: float = 15.0, h: float = 10.0,
-                 max_shots: int = 5, memory_fraction: float = 0.6, lanes: int | None = None):
+                 max_shots: int = 5, memory_fraction: float = 0.6, lanes: int | None = None, fused: bool = True):
         self.solves = list(solves)
+        self.fused = bool(fused)
         self.dt, self.peak_hz, self.h, self.max_shots = float(dt), float(peak_hz), float(h), int(max_shots)
         self._lib = _lib.load()
         self.graph: torch.cuda.CUDAGraph | None = None
@@ -104,6 +110,14 @@ class UnevenBatch:
         for g in self.groups:
             g.lane = min(range(len(load)), key=load.__getitem__)
             load[g.lane] += g.cost
+        # the arguments of the fused call (every buffer is allocated once)
+        self._group_args = (_lib.GroupArgs * len(self.groups))()
+        for k, g in enumerate(self.groups):
+            b = g.buf
+            self._group_args[k] = _lib.GroupArgs(
+                g.engine.handle, len(g.ids), b["cell"].data_ptr(), b["w"].data_ptr(), b["active"].data_ptr(),
+                b["vel"].data_ptr(), b["wav"].data_ptr(), b["obs"].data_ptr(), b["loss"].data_ptr(),
+                b["grad"].data_ptr(), b["traces"].data_ptr(), b["energy"].data_ptr())
 
     # -- set-up --------------------------------------------------------------
     def _setup_group(self, g: _Group, snap_budget: int) -> None:
@@ -238,8 +252,12 @@ class UnevenBatch:
             _p(b["obs"]), _p(b["loss"]), _p(b["grad"]), _p(b["traces"]), _p(b["energy"]), _stream()))
 
     def _run(self) -> None:
-        """Every group on its lane, forked from and joined back into the
+        """Fused: one engine call for every group on the current stream.
+        Else every group on its lane, forked from and joined back into the
         current stream (also inside a graph capture)."""
+        if self.fused:
+            _lib.check(self._lib.wg_loss_grad_groups_dev(len(self.groups), ctypes.byref(self._group_args), _stream()))
+            return
         cur = torch.cuda.current_stream()
         fork = torch.cuda.Event()
         fork.record(cur)

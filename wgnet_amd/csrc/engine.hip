@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -152,6 +153,7 @@ struct wg_engine {
     int band_bounds[wg::kMaxBands + 1] = {};
     int band_on = 1;               // 0: streamed kernels, 1: band kernels (compiled where the grid has them), 2: runtime-geometry band kernels
     DevBuf bxbuf[2], bqflags[2];   // per band-launch slot (one per stream in flight): exchange words; [next, err]
+    DevBuf mtable, mnext, mslots;  // one-launch batches led by this engine: group table, [counter, slot flags..., err], snapshot slots
     DevBuf qstate, qflags;         // work queues: per-unit part state; counters, error flag, done flags
     hipStream_t last = nullptr;    // stream of the last enqueued call
     bool last_valid = false;
@@ -1114,7 +1116,8 @@ void wg_engine_destroy(wg_engine* e) {
         if (p) (void)hipFree(p);
     for (DevBuf* b : {&e->snaps, &e->traces, &e->resid, &e->gunit, &e->partials, &e->scale, &e->energy, &e->vel,
                       &e->obs, &e->loss, &e->grad, &e->fstate, &e->astate, &e->ckpt, &e->qstate, &e->qflags,
-                      &e->bxbuf[0], &e->bqflags[0], &e->bxbuf[1], &e->bqflags[1]})
+                      &e->bxbuf[0], &e->bqflags[0], &e->bxbuf[1], &e->bqflags[1], &e->mtable, &e->mnext,
+                      &e->mslots})
         if (b->p) (void)hipFree(b->p);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     if (e->side) {
@@ -1426,6 +1429,172 @@ int32_t wg_loss_grad_units_dev(wg_engine* e, int32_t B, const int32_t* src_cell_
                            src_cell_dev, src_w_dev, active_dev);
 }
 
+
+// The loss and gradient of several engines' unit batches (wg_loss_grad_units_dev
+// arguments per group) in ONE launch, when every group's grid has compiled
+// kernels for it (70 x 70 as whole units, 100 x 100 and 141 x 141 in their
+// compiled band splits); otherwise the groups run one after another.  Each
+// task (a unit, or a band of one) runs forward, residual and adjoint with its
+// snapshots in a slot held for the task's lifetime: slots for the resident
+// workgroups (one per compute unit) and a margin, so the snapshot memory does
+// not grow with the batch.  Tasks are laid out longest first (the groups'
+// step counts times the kind's step time).
+int32_t wg_loss_grad_groups_dev(int32_t n, const wg_group_args* ga, void* stream) {
+    if (n < 1 || !ga) return fail(WG_E_INVALID, "invalid argument");
+    if (n > wg::kMultiMaxGroups) return fail(WG_E_INVALID, "at most 64 groups in one call");
+    for (int i = 0; i < n; ++i) {
+        const wg_group_args& a = ga[i];
+        if (!a.engine || a.B < 0 || !a.src_cell_dev || !a.src_w_dev || !a.vel_dev || !a.wav_dev || !a.obs_dev ||
+            !a.loss_dev || !a.grad_dev)
+            return fail(WG_E_INVALID, "invalid argument");
+    }
+    std::vector<wg_engine*> order(n);
+    for (int i = 0; i < n; ++i) order[i] = ga[i].engine;
+    std::sort(order.begin(), order.end());
+    for (int i = 1; i < n; ++i)
+        if (order[i] == order[i - 1]) return fail(WG_E_INVALID, "an engine appears in two groups");
+    for (int i = 1; i < n; ++i)
+        if (order[i]->device != order[0]->device) return fail(WG_E_INVALID, "the engines are on different devices");
+    std::vector<std::unique_lock<std::mutex>> locks;  // in address order: callers cannot deadlock
+    for (wg_engine* e : order) locks.emplace_back(e->mu);
+    std::vector<std::unique_ptr<WavScope>> wscopes;
+    std::vector<std::unique_ptr<UnitScope>> uscopes;
+    for (int i = 0; i < n; ++i) {
+        wscopes.emplace_back(new WavScope(ga[i].engine, ga[i].wav_dev));
+        uscopes.emplace_back(new UnitScope(ga[i].engine, ga[i].src_cell_dev, ga[i].src_w_dev, ga[i].active_dev));
+    }
+    HIPCK(hipSetDevice(order[0]->device));
+    hipStream_t st = (hipStream_t)stream;
+    int rc;
+    for (int i = 0; i < n; ++i)
+        if ((rc = fence(ga[i].engine, st))) return rc;
+    auto energy_of = [&](int i, double** out) -> int {
+        wg_engine* e = ga[i].engine;
+        int r;
+        if ((r = ensure(e, e->scale, (size_t)ga[i].B * sizeof(float)))) return r;
+        *out = ga[i].energy_dev;
+        if (!*out) {
+            if ((r = ensure(e, e->energy, (size_t)ga[i].B * sizeof(double)))) return r;
+            *out = (double*)e->energy.p;
+        }
+        return WG_OK;
+    };
+    // per group: its kind in the one launch, or -1 (then every group runs alone)
+    std::vector<int> kind(n, -1);
+    bool fused = true;
+    for (int i = 0; i < n && fused; ++i) {
+        wg_engine* e = ga[i].engine;
+        if (ga[i].B == 0) continue;
+        int one[wg::kMaxBands + 1] = {0, e->g.nzi};
+        if (!e->streamed) kind[i] = wg::multi_kind(e->g, 1, one);
+        else if (use_bands(e) && e->band_on == 1) kind[i] = wg::multi_kind(e->g, e->bands, e->band_bounds);
+        fused = kind[i] >= 0;
+    }
+    if (!fused) {
+        for (int i = 0; i < n; ++i) {
+            if (ga[i].B == 0) continue;
+            wg_engine* e = ga[i].engine;
+            double* energy;
+            if ((rc = energy_of(i, &energy))) return rc;
+            if ((rc = loss_grad_core(e, ga[i].B, ga[i].vel_dev, ga[i].obs_dev, ga[i].loss_dev, ga[i].grad_dev,
+                                     ga[i].traces_dev, energy, (float*)e->scale.p, true, st)))
+                return rc;
+        }
+        return WG_OK;
+    }
+    // longest tasks first: step count times the kind's forward + adjoint step
+    // time (us, measured: 70 x 70 whole unit, 100 x 100 band, 141 x 141 band)
+    const double step_us[3] = {17.6, 14.9, 13.6};
+    std::vector<int> idx;
+    for (int i = 0; i < n; ++i)
+        if (ga[i].B > 0) idx.push_back(i);
+    if (idx.empty()) return WG_OK;
+    std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) {
+        return ga[x].engine->g.nt * step_us[kind[x]] > ga[y].engine->g.nt * step_us[kind[y]];
+    });
+    wg_engine* lead = ga[0].engine;
+    std::vector<wg::MultiGroup> tab(idx.size());
+    std::vector<double*> energy(n, nullptr);
+    size_t lds = 0, slot_floats = 0;
+    int tasks = 0;
+    for (size_t k = 0; k < idx.size(); ++k) {
+        const int i = idx[k];
+        wg_engine* e = ga[i].engine;
+        const wg::Grid& g = e->g;
+        const int B = ga[i].B, S = e->cfg.shots, units = B * S;
+        const long long per = (long long)S * g.nt * g.nrec;
+        const long long chunks = (per + 255) / 256;
+        if ((rc = energy_of(i, &energy[i]))) return rc;
+        if ((rc = ensure(e, e->resid, (size_t)B * per * sizeof(float)))) return rc;
+        if ((rc = ensure(e, e->partials, (size_t)B * chunks * sizeof(float)))) return rc;
+        if ((rc = ensure(e, e->gunit, (size_t)units * g.nx * g.nz * sizeof(float)))) return rc;
+        if (!ga[i].traces_dev && (rc = ensure(e, e->traces, (size_t)B * per * sizeof(float)))) return rc;
+        float* traces = ga[i].traces_dev ? ga[i].traces_dev : (float*)e->traces.p;
+        int P = 1, bounds[wg::kMaxBands + 1] = {0, g.nzi};
+        if (kind[i] != 0) {
+            P = e->bands;
+            for (int q = 0; q <= P; ++q) bounds[q] = e->band_bounds[q];
+        }
+        const size_t words = P > 1 ? wg::band_xbuf_words(g, units, P) : 2;
+        if ((rc = ensure(e, e->bxbuf[0], words * sizeof(unsigned long long)))) return rc;
+        if ((rc = ensure(e, e->bqflags[0], 2 * sizeof(int)))) return rc;
+        wg::MultiGroup& m = tab[k];
+        m.f = fwd_params(e, ga[i].vel_dev, traces);
+        m.a = adj_params(e, ga[i].vel_dev, (const float*)e->resid.p, (float*)e->gunit.p);
+        m.b.P = P;
+        for (int q = 0; q <= wg::kMaxBands; ++q) m.b.bounds[q] = P > 1 && q <= P ? bounds[q] : 0;
+        m.b.xbuf = (unsigned long long*)e->bxbuf[0].p;
+        m.b.next = nullptr;
+        m.b.err = (int*)e->bqflags[0].p + 1;
+        m.b.spin = e->qspin < 0 ? -1 : std::min(e->qspin, 1 << 22);
+        m.b.fixed = 1;
+        m.obs = ga[i].obs_dev;
+        m.scale = (const float*)e->scale.p;
+        m.resid = (float*)e->resid.p;
+        m.kind = kind[i];
+        m.task0 = tasks;
+        tasks += units * wg::multi_tasks_per_unit(kind[i]);
+        const wg::Geometry geo = e->geo();
+        lds = std::max({lds, wg::multi_lds_bytes(kind[i], g, geo, false), wg::multi_lds_bytes(kind[i], g, geo, true)});
+        slot_floats = std::max(slot_floats, (size_t)g.nt * g.inner);
+    }
+    // snapshot slots: one per resident workgroup (one per compute unit: the
+    // kernel's LDS and registers allow no second) and a margin
+    const int ns = std::min(tasks, lead->cus + 16);
+    if ((rc = ensure(lead, lead->mtable, idx.size() * sizeof(wg::MultiGroup)))) return rc;
+    if ((rc = ensure(lead, lead->mnext, (size_t)(ns + 2) * sizeof(int)))) return rc;
+    if ((rc = ensure(lead, lead->mslots, (size_t)ns * slot_floats * sizeof(float)))) return rc;
+    int* next = (int*)lead->mnext.p;  // [counter, ns slot flags, slot error]
+    int* slot_err = next + 1 + ns;
+    for (size_t k = 0; k < idx.size(); ++k) {
+        const int i = idx[k];
+        wg_engine* e = ga[i].engine;
+        const long long per = (long long)e->cfg.shots * e->g.nt * e->g.nrec;
+        LAUNCH(wg::launch_energy(ga[i].obs_dev, per, ga[i].B, energy[i], (float*)e->scale.p, st));
+        LAUNCH(wg::launch_zero((float*)e->bxbuf[0].p, e->bxbuf[0].n / sizeof(float), st));
+        LAUNCH(wg::launch_zero((float*)e->bqflags[0].p, 2, st));
+        LAUNCH(wg::launch_multi_put((wg::MultiGroup*)lead->mtable.p, (int)k, tab[k], st));
+    }
+    LAUNCH(wg::launch_zero((float*)next, (size_t)ns + 2, st));
+    LAUNCH(wg::launch_fused((const wg::MultiGroup*)lead->mtable.p, (int)idx.size(), tasks, next,
+                            (float*)lead->mslots.p, slot_floats, next + 1, ns, slot_err, lds, st));
+    for (size_t k = 0; k < idx.size(); ++k) {
+        const int i = idx[k];
+        wg_engine* e = ga[i].engine;
+        const wg::Grid& g = e->g;
+        const long long per = (long long)e->cfg.shots * g.nt * g.nrec;
+        const long long chunks = (per + 255) / 256;
+        // the loss partials (the residual again, the same values the tasks used)
+        LAUNCH(wg::launch_residual(tab[k].f.traces, ga[i].obs_dev, (const float*)e->scale.p, per, ga[i].B,
+                                   (float*)e->resid.p, (float*)e->partials.p, st));
+        LAUNCH(wg::launch_loss_sum((const float*)e->partials.p, chunks, ga[i].B, energy[i], ga[i].loss_dev, st));
+        LAUNCH(wg::launch_fold(g, (const float*)e->gunit.p, ga[i].vel_dev, energy[i], ga[i].B, ga[i].grad_dev, st));
+        const long long nvis = (long long)g.nvx * g.nvz;
+        LAUNCH(wg::launch_flag_nan((const int*)e->bqflags[0].p + 1, ga[i].loss_dev, ga[i].B, ga[i].grad_dev, nvis, st));
+        LAUNCH(wg::launch_flag_nan(slot_err, ga[i].loss_dev, ga[i].B, ga[i].grad_dev, nvis, st));
+    }
+    return WG_OK;
+}
 
 int32_t wg_adjoint_apply_dev(wg_engine* e, int32_t B, const float* vel_dev, const float* resid_dev, float* grad_dev,
                              float* dwav_dev, float* traces_dev, void* stream) {

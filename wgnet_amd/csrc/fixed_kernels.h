@@ -761,12 +761,13 @@ struct AState {
 // (snapshots at launch position blk); st_in (t_hi < nt) restores, st_out
 // (t_lo > 0) saves the state; the gradient is written when t_lo = 0.  A band
 // (G not the whole grid) exchanges V with band bp's neighbours through X
-// after every phase 1 and writes the gradient of its own rows, NaN when a
-// wait gave up.
+// after every phase 1 (step numbers tag0 + 1, tag0 + 2, ...: a band that ran
+// its forward on the same exchange words continues that numbering) and
+// writes the gradient of its own rows, NaN when a wait gave up.
 template <class G, bool DWAV>
 __device__ __forceinline__ void adjoint_run(const AdjParams& P, float* lds, int blk, int t_hi, int t_lo,
                                             const float* st_in, float* st_out, const Xchg* X = nullptr,
-                                            int bp = 0, int* fail_s = nullptr) {
+                                            int bp = 0, int* fail_s = nullptr, int tag0 = 0) {
     static_assert(G::WHOLE || !DWAV, "bands run the adjoint without the wavelet gradient");
     using RR = Rects<G, G::REGA - 2>;
     const Grid& g = P.g;
@@ -932,7 +933,7 @@ __device__ __forceinline__ void adjoint_run(const AdjParams& P, float* lds, int 
         a_phase1<G, typename RR::CB, DWAV>(sCB, pCB, C, cx, so[5]);
         __syncthreads();
         if constexpr (!G::WHOLE) {
-            if (!band_exchange(*X, C.V, G::band(), G::NX, blk, bp, g.nt - t, fail_s)) {
+            if (!band_exchange(*X, C.V, G::band(), G::NX, blk, bp, g.nt - t + tag0, fail_s)) {
                 failed = true;
                 break;
             }

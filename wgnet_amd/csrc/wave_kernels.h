@@ -218,6 +218,39 @@ bool band_split_small_fx(const Grid& g, const Geometry& geo, int units, int cus,
 bool launch_forward_band_fx(const FwdParams& p, int units, bool store, const BandLaunch& b, hipStream_t st,
                             hipError_t* err);
 bool launch_adjoint_band_fx(const AdjParams& p, int units, const BandLaunch& b, hipStream_t st, hipError_t* err);
+
+// An uneven batch (one group per (grid, nt)) in one launch: every group's
+// units (70 x 70: one task per unit) or bands (100 x 100 and 141 x 141 in
+// their compiled splits: one task per band) are tasks of ONE launch; a
+// workgroup takes a task from a device counter and runs that unit's or
+// band's forward, the unit's residual and its adjoint, with the snapshots in
+// a slot it holds for the task's lifetime.  The table lists the groups in
+// task order.
+struct MultiGroup {
+    FwdParams f;          // the group's forward (f.snaps is set per task)
+    AdjParams a;          // and adjoint (a.resid: written by the task)
+    BandLaunch b;         // its exchange words and error flag (b.P: bands per unit, 1 for whole units)
+    const float* obs;     // [units, nt, nrec]
+    const float* scale;   // [models]
+    float* resid;         // [units, nt, nrec] (= a.resid)
+    int kind;             // multi_kind
+    int task0;            // first task of the group
+};
+constexpr int kMultiMaxGroups = 64;
+// The kind of a grid in the one-launch batch (0: 70 x 70 as whole units, 1:
+// 100 x 100 in 2 bands, 2: 141 x 141 in 4 bands; bounds must be the compiled
+// split), or -1 when the grid has no compiled kernel for it.
+int multi_kind(const Grid& g, int P, const int* bounds);
+// Tasks per unit and the dynamic LDS a group of that kind needs.
+int multi_tasks_per_unit(int kind);
+size_t multi_lds_bytes(int kind, const Grid& g, const Geometry& geo, bool adjoint);
+// Writes group entry i of the device table (a kernel: capturable).
+hipError_t launch_multi_put(MultiGroup* table, int i, const MultiGroup& m, hipStream_t st);
+// The one launch: next (task counter) zero and slot_flags (ns of them) zero
+// at launch; ns snapshot slots of slot_floats floats; err set when a task
+// found no free slot.
+hipError_t launch_fused(const MultiGroup* table, int n, int tasks, int* next, float* slots,
+                        unsigned long long slot_floats, int* slot_flags, int ns, int* err, size_t lds, hipStream_t st);
 hipError_t launch_adjoint(const AdjParams& p, int units, hipStream_t st);
 
 // Misfit: energy (f64), residual (p-d)*scale and the reference's 256-chunk
