@@ -353,13 +353,14 @@ struct FState {
 
 // Steps [t0, t1) of the forward of unit P.unit0 + blk (snapshots at launch
 // position blk).  st_in (t0 > 0) restores the state a previous part saved;
-// st_out (t1 < nt) saves it for the next part.  A band (G not the whole
+// st_out (t1 < nt) saves it for the next part; snaps_at, when set, is where
+// this unit's snapshots go (else P.snaps at position blk).  A band (G not the whole
 // grid) exchanges its edge rows with band bp's neighbours through X after
 // every step; the top band records the gathers, NaN when a wait gave up.
 template <class G, bool STORE, bool BORN>
 __device__ __forceinline__ void forward_run(const FwdParams& P, float* lds, int blk, int t0, int t1,
                                             const float* st_in, float* st_out, const Xchg* X = nullptr,
-                                            int bp = 0, int* fail_s = nullptr) {
+                                            int bp = 0, int* fail_s = nullptr, float* snaps_at = nullptr) {
     using RR = Rects<G, G::REG - 2>;
     const Grid& g = P.g;
     FCtxF C;
@@ -420,7 +421,7 @@ __device__ __forceinline__ void forward_run(const FwdParams& P, float* lds, int 
     const float* dvis = BORN ? P.dvel + (size_t)b * G::NVX * G::NVZ : nullptr;
     const float* wav = P.wavelet + (size_t)unit * P.wav_stride;
     float* traces = P.traces + (size_t)unit * g.nt * g.nrec;
-    float* snaps = STORE ? P.snaps + (size_t)blk * g.nt * G::INNER : nullptr;
+    float* snaps = STORE ? (snaps_at ? snaps_at : P.snaps + (size_t)blk * g.nt * G::INNER) : nullptr;
     const float* bsnaps = BORN ? P.born_snaps + (size_t)blk * g.nt * G::INNER : nullptr;
 
     Place<typename RR::I> pI;
@@ -767,7 +768,8 @@ struct AState {
 template <class G, bool DWAV>
 __device__ __forceinline__ void adjoint_run(const AdjParams& P, float* lds, int blk, int t_hi, int t_lo,
                                             const float* st_in, float* st_out, const Xchg* X = nullptr,
-                                            int bp = 0, int* fail_s = nullptr, int tag0 = 0) {
+                                            int bp = 0, int* fail_s = nullptr, int tag0 = 0,
+                                            const float* snaps_at = nullptr) {
     static_assert(G::WHOLE || !DWAV, "bands run the adjoint without the wavelet gradient");
     using RR = Rects<G, G::REGA - 2>;
     const Grid& g = P.g;
@@ -829,7 +831,7 @@ __device__ __forceinline__ void adjoint_run(const AdjParams& P, float* lds, int 
     const float* vvis = P.vel + (size_t)b * G::NVX * G::NVZ;
     const float* wav = P.wavelet + (size_t)unit * P.wav_stride;
     const float* resid = P.resid + (size_t)unit * g.nt * g.nrec;
-    const float* snaps = P.snaps + (size_t)blk * g.nt * G::INNER;
+    const float* snaps = snaps_at ? snaps_at : P.snaps + (size_t)blk * g.nt * G::INNER;
 
     if (tid < 4) {
         const int cell = P.geo.src_cell[s * 4 + tid];

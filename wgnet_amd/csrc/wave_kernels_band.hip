@@ -30,25 +30,26 @@ struct Split {
 };
 
 template <class S, int p, bool STORE>
-__device__ __forceinline__ void fwd_band(const FwdParams& P, float* lds, int u, int q, const Xchg& X, int* fail_s) {
+__device__ __forceinline__ void fwd_band(const FwdParams& P, float* lds, int u, int q, const Xchg& X, int* fail_s,
+                                         float* snaps_at = nullptr) {
     if constexpr (p < S::P) {
         if (q == p)
             forward_run<BandG<typename S::Grid, S::B[p], S::B[p + 1]>, STORE, false>(P, lds, u, 0, P.g.nt, nullptr,
-                                                                                    nullptr, &X, p, fail_s);
+                                                                                    nullptr, &X, p, fail_s, snaps_at);
         else
-            fwd_band<S, p + 1, STORE>(P, lds, u, q, X, fail_s);
+            fwd_band<S, p + 1, STORE>(P, lds, u, q, X, fail_s, snaps_at);
     }
 }
 
 template <class S, int p>
 __device__ __forceinline__ void adj_band(const AdjParams& P, float* lds, int u, int q, const Xchg& X, int* fail_s,
-                                         int tag0 = 0) {
+                                         int tag0 = 0, const float* snaps_at = nullptr) {
     if constexpr (p < S::P) {
         if (q == p)
             adjoint_run<BandG<typename S::Grid, S::B[p], S::B[p + 1]>, false>(P, lds, u, P.g.nt, 0, nullptr, nullptr,
-                                                                             &X, p, fail_s, tag0);
+                                                                             &X, p, fail_s, tag0, snaps_at);
         else
-            adj_band<S, p + 1>(P, lds, u, q, X, fail_s, tag0);
+            adj_band<S, p + 1>(P, lds, u, q, X, fail_s, tag0, snaps_at);
     }
 }
 
@@ -195,24 +196,23 @@ __global__ __launch_bounds__(NT) void k_fused(const MultiGroup* __restrict__ T, 
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     const BandLaunch b = M->b;
     const Xchg X{b.xbuf, b.err, b.P, b.spin};
-    // the slot holds this unit's snapshots: unit position u maps to it
+    // the slot holds this unit's snapshots; the parameters are read in place
+    // (scalar loads from the table)
     float* sbase = SL.base + (size_t)slot * SL.floats;
-    FwdParams f = M->f;
-    f.snaps = sbase - (size_t)u * f.g.nt * f.g.inner;
-    if (kind == 0) forward_run<G70, true, false>(f, lds, u, 0, f.g.nt, nullptr, nullptr);
-    else if (kind == 1) fwd_band<S100, 0, true>(f, lds, u, q, X, &fail_s);
-    else fwd_band<S141, 0, true>(f, lds, u, q, X, &fail_s);
+    const FwdParams& f = M->f;
+    if (kind == 0) forward_run<G70, true, false>(f, lds, u, 0, f.g.nt, nullptr, nullptr, nullptr, 0, nullptr, sbase);
+    else if (kind == 1) fwd_band<S100, 0, true>(f, lds, u, q, X, &fail_s, sbase);
+    else fwd_band<S141, 0, true>(f, lds, u, q, X, &fail_s, sbase);
     __syncthreads();  // the gathers and snapshots are read back by this workgroup only
     const int unit = f.unit0 + u;
     if (q == 0 && unit_active(f.geo, unit)) {
         unit_residual(*M, unit);
         __syncthreads();
     }
-    AdjParams a = M->a;
-    a.snaps = sbase - (size_t)u * a.g.nt * a.g.inner;
-    if (kind == 0) adjoint_run<G70, false>(a, lds, u, a.g.nt, 0, nullptr, nullptr);
-    else if (kind == 1) adj_band<S100, 0>(a, lds, u, q, X, &fail_s, a.g.nt);
-    else adj_band<S141, 0>(a, lds, u, q, X, &fail_s, a.g.nt);
+    const AdjParams& a = M->a;
+    if (kind == 0) adjoint_run<G70, false>(a, lds, u, a.g.nt, 0, nullptr, nullptr, nullptr, 0, nullptr, 0, sbase);
+    else if (kind == 1) adj_band<S100, 0>(a, lds, u, q, X, &fail_s, a.g.nt, sbase);
+    else adj_band<S141, 0>(a, lds, u, q, X, &fail_s, a.g.nt, sbase);
     __syncthreads();
     if (threadIdx.x == 0) __hip_atomic_store(SL.flags + slot, 0, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
