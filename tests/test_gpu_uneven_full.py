@@ -11,7 +11,9 @@ Checked:
     forward of the true model, rel-L2 <= 1e-5;
   - all 512 solves: finite losses >= 0 and finite gradients, at least one
     positive loss per group; the captured HIP graph replayed twice, with eager
-    work between, equal to the eager run bit for bit."""
+    work between, equal to the eager run bit for bit.
+The batch runs as one launch (UnevenBatch's default): every unit or row band
+a task of one kernel, forward, residual and adjoint."""
 
 from __future__ import annotations
 
