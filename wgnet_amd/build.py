@@ -2,6 +2,9 @@
 
     python -m wgnet_amd.build
 
+Each source compiles to its own object in parallel (wgnet_amd/_build/), then
+hipcc links the shared library.
+
 Flags:
   -ffp-contract=off        no FMA contraction: the kernels reproduce the
                            reference's float32 operation order exactly
@@ -15,13 +18,24 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libwgnet_hip.so")
 SOURCES = ["wave_kernels.hip", "wave_kernels_fixed.hip", "wave_kernels_band.hip", "stream_kernels.hip",
            "misfit_kernels.hip", "precond.hip", "engine.hip"]
 ARCH = os.environ.get("WGNET_OFFLOAD_ARCH", "gfx950")
+FLAGS = [
+    "-O3",
+    "-std=c++17",
+    "-ffp-contract=off",
+    "-fno-gpu-flush-denormals-to-zero",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-fPIC",
+    "-Wall",
+]
 
 
 def hipcc() -> str:
@@ -31,34 +45,34 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found: the wgnet_amd HIP library cannot be built")
 
 
-def command(out: str = LIB) -> list[str]:
-    return [
-        hipcc(),
-        f"--offload-arch={ARCH}",
-        "-O3",
-        "-std=c++17",
-        "-ffp-contract=off",
-        "-fno-gpu-flush-denormals-to-zero",
-        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-        "-fPIC",
-        "-shared",
-        "-Wall",
-        *[os.path.join(CSRC, s) for s in SOURCES],
-        "-o", out,
-        "-Wl,-rpath,/opt/rocm/lib",
-    ]
+def compile_command(src: str) -> list[str]:
+    obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
+    return [hipcc(), f"--offload-arch={ARCH}", *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
 
 
-def build(verbose: bool = False) -> str:
-    """Compiles every HIP source into libwgnet_hip.so (always rebuilds)."""
-    tmp = LIB + ".tmp"
-    cmd = command(tmp)
+def link_command(out: str = LIB) -> list[str]:
+    objs = [os.path.join(OBJ, os.path.splitext(s)[0] + ".o") for s in SOURCES]
+    return [hipcc(), f"--offload-arch={ARCH}", "-fPIC", "-shared", *objs, "-o", out, "-Wl,-rpath,/opt/rocm/lib"]
+
+
+def _run(cmd: list[str], verbose: bool) -> None:
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError("hipcc failed building libwgnet_hip.so")
+        raise RuntimeError(f"hipcc failed: {' '.join(cmd[-3:])}")
+
+
+def build(verbose: bool = False) -> str:
+    """Compiles every HIP source and links libwgnet_hip.so (always rebuilds)."""
+    os.makedirs(OBJ, exist_ok=True)
+    jobs = max(1, min(len(SOURCES), os.cpu_count() or 1, 16))
+    with ThreadPoolExecutor(jobs) as pool:
+        for f in [pool.submit(_run, compile_command(s), verbose) for s in SOURCES]:
+            f.result()
+    tmp = LIB + ".tmp"
+    _run(link_command(tmp), verbose)
     os.replace(tmp, LIB)
     return LIB
 
