@@ -391,7 +391,8 @@ def refine_summary(args) -> dict:
     same run: the network trained in-run on solver pairs, then plain projected
     L-BFGS at the full budget (its final misfit is the target) and the
     learned arm at a quarter and at half of that budget."""
-    r = run_refine(args, plain_fracs=(1.0,), learned_fracs=(0.25, 0.5), pseudo=False)
+    r = run_refine(args, plain_fracs=(1.0,), learned_fracs=(0.25, 0.5), pseudo=False,
+                   other_modes=("first", "replace"), other_fracs=(0.5, 1.0))
     v = r["value"]
     return {
         "metric": "refine solves-to-misfit (learned / plain L-BFGS)",
@@ -402,6 +403,7 @@ def refine_summary(args) -> dict:
         "reached_at": r["reached_at"],
         "median_final_misfit": r["median_final_misfit"],
         "config": r["config"],
+        "other_modes": r["other_modes"],
         "train": r["train"],
         "wall_s": r["wall_s"],
     }
@@ -438,13 +440,16 @@ def refine_lattice_field():
 
 
 def run_refine(args, plain_fracs=(0.25, 0.5, 0.75, 1.0), learned_fracs=(0.25, 0.5, 0.75, 1.0),
-               pseudo: bool = True) -> dict:
+               pseudo: bool = True, other_modes=(), other_fracs=(0.5, 1.0)) -> dict:
     """Learned-preconditioned refinement against plain projected L-BFGS:
     solves needed to reach plain L-BFGS's final data misfit.  The network is
     trained in the run on pairs from the batched solver; plain L-BFGS runs at
     `plain_fracs` of the budget (its final misfit at the full budget is the
     target), the learned arm at `learned_fracs` and, with `pseudo`, the
-    illumination-scaled arm at the plain fractions."""
+    illumination-scaled arm at the plain fractions.  Each mode of
+    `other_modes` ("first", "replace": the same network as a preconditioner
+    of the L-BFGS direction instead of the --pc-mode arm) runs at
+    `other_fracs`."""
     import torch
 
     from wgnet_amd.physics import WavePhysics
@@ -511,6 +516,19 @@ def run_refine(args, plain_fracs=(0.25, 0.5, 0.75, 1.0), learned_fracs=(0.25, 0.
         print(f"refine: learned preconditioner {int(B * f)} solves: median final misfit "
               f"{np.median(pc_final[f]):.4e}, reaches plain in {np.mean(pc_final[f] <= m_plain):.3f} of models",
               file=sys.stderr, flush=True)
+    other_final = {}
+    for mode in other_modes:
+        if mode == args.pc_mode:
+            continue
+        kw = {"update": net.update_hook(), "jumps_per_band": args.jumps} if mode == "jump" else \
+            {"precondition": net.hook(), "pc_mode": mode}
+        other_final[mode] = {}
+        for f in other_fracs:
+            res = refine_batch(start_km, gauss, obs, ph, budget_solves=int(B * f), bands_hz=bands, **kw)
+            other_final[mode][f] = np.array([r.final_loss for r in res])
+            print(f"refine: network in {mode!r} mode, {int(B * f)} solves: median final misfit "
+                  f"{np.median(other_final[mode][f]):.4e}, reaches plain in "
+                  f"{np.mean(other_final[mode][f] <= m_plain):.3f} of models", file=sys.stderr, flush=True)
     phs_final = {}
     if pseudo:
         # the physics baseline: diagonal pseudo-Hessian (illumination) scaling;
@@ -562,6 +580,14 @@ def run_refine(args, plain_fracs=(0.25, 0.5, 0.75, 1.0), learned_fracs=(0.25, 0.
         "plain_refine_s": round(t_plain, 2),
         "wall_s": round(time.perf_counter() - t_all, 1),
     }
+    out["other_modes"] = {}
+    for mode, fin in other_final.items():
+        need_m = solves_needed(fin)
+        med = float(np.median(need_m))
+        out["other_modes"][mode] = {
+            "solves_needed_median": med if np.isfinite(med) else None,
+            "reached_at": {str(f): float(np.mean(need_m <= f)) for f in fin},
+            "median_final_misfit": {str(f): float(np.median(fin[f])) for f in fin}}
     if pseudo:
         need_ph = solves_needed(phs_final)
         out["pseudo_hessian"] = {"solves_needed_median": float(np.median(need_ph)),
