@@ -136,7 +136,7 @@ struct wg_engine {
     // HBM-streamed path (grids too large for one workgroup): step state,
     // checkpoints every seg_len steps, one segment of snapshots.
     bool streamed = false;
-    DevBuf fstate, astate, ckpt;
+    DevBuf fstate, astate, ckpt, vpad;  // vpad: clamped velocity on the padded grid (streamed steps)
     hipStream_t stream = nullptr;  // host-pointer calls
     hipStream_t side = nullptr;    // second stream of the overlapped chunk pipeline
     hipEvent_t ev_pipe[2] = {nullptr, nullptr};
@@ -652,11 +652,26 @@ int ckpt_copy(wg_engine* e, int units, int slot, const FwdRot& rot, bool save, h
 // Forward of units [unit0, unit0 + units) over all nt steps from rest:
 // records traces (if non-null), stores all snapshots when P.nseg == 1 and
 // `store`, and checkpoints the state every P.seg_len steps when P.nseg > 1.
+// The clamped velocity of the models of units [unit0, unit0 + units) on the
+// padded grid (e->vpad), for the step kernels' lean interior path.
+int stream_pad_velocity(wg_engine* e, const float* vel, int units, int unit0, hipStream_t st, wg::StepParams* p) {
+    const int S = e->cfg.shots, b0 = unit0 / S, nb = (unit0 + units - 1) / S + 1 - b0;
+    const size_t cells = (size_t)e->g.nx * e->g.nz;
+    int rc;
+    if ((rc = ensure(e, e->vpad, (size_t)nb * cells * sizeof(float)))) return rc;
+    LAUNCH(wg::launch_pad_velocity(e->g, vel, b0, nb, (float*)e->vpad.p, st));
+    p->vpad = (const float*)e->vpad.p;
+    p->vpad_b0 = b0;
+    return WG_OK;
+}
+
 int stream_forward(wg_engine* e, const StreamPlan& P, int units, int unit0, const float* vel, const float* wav,
                    long long wav_stride, float* traces, bool store, hipStream_t st) {
     const size_t cells = (size_t)e->g.nx * e->g.nz;
     LAUNCH(wg::launch_zero((float*)e->fstate.p, (size_t)wg::kFwdArrays * units * cells, st));
     wg::StepParams p = step_params(e, vel, units, unit0);
+    int rc0;
+    if ((rc0 = stream_pad_velocity(e, vel, units, unit0, st, &p))) return rc0;
     p.wavelet = wav;
     p.wav_stride = wav_stride;
     p.traces = traces;
@@ -693,6 +708,10 @@ int stream_adjoint(wg_engine* e, const StreamPlan& P, int units, int unit0, cons
     pf.seg_len = P.seg_len;
     pf.state = (float*)e->fstate.p;
     wg::StepParams pa = step_params(e, vel, units, unit0);
+    int rc0;
+    if ((rc0 = stream_pad_velocity(e, vel, units, unit0, st, &pf))) return rc0;
+    pa.vpad = pf.vpad;
+    pa.vpad_b0 = pf.vpad_b0;
     pa.snaps = (float*)e->snaps.p;
     pa.seg_len = P.seg_len;
     pa.state = (float*)e->astate.p;
@@ -1115,7 +1134,7 @@ void wg_engine_destroy(wg_engine* e) {
                     (void*)e->d_wavelet})
         if (p) (void)hipFree(p);
     for (DevBuf* b : {&e->snaps, &e->traces, &e->resid, &e->gunit, &e->partials, &e->scale, &e->energy, &e->vel,
-                      &e->obs, &e->loss, &e->grad, &e->fstate, &e->astate, &e->ckpt, &e->qstate, &e->qflags,
+                      &e->obs, &e->loss, &e->grad, &e->fstate, &e->astate, &e->ckpt, &e->vpad, &e->qstate, &e->qflags,
                       &e->bxbuf[0], &e->bqflags[0], &e->bxbuf[1], &e->bqflags[1], &e->mtable, &e->mnext,
                       &e->mslots})
         if (b->p) (void)hipFree(b->p);

@@ -14,6 +14,8 @@
 // 2-cell halo.  Tiles with no C-PML cell along an axis skip that axis's strips
 // and memory variables (they are zero there).  Per-cell arithmetic is
 // stencil.h's, in the reference's order.
+#include <algorithm>
+
 #include "stencil.h"
 #include "wave_kernels.h"
 
@@ -49,6 +51,12 @@ struct TileId {
     int x, z, unit;
 };
 
+// n / d with the launcher's multiplier m = ceil(2^32 / d) (exact for the n a
+// launch divides; m = 0: a plain division).  The scalar unit has no divider:
+// a division by a value known only at run time is a long scalar sequence in
+// every wave of every tile.
+__device__ __forceinline__ unsigned udiv(unsigned n, unsigned d, unsigned m) { return m ? __umulhi(n, m) : n / d; }
+
 __device__ __forceinline__ TileId tile_id(const StepParams& P) {
     if (!P.xcd_order) return TileId{(int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z};
     const unsigned gx = gridDim.x, gy = gridDim.y;
@@ -56,7 +64,15 @@ __device__ __forceinline__ TileId tile_id(const StepParams& P) {
     const unsigned lin = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
     const unsigned k = lin % kXcds, j = lin / kXcds, q = n / kXcds, r = n % kXcds;
     const unsigned t = k * q + (k < r ? k : r) + j;  // a bijection of [0, n)
-    return TileId{(int)(t % gx), (int)((t / gx) % gy), (int)(t / (gx * gy))};
+    const unsigned u = udiv(t, gx * gy, P.m_gxy), w = t - u * gx * gy;
+    const unsigned tz = udiv(w, gx, P.m_gx);
+    return TileId{(int)(w - tz * gx), (int)tz, (int)u};
+}
+
+// Model and source slot of a unit.
+__device__ __forceinline__ void unit_model(const StepParams& P, int unit, int& b, int& s) {
+    b = (int)udiv((unsigned)unit, (unsigned)P.g.shots, P.m_shots);
+    s = src_slot(P.geo, unit, unit - b * P.g.shots);
 }
 
 #define WG_TILE_CONSTANTS                                              \
@@ -77,6 +93,179 @@ __device__ __forceinline__ float vis_raw(const float* vvis, const Grid& g, int x
 }
 
 // --------------------------------------------------------------------------
+// Lean paths of the single steps for tiles with no C-PML cell, receiver or
+// source (most of a large grid): the tile and its halo lie inside the grid,
+// so there are no bounds or C-PML tests; the clamped velocity comes from the
+// padded array P.vpad; the staging needs no division by the halo width (the
+// tile's own columns are staged by the column map of its cells, the four halo
+// columns separately).  Per-cell arithmetic is the general path's, operation
+// for operation.
+// --------------------------------------------------------------------------
+template <class TL>
+__device__ __forceinline__ void stage_interior(float* sF, const float* F, int nx, int xi0, int zi0) {
+    WG_TILE_CONSTANTS
+    const int tid = threadIdx.x, lx = tid % TX, r0 = tid / TX;
+    constexpr int NP = (HZ + RZ - 1) / RZ;  // passes over the tile's own columns
+    constexpr int NHC = 4 * HZ;             // halo-column words
+    constexpr int NH = (NHC + NT - 1) / NT;
+    float v[NP], h[NH];
+    const float* src = F + (size_t)zi0 * nx + xi0;  // halo origin (padded row zi0, column xi0)
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+        const int r = r0 + k * RZ;
+        v[k] = r < HZ ? src[r * nx + 2 + lx] : 0.0f;
+    }
+#pragma unroll
+    for (int k = 0; k < NH; ++k) {
+        const int j = tid + k * NT;
+        const int r = j >> 2, c = j & 3, hx = c < 2 ? c : TX + c;
+        h[k] = j < NHC ? src[r * nx + hx] : 0.0f;
+    }
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+        const int r = r0 + k * RZ;
+        if (r < HZ) sF[r * HX + 2 + lx] = v[k];
+    }
+#pragma unroll
+    for (int k = 0; k < NH; ++k) {
+        const int j = tid + k * NT;
+        const int r = j >> 2, c = j & 3, hx = c < 2 ? c : TX + c;
+        if (j < NHC) sF[r * HX + hx] = h[k];
+    }
+}
+
+// Does a source corner of slot s lie in the tile's cells?  The forward
+// injects only corners of nonzero weight (nonzero_only); the adjoint adds the
+// gradient term of every corner.
+__device__ __forceinline__ bool tile_has_source(const StepParams& P, int s, int xi0, int zi0, int TX, int TZ,
+                                                bool nonzero_only) {
+    bool any = false;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const int cell = P.geo.src_cell[s * 4 + c];
+        const int cz = (int)udiv((unsigned)cell, (unsigned)P.g.nx, P.m_nx), cx = cell - cz * P.g.nx;
+        any |= (!nonzero_only || P.geo.src_w[s * 4 + c] != 0.0f) && cx >= xi0 + 2 && cx < xi0 + TX + 2 &&
+               cz >= zi0 + 2 && cz < zi0 + TZ + 2;
+    }
+    return any;
+}
+
+template <class TL, bool STORE>
+__device__ __forceinline__ void fwd_interior(const StepParams& P, float* sU, int ul, int unit, int b, int xi0,
+                                             int zi0) {
+    WG_TILE_CONSTANTS
+    const Grid& g = P.g;
+    const size_t cells = (size_t)g.nx * g.nz;
+    const int nx = g.nx;
+    const int tid = threadIdx.x, lx = tid % TX, lz0 = tid / TX;
+    const float* Ucur = P.state + ((size_t)fwd_u_arr(P.u_cur) * P.units + ul) * cells;
+    float* Unxt = P.state + ((size_t)fwd_u_arr(P.u_prev) * P.units + ul) * cells;
+    const float* vp = P.vpad + (size_t)(b - P.vpad_b0) * cells;
+    const int xo = xi0 + lx + 2;
+    const int i0 = (zi0 + lz0 + 2) * nx + xo;  // the thread's first cell
+    float pu[CPT], pv[CPT];
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+        pu[k] = Unxt[i0 + k * RZ * nx];
+        pv[k] = vp[i0 + k * RZ * nx];
+    }
+    stage_interior<TL>(sU, Ucur, nx, xi0, zi0);
+    __syncthreads();
+    float* snap = STORE ? P.snaps + ((size_t)(P.t - P.snap_t0) * P.units + ul) * g.inner : nullptr;
+    const int q0 = (zi0 + lz0) * g.ni + xi0 + lx;  // snapshot index of the first cell
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+        const int lz = lz0 + RZ * k;
+        const int li = (lz + 2) * HX + (lx + 2);
+        const float u = sU[li];
+        Pml none{0, 0, 0, 0};
+        float dummy = 0.0f;
+        const float w = fwd_w<false, false>(sU, li, HX, u, g.rh, g.rh2, nullptr, none, dummy, dummy, nullptr, TX, none,
+                                            dummy, dummy);
+        const float v = pv[k];
+        if (STORE) snap[q0 + k * RZ * g.ni] = 2.0f * v * g.dt2 * w;
+        Unxt[i0 + k * RZ * nx] = v * v * g.dt2 * w + 2.0f * u - pu[k];
+    }
+}
+
+// Adjoint: V = v^2 dt^2 lambda^t staged from lambda^t and the padded
+// velocity, then the interior update and the gradient term.
+template <class TL>
+__device__ __forceinline__ void adj_interior(const StepParams& P, float* sV, int ul, int unit, int b, int xi0,
+                                             int zi0) {
+    WG_TILE_CONSTANTS
+    const Grid& g = P.g;
+    const size_t cells = (size_t)g.nx * g.nz;
+    const int nx = g.nx;
+    const int tid = threadIdx.x, lx = tid % TX, r0 = tid / TX;
+    auto arr = [&](int a) { return P.state + ((size_t)a * P.units + ul) * cells; };
+    const float* Lcur = arr(P.lam_cur < 2 ? P.lam_cur : P.lam_cur + 8);
+    float* Lnxt = arr(P.lam_prev < 2 ? P.lam_prev : P.lam_prev + 8);
+    float* grad = P.grad + (size_t)unit * cells;
+    const float* vp = P.vpad + (size_t)(b - P.vpad_b0) * cells;
+    const int xo = xi0 + lx + 2;
+    const int i0 = (zi0 + r0 + 2) * nx + xo;
+    const float* snap = P.snaps + ((size_t)(P.t - P.snap_t0) * P.units + ul) * g.inner;
+    const int q0 = (zi0 + r0) * g.ni + xi0 + lx;
+    float pl[CPT], pn[CPT], ps[CPT], pg[CPT];
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+        const int i = i0 + k * RZ * nx;
+        pl[k] = Lcur[i];
+        pn[k] = Lnxt[i];
+        ps[k] = snap[q0 + k * RZ * g.ni];
+        pg[k] = grad[i];
+    }
+    {
+        constexpr int NP = (HZ + RZ - 1) / RZ;
+        constexpr int NHC = 4 * HZ;
+        constexpr int NH = (NHC + NT - 1) / NT;
+        float tl[NP], tv[NP], hl[NH], hv[NH];
+        const size_t o = (size_t)zi0 * nx + xi0;
+#pragma unroll
+        for (int k = 0; k < NP; ++k) {
+            const int r = r0 + k * RZ;
+            tl[k] = r < HZ ? Lcur[o + r * nx + 2 + lx] : 0.0f;
+            tv[k] = r < HZ ? vp[o + r * nx + 2 + lx] : 0.0f;
+        }
+#pragma unroll
+        for (int k = 0; k < NH; ++k) {
+            const int j = tid + k * NT;
+            const int r = j >> 2, c = j & 3, hx = c < 2 ? c : TX + c;
+            hl[k] = j < NHC ? Lcur[o + r * nx + hx] : 0.0f;
+            hv[k] = j < NHC ? vp[o + r * nx + hx] : 0.0f;
+        }
+#pragma unroll
+        for (int k = 0; k < NP; ++k) {
+            const int r = r0 + k * RZ;
+            if (r < HZ) sV[r * HX + 2 + lx] = tv[k] * tv[k] * g.dt2 * tl[k];
+        }
+#pragma unroll
+        for (int k = 0; k < NH; ++k) {
+            const int j = tid + k * NT;
+            const int r = j >> 2, c = j & 3, hx = c < 2 ? c : TX + c;
+            if (j < NHC) sV[r * HX + hx] = hv[k] * hv[k] * g.dt2 * hl[k];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+        const int lz = r0 + RZ * k;
+        const int li = (lz + 2) * HX + (lx + 2);
+        const float v0 = sV[li];
+        const float rh = g.rh;
+        float w = adj_axis_int(sV, li, 1, v0, rh);
+        w += adj_axis_int(sV, li, HX, v0, rh);
+        const float lam = pl[k];
+        const float next = w + 2.0f * lam - pn[k];
+        const float gk = lam * ps[k];
+        const int i = i0 + k * RZ * nx;
+        grad[i] = pg[k] + gk;
+        Lnxt[i] = next;
+    }
+}
+
+// --------------------------------------------------------------------------
 // Forward step (lib.rs:145-220 for one t, every unit of the chunk).
 // --------------------------------------------------------------------------
 template <class TL, bool STORE, bool BORN>
@@ -89,7 +278,8 @@ __global__ __launch_bounds__(NT) void k_fwd_step(StepParams P) {
     const TileId tile = tile_id(P);
     const int ul = tile.unit;              // unit within the chunk
     const int unit = P.unit0 + ul;
-    const int b = unit / g.shots, s = src_slot(P.geo, unit, unit - (unit / g.shots) * g.shots);
+    int b, s;
+    unit_model(P, unit, b, s);
     const size_t cells = (size_t)g.nx * g.nz;
     const int t = P.t, h = P.ps_half;
     auto arr = [&](int a) { return P.state + ((size_t)a * P.units + ul) * cells; };
@@ -129,6 +319,15 @@ __global__ __launch_bounds__(NT) void k_fwd_step(StepParams P) {
     // does the tile hold C-PML cells along x / z (padded coords [xi0+2, xi0+TX+2))?
     const bool tpx = xi0 + 2 < g.region || xi0 + TX + 2 > g.nx - g.region;
     const bool tpz = zi0 + 2 < g.region || zi0 + TZ + 2 > g.nz - g.region;
+    if constexpr (!BORN) {
+        // (the C-PML region is wider than the 2-cell halo, so such a tile and
+        // its halo lie inside the grid; tile (0, 0), which records the
+        // gathers, is in the region)
+        if (P.vpad && !tpx && !tpz && !tile_has_source(P, s, xi0, zi0, TX, TZ, true)) {
+            fwd_interior<TL, STORE>(P, sU, ul, unit, b, xi0, zi0);
+            return;
+        }
+    }
     // Everything a thread reads from HBM for its own cells is loaded before
     // the staging so that its latency overlaps it: u^{t-1}, the velocity,
     // the C-PML memory variables and profile coefficients, the unit's
@@ -305,7 +504,8 @@ __global__ __launch_bounds__(NT) void k_fwd_step2(StepParams P) {
     const TileId tile = tile_id(P);
     const int ul = tile.unit;
     const int unit = P.unit0 + ul;
-    const int b = unit / g.shots, s = src_slot(P.geo, unit, unit - (unit / g.shots) * g.shots);
+    int b, s;
+    unit_model(P, unit, b, s);
     const size_t cells = (size_t)g.nx * g.nz;
     const int t = P.t, h = P.ps_half;
     auto arr = [&](int a) { return P.state + ((size_t)a * P.units + ul) * cells; };
@@ -642,7 +842,8 @@ __global__ __launch_bounds__(NT) void k_adj_step(StepParams P) {
     const TileId tile = tile_id(P);
     const int ul = tile.unit;
     const int unit = P.unit0 + ul;
-    const int b = unit / g.shots, s = src_slot(P.geo, unit, unit - (unit / g.shots) * g.shots);
+    int b, s;
+    unit_model(P, unit, b, s);
     const size_t cells = (size_t)g.nx * g.nz;
     const int t = P.t, q = P.half;
     // arrays: lam0 lam1 | Px0 Px1 | Zx0 Zx1 | Pz0 Pz1 | Zz0 Zz1 | lam2 lam3; the
@@ -684,6 +885,18 @@ __global__ __launch_bounds__(NT) void k_adj_step(StepParams P) {
             acc += (-(v * v * g.dt2) * P.geo.src_w[s * 4 + c]) * Lcur[cell];
         }
         P.dwav[(size_t)unit * g.nt + t] = acc;
+    }
+    // does the tile hold cells of the adjoint C-PML region along x / z?
+    const bool tax = xi0 + 2 < g.region_adj || xi0 + TX + 2 > g.nx - g.region_adj;
+    const bool taz = zi0 + 2 < g.region_adj || zi0 + TZ + 2 > g.nz - g.region_adj;
+    if constexpr (!DWAV) {
+        // receiver rows [band_z0, band_z0 + band_rows) and source corners stay
+        // on the general path
+        const bool rec = zi0 + 2 < P.geo.band_z0 + P.geo.band_rows && zi0 + TZ + 2 > P.geo.band_z0;
+        if (P.vpad && !tax && !taz && !rec && !tile_has_source(P, s, xi0, zi0, TX, TZ, false)) {
+            adj_interior<TL>(P, sV, ul, unit, b, xi0, zi0);
+            return;
+        }
     }
     // Everything a thread reads from HBM for its own cells is loaded before
     // the staging so that its latency overlaps it: lambda^t, lambda^{t+1},
@@ -741,9 +954,6 @@ __global__ __launch_bounds__(NT) void k_adj_step(StepParams P) {
     }
     __syncthreads();
     // T = (1+b)V + bZ, U1 = db T + bP of the x-neighbours (x halo) and z-neighbours
-    // does the tile hold cells of the adjoint C-PML region along x / z?
-    const bool tax = xi0 + 2 < g.region_adj || xi0 + TX + 2 > g.nx - g.region_adj;
-    const bool taz = zi0 + 2 < g.region_adj || zi0 + TZ + 2 > g.nz - g.region_adj;
     if (tax)
         for (int j = tid; j < TZ * HX; j += NT) {
             const int hx = j % HX, hz = j / HX;
@@ -856,7 +1066,8 @@ __global__ __launch_bounds__(NT) void k_adj_step2(StepParams P) {
     const TileId tile = tile_id(P);
     const int ul = tile.unit;
     const int unit = P.unit0 + ul;
-    const int b = unit / g.shots, s = src_slot(P.geo, unit, unit - (unit / g.shots) * g.shots);
+    int b, s;
+    unit_model(P, unit, b, s);
     const size_t cells = (size_t)g.nx * g.nz;
     const int t = P.t, h = P.half;
     auto arr = [&](int a) { return P.state + ((size_t)a * P.units + ul) * cells; };
@@ -1213,9 +1424,21 @@ __global__ __launch_bounds__(NT) void k_adj_step2(StepParams P) {
 
 namespace {
 
+// ceil(2^32 / d) when n / d = mulhi(n, m) is exact for every n < n_end
+// (n_end <= 2^32 / d), else 0.
+unsigned div_magic(unsigned long long d, unsigned long long n_end) {
+    if (d < 2 || n_end > (1ull << 32) / d) return 0;
+    return (unsigned)(((1ull << 32) + d - 1) / d);
+}
+
 StepParams ordered(const StepParams& p, const dim3& grid) {
     StepParams q = p;
     q.xcd_order = grid.x * grid.y >= st::kXcdMinTiles ? 1 : 0;
+    const unsigned long long tiles = (unsigned long long)grid.x * grid.y * grid.z;
+    q.m_gx = div_magic(grid.x, tiles);
+    q.m_gxy = div_magic((unsigned long long)grid.x * grid.y, tiles);
+    q.m_shots = div_magic(p.g.shots, (unsigned long long)p.unit0 + grid.z);
+    q.m_nx = div_magic(p.g.nx, (unsigned long long)p.g.nx * p.g.nz);
     return q;
 }
 
@@ -1293,6 +1516,26 @@ hipError_t launch_adj_step2(const StepParams& p, int units, hipStream_t st) {
         case 3: return adj_step2_t<T3>(p, units, st);
         default: return adj_step2_t<T0>(p, units, st);
     }
+}
+
+namespace st {
+__global__ void k_pad_velocity(Grid g, const float* vel, int b0, int nb, float* vpad) {
+    const size_t cells = (size_t)g.nx * g.nz, n = cells * nb;
+    for (size_t j = blockIdx.x * (size_t)blockDim.x + threadIdx.x; j < n; j += (size_t)gridDim.x * blockDim.x) {
+        const int m = (int)(j / cells);
+        const int i = (int)(j - (size_t)m * cells);
+        const int z = i / g.nx, x = i - z * g.nx;
+        vpad[j] = vel_at(vel + (size_t)(b0 + m) * g.nvx * g.nvz, g, x, z);
+    }
+}
+}  // namespace st
+
+hipError_t launch_pad_velocity(const Grid& g, const float* vel, int b0, int nb, float* vpad, hipStream_t st) {
+    if (nb < 1) return hipErrorInvalidValue;
+    const size_t n = (size_t)g.nx * g.nz * nb;
+    const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(st::k_pad_velocity, dim3(blocks), dim3(256), 0, st, g, vel, b0, nb, vpad);
+    return hipGetLastError();
 }
 
 hipError_t launch_adj_step(const StepParams& p, int units, hipStream_t st) {

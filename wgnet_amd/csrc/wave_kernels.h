@@ -116,7 +116,19 @@ struct StepParams {
     // t - 1 (u^{t-1}, which the launch before it computed but could not record)
     int u_cur, u_prev, u_out0, u_out1, ps_half, ze_half, trace_prev;
     int xcd_order;           // set by the launchers: tiles renumbered per XCD (large grids)
+    // The clamped velocity on the padded grid, [models, nz, nx], entry 0 for
+    // model vpad_b0 (launch_pad_velocity), or null.  With it, tiles with no
+    // C-PML cell, receiver or source take a lean path with no per-cell
+    // clamping, bounds or C-PML tests.
+    const float* vpad;
+    int vpad_b0;
+    // Set by the launchers: ceil(2^32 / d) for d = the tile-grid width, the
+    // tiles of a unit, the shots and nx (0: divide); n / d = mulhi(n, m)
+    // exactly for n < 2^32 / d, which the launchers check.
+    unsigned m_gx, m_gxy, m_shots, m_nx;
 };
+// vpad[m, z, x] = clamp(vel[b0 + m] at the clamped model cell of padded (x, z))
+hipError_t launch_pad_velocity(const Grid& g, const float* vel, int b0, int nb, float* vpad, hipStream_t st);
 constexpr int kStreamTiles = 4;
 
 // Arrays of the streamed forward state, each [units, nx * nz]: wavefield
