@@ -287,7 +287,7 @@ int32_t wg_stream_tiles(void);
 int32_t wg_set_overlap(wg_engine *e, int32_t on);
 int32_t wg_set_stream_tile(wg_engine *e, int32_t tile);
 /* Temporal blocking of the HBM-streamed adjoint (no reference counterpart):
- * time steps per launch, 1 (default) or 2.  A two-step launch recomputes a
+ * time steps per launch, 1 or 2 (default).  A two-step launch recomputes a
  * 2-cell ring around each tile and reads and writes the wavefield, memory
  * variables and gradient once per two steps.  Results are identical either
  * way; which is faster depends on the grid (an autotuner times both); the

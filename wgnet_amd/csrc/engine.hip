@@ -142,7 +142,7 @@ struct wg_engine {
     hipEvent_t ev_pipe[2] = {nullptr, nullptr};
     int cus = 256;                 // compute units of the device
     bool overlap = true;           // overlap chunk forwards with the previous chunk's adjoint
-    int time_block = 1;            // adjoint steps per streamed launch (1 or 2)
+    int time_block = 2;            // adjoint steps per streamed launch (1 or 2)
     int fwd_block = 1;             // forward steps per streamed launch (1 or 2)
     int qparts = 4;                // time parts per unit of the resident work queues (0: chunk pipeline)
     int qspin = 1 << 24;           // polls before a work-queue wait gives up

@@ -137,17 +137,22 @@ __device__ __forceinline__ void stage_interior(float* sF, const float* F, int nx
 // Does a source corner of slot s lie in the tile's cells?  The forward
 // injects only corners of nonzero weight (nonzero_only); the adjoint adds the
 // gradient term of every corner.
-__device__ __forceinline__ bool tile_has_source(const StepParams& P, int s, int xi0, int zi0, int TX, int TZ,
-                                                bool nonzero_only) {
+// Does a source corner of slot s lie in padded columns [x0, x1) and rows
+// [z0, z1)?
+__device__ __forceinline__ bool box_has_source(const StepParams& P, int s, int x0, int x1, int z0, int z1,
+                                               bool nonzero_only) {
     bool any = false;
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
         const int cell = P.geo.src_cell[s * 4 + c];
         const int cz = (int)udiv((unsigned)cell, (unsigned)P.g.nx, P.m_nx), cx = cell - cz * P.g.nx;
-        any |= (!nonzero_only || P.geo.src_w[s * 4 + c] != 0.0f) && cx >= xi0 + 2 && cx < xi0 + TX + 2 &&
-               cz >= zi0 + 2 && cz < zi0 + TZ + 2;
+        any |= (!nonzero_only || P.geo.src_w[s * 4 + c] != 0.0f) && cx >= x0 && cx < x1 && cz >= z0 && cz < z1;
     }
     return any;
+}
+__device__ __forceinline__ bool tile_has_source(const StepParams& P, int s, int xi0, int zi0, int TX, int TZ,
+                                                bool nonzero_only) {
+    return box_has_source(P, s, xi0 + 2, xi0 + TX + 2, zi0 + 2, zi0 + TZ + 2, nonzero_only);
 }
 
 template <class TL, bool STORE>
@@ -539,6 +544,9 @@ __global__ __launch_bounds__(NT) void k_fwd_step2(StepParams P) {
             }
         return;
     }
+    if (tile.x >= P.in_x0 && tile.x < P.in_x1 && tile.z >= P.in_z0 && tile.z < P.in_z1 &&
+        !box_has_source(P, s, xi0, xi0 + TX + 4, zi0, zi0 + TZ + 4, true))
+        return;  // k_fwd_step2_in runs this tile
     // Receivers record u^t (and u^{t-1}) before the update (lib.rs:146-149).
     if (P.traces && tile.x == 0 && tile.z == 0) {
         for (int r = tid; r < g.nrec; r += NT) {
@@ -1033,6 +1041,295 @@ __global__ __launch_bounds__(NT) void k_adj_step(StepParams P) {
 
 
 // --------------------------------------------------------------------------
+// Two forward steps on the tiles whose cells and 2-cell ring lie outside the
+// C-PML region and hold no source corner (the gathers are recorded by tile
+// (0, 0), which lies in the region): k_fwd_step2's scheme with no memory
+// variables, strips, sources or bounds tests, and the clamped velocity from
+// P.vpad; launched beside k_fwd_step2, which skips these tiles.
+// --------------------------------------------------------------------------
+template <class TL, bool STORE>
+__global__ __launch_bounds__(NT) void k_fwd_step2_in(StepParams P) {
+    WG_TILE_CONSTANTS
+    constexpr int AX = TX + 8, AZ = TZ + 8;  // tile + 4-cell halo
+    constexpr int BX = TX + 4;               // tile + 2-cell halo
+    constexpr int RING = 4 * TX + 4 * TZ;    // 2-cell ring without its corners
+    constexpr int RPT = (RING + NT - 1) / NT;
+    __shared__ float sU[AZ * AX];  // u^t on tile + 4 (pitch AX), then u^{t+1} on tile + 2 (pitch BX)
+    const Grid& g = P.g;
+    const TileId tl = tile_id(P);
+    const int ul = tl.unit;
+    const int unit = P.unit0 + ul;
+    int b, s;
+    unit_model(P, unit, b, s);
+    if (!unit_active(P.geo, unit)) return;
+    const int xi0 = (P.in_x0 + tl.x) * TX, zi0 = (P.in_z0 + tl.z) * TZ;
+    if (box_has_source(P, s, xi0, xi0 + TX + 4, zi0, zi0 + TZ + 4, true)) return;  // k_fwd_step2 runs it
+    const size_t cells = (size_t)g.nx * g.nz;
+    const int nx = g.nx, t = P.t;
+    auto arr = [&](int a) { return P.state + ((size_t)a * P.units + ul) * cells; };
+    const float* Ucur = arr(fwd_u_arr(P.u_cur));    // u^t
+    const float* Uprev = arr(fwd_u_arr(P.u_prev));  // u^{t-1}
+    float* O0 = arr(fwd_u_arr(P.u_out0));           // u^{t+1}
+    float* O1 = arr(fwd_u_arr(P.u_out1));           // u^{t+2}
+    const float* vp = P.vpad + (size_t)(b - P.vpad_b0) * cells;
+    const int tid = threadIdx.x, lx = tid % TX, r0 = tid / TX;
+
+    const int i0 = (zi0 + 2 + r0) * nx + xi0 + 2 + lx;
+    float pu[CPT], pv[CPT];
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+        pu[k] = Uprev[i0 + k * RZ * nx];
+        pv[k] = vp[i0 + k * RZ * nx];
+    }
+    int rrx[RPT], rrz[RPT];
+    float ru[RPT], rv[RPT];
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+        const int j = tid + NT * r;
+        int rx = 0, rz = -100;  // rz = -100: no ring cell for this thread
+        if (j < 2 * TX) {
+            rz = -2 + j / TX;
+            rx = j % TX;
+        } else if (j < 4 * TX) {
+            rz = TZ + (j - 2 * TX) / TX;
+            rx = (j - 2 * TX) % TX;
+        } else if (j < 4 * TX + 2 * TZ) {
+            rz = (j - 4 * TX) / 2;
+            rx = -2 + ((j - 4 * TX) & 1);
+        } else if (j < RING) {
+            rz = (j - 4 * TX - 2 * TZ) / 2;
+            rx = TX + ((j - 4 * TX - 2 * TZ) & 1);
+        }
+        rrx[r] = rx;
+        rrz[r] = rz;
+        ru[r] = rv[r] = 0.0f;
+        if (rz != -100) {
+            const int i = (zi0 + 2 + rz) * nx + (xi0 + 2 + rx);
+            ru[r] = Uprev[i];
+            rv[r] = vp[i];
+        }
+    }
+    {
+        constexpr int NS = (AZ * AX + NT - 1) / NT;
+        float tu[NS];
+        const size_t o = (size_t)(zi0 - 2) * nx + (xi0 - 2);
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            const int j = tid + k * NT;
+            tu[k] = j < AZ * AX ? Ucur[o + (size_t)(j / AX) * nx + (j % AX)] : 0.0f;
+        }
+#pragma unroll
+        for (int k = 0; k < NS; ++k)
+            if (tid + k * NT < AZ * AX) sU[tid + k * NT] = tu[k];
+    }
+    __syncthreads();
+    Pml none{0, 0, 0, 0};
+    float dummy = 0.0f;
+    float* snap0 = STORE ? P.snaps + ((size_t)(t - P.snap_t0) * P.units + ul) * g.inner : nullptr;
+    float* snap1 = STORE ? P.snaps + ((size_t)(t + 1 - P.snap_t0) * P.units + ul) * g.inner : nullptr;
+    const int q0 = (zi0 + r0) * g.ni + xi0 + lx;
+    // Step A (time t) on the tile and its ring: u^{t+1}
+    float ut[CPT], un[CPT];
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+        const int li = (r0 + RZ * k + 4) * AX + (lx + 4);
+        const float u = sU[li];
+        ut[k] = u;
+        const float w = fwd_w<false, false>(sU, li, AX, u, g.rh, g.rh2, nullptr, none, dummy, dummy, nullptr, BX,
+                                            none, dummy, dummy);
+        const float v = pv[k];
+        if (STORE) snap0[q0 + k * RZ * g.ni] = 2.0f * v * g.dt2 * w;
+        un[k] = v * v * g.dt2 * w + 2.0f * u - pu[k];
+    }
+    float rn[RPT];
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+        rn[r] = 0.0f;
+        if (rrz[r] != -100) {
+            const int li = (rrz[r] + 4) * AX + (rrx[r] + 4);
+            const float u = sU[li];
+            const float w = fwd_w<false, false>(sU, li, AX, u, g.rh, g.rh2, nullptr, none, dummy, dummy, nullptr, BX,
+                                                none, dummy, dummy);
+            const float v = rv[r];
+            rn[r] = v * v * g.dt2 * w + 2.0f * u - ru[r];
+        }
+    }
+    __syncthreads();  // step A's reads of sU are done
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) sU[(r0 + RZ * k + 2) * BX + (lx + 2)] = un[k];
+#pragma unroll
+    for (int r = 0; r < RPT; ++r)
+        if (rrz[r] != -100) sU[(rrz[r] + 2) * BX + (rrx[r] + 2)] = rn[r];
+    __syncthreads();
+    // Step B (time t + 1) on the tile
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+        const int li = (r0 + RZ * k + 2) * BX + (lx + 2);
+        const float u = sU[li];
+        const float w = fwd_w<false, false>(sU, li, BX, u, g.rh, g.rh2, nullptr, none, dummy, dummy, nullptr, TX,
+                                            none, dummy, dummy);
+        const float v = pv[k];
+        if (STORE) snap1[q0 + k * RZ * g.ni] = 2.0f * v * g.dt2 * w;
+        const int i = i0 + k * RZ * nx;
+        O0[i] = u;
+        O1[i] = v * v * g.dt2 * w + 2.0f * u - ut[k];
+    }
+}
+
+// --------------------------------------------------------------------------
+// Two adjoint steps on the tiles whose cells and 2-cell ring lie outside the
+// C-PML region and the receiver rows and hold no source corner (the bulk of
+// a large grid; P.in_x0 .. P.in_z1, launched beside k_adj_step2, which skips
+// them): k_adj_step2's scheme with no memory variables, strips, receivers or
+// bounds tests, and the clamped velocity from P.vpad.  Same arithmetic in the
+// same order, so the two kernels together equal two single steps bit for bit.
+// --------------------------------------------------------------------------
+template <class TL>
+__global__ __launch_bounds__(NT) void k_adj_step2_in(StepParams P) {
+    WG_TILE_CONSTANTS
+    constexpr int AX = TX + 8, AZ = TZ + 8;  // tile + 4-cell halo
+    constexpr int BX = TX + 4;               // tile + 2-cell halo
+    constexpr int RING = 4 * TX + 4 * TZ;    // 2-cell ring without its corners
+    constexpr int RPT = (RING + NT - 1) / NT;
+    __shared__ float sV[AZ * AX];  // V^t on tile + 4 (pitch AX), then V^{t-1} on tile + 2 (pitch BX)
+    const Grid& g = P.g;
+    const TileId tl = tile_id(P);
+    const int ul = tl.unit;
+    const int unit = P.unit0 + ul;
+    int b, s;
+    unit_model(P, unit, b, s);
+    if (!unit_active(P.geo, unit)) return;
+    const int xi0 = (P.in_x0 + tl.x) * TX, zi0 = (P.in_z0 + tl.z) * TZ;
+    if (tile_has_source(P, s, xi0, zi0, TX, TZ, false)) return;  // k_adj_step2 runs this tile
+    const size_t cells = (size_t)g.nx * g.nz;
+    const int nx = g.nx, t = P.t;
+    auto lam = [&](int k) { return P.state + ((size_t)(k < 2 ? k : k + 8) * P.units + ul) * cells; };
+    const float* Lcur = lam(P.lam_cur);    // lambda^t
+    const float* Lprev = lam(P.lam_prev);  // lambda^{t+1}
+    float* O0 = lam(P.lam_out0);           // lambda^{t-1}
+    float* O1 = lam(P.lam_out1);           // lambda^{t-2}
+    float* grad = P.grad + (size_t)unit * cells;
+    const float* vp = P.vpad + (size_t)(b - P.vpad_b0) * cells;
+    const int tid = threadIdx.x, lx = tid % TX, r0 = tid / TX;
+    const float rh = g.rh;
+
+    // the thread's tile cells and ring cells, loaded before the staging
+    const int i0 = (zi0 + 2 + r0) * nx + xi0 + 2 + lx;
+    const float* sn0 = P.snaps + ((size_t)(t - P.snap_t0) * P.units + ul) * g.inner;
+    const float* sn1 = P.snaps + ((size_t)(t - 1 - P.snap_t0) * P.units + ul) * g.inner;
+    const int q0 = (zi0 + r0) * g.ni + xi0 + lx;
+    float pl[CPT], pp[CPT], ps0[CPT], ps1[CPT], pg[CPT], pv[CPT];
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+        const int i = i0 + k * RZ * nx, q = q0 + k * RZ * g.ni;
+        pl[k] = Lcur[i];
+        pp[k] = Lprev[i];
+        ps0[k] = sn0[q];
+        ps1[k] = sn1[q];
+        pg[k] = grad[i];
+        pv[k] = vp[i];
+    }
+    int rrx[RPT], rrz[RPT];
+    float rl[RPT], rp[RPT], rv[RPT];
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+        const int j = tid + NT * r;
+        int rx = 0, rz = -100;  // rz = -100: no ring cell for this thread
+        if (j < 2 * TX) {
+            rz = -2 + j / TX;
+            rx = j % TX;
+        } else if (j < 4 * TX) {
+            rz = TZ + (j - 2 * TX) / TX;
+            rx = (j - 2 * TX) % TX;
+        } else if (j < 4 * TX + 2 * TZ) {
+            rz = (j - 4 * TX) / 2;
+            rx = -2 + ((j - 4 * TX) & 1);
+        } else if (j < RING) {
+            rz = (j - 4 * TX - 2 * TZ) / 2;
+            rx = TX + ((j - 4 * TX - 2 * TZ) & 1);
+        }
+        rrx[r] = rx;
+        rrz[r] = rz;
+        rl[r] = rp[r] = rv[r] = 0.0f;
+        if (rz != -100) {
+            const int i = (zi0 + 2 + rz) * nx + (xi0 + 2 + rx);
+            rl[r] = Lcur[i];
+            rp[r] = Lprev[i];
+            rv[r] = vp[i];
+        }
+    }
+    // V^t on tile + 4 (vw(), wgsl:53-56)
+    {
+        constexpr int NS = (AZ * AX + NT - 1) / NT;
+        float tl_[NS], tv[NS];
+        const size_t o = (size_t)(zi0 - 2) * nx + (xi0 - 2);
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            const int j = tid + k * NT;
+            const int hx = j % AX, hz = j / AX;
+            tl_[k] = tv[k] = 0.0f;
+            if (j < AZ * AX) {
+                tl_[k] = Lcur[o + (size_t)hz * nx + hx];
+                tv[k] = vp[o + (size_t)hz * nx + hx];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < NS; ++k)
+            if (tid + k * NT < AZ * AX) sV[tid + k * NT] = tv[k] * tv[k] * g.dt2 * tl_[k];
+    }
+    __syncthreads();
+    // Step A (time t) on the tile and its ring: lambda^{t-1} and V^{t-1}
+    auto step_a = [&](int rx, int rz, float l0, float lp, float v, float& l1, float& V1) {
+        const int li = (rz + 4) * AX + (rx + 4);
+        const float v0 = sV[li];
+        float w = adj_axis_int(sV, li, 1, v0, rh);
+        w += adj_axis_int(sV, li, AX, v0, rh);
+        const float next = w + 2.0f * l0 - lp;
+        l1 = next;
+        V1 = v * v * g.dt2 * next;
+    };
+    float lt1[CPT], Vo[CPT], gk[CPT];
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+        step_a(lx, r0 + RZ * k, pl[k], pp[k], pv[k], lt1[k], Vo[k]);
+        gk[k] = pl[k] * ps0[k];
+    }
+    float rV[RPT];
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+        float l1;
+        rV[r] = 0.0f;
+        if (rrz[r] != -100) step_a(rrx[r], rrz[r], rl[r], rp[r], rv[r], l1, rV[r]);
+    }
+    __syncthreads();  // step A's reads of sV are done
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) sV[(r0 + RZ * k + 2) * BX + (lx + 2)] = Vo[k];
+#pragma unroll
+    for (int r = 0; r < RPT; ++r)
+        if (rrz[r] != -100) sV[(rrz[r] + 2) * BX + (rrx[r] + 2)] = rV[r];
+    __syncthreads();
+    // Step B (time t - 1) on the tile
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+        const int lz = r0 + RZ * k;
+        const int li = (lz + 2) * BX + (lx + 2);
+        const float v0 = sV[li];
+        float w = adj_axis_int(sV, li, 1, v0, rh);
+        w += adj_axis_int(sV, li, BX, v0, rh);
+        const float l1 = lt1[k];
+        const float next = w + 2.0f * l1 - pl[k];
+        const float gb = l1 * ps1[k];
+        float gacc = pg[k];
+        gacc += gk[k];
+        gacc += gb;
+        const int i = i0 + k * RZ * nx;
+        grad[i] = gacc;
+        O0[i] = l1;
+        O1[i] = next;
+    }
+}
+
+// --------------------------------------------------------------------------
 // Two adjoint steps, t and t - 1, per launch (temporal blocking).  Step A
 // advances the tile and the 2-cell ring around it (the cells step B's
 // stencils read; the ring's corners are not needed) from lambda^t,
@@ -1098,6 +1395,9 @@ __global__ __launch_bounds__(NT) void k_adj_step2(StepParams P) {
     const float rh = g.rh;
 
     if (!unit_active(P.geo, unit)) return;  // padding unit; its gradient rows are zeroed by the host
+    if (tile.x >= P.in_x0 && tile.x < P.in_x1 && tile.z >= P.in_z0 && tile.z < P.in_z1 &&
+        !tile_has_source(P, s, xi0, zi0, TX, TZ, false))
+        return;  // k_adj_step2_in runs this tile
 
     // cells of this thread: CPT of the tile (rows tid / TX + k RZ of column
     // tid % TX) and up to RPT of the ring (top and bottom rows, then the left
@@ -1442,6 +1742,29 @@ StepParams ordered(const StepParams& p, const dim3& grid) {
     return q;
 }
 
+// Tile ranges [x0, x1) x [z0, z1) whose cells and 2-cell ring (padded
+// columns [i TX, i TX + TX + 4), rows likewise) lie outside the adjoint C-PML
+// region, when the receiver rows stay outside the region's inner rows; empty
+// without the padded velocity.
+void interior_tiles(const StepParams& p, int TX, int TZ, bool adjoint, int* x0, int* x1, int* z0, int* z1) {
+    *x0 = *x1 = *z0 = *z1 = 0;
+    const Grid& g = p.g;
+    const int ra = adjoint ? g.region_adj : g.region;
+    if (!p.vpad) return;
+    // (the adjoint injects the residual at the receiver rows; the forward
+    // records the gathers in tile (0, 0), which lies in the region)
+    if (adjoint && p.geo.band_rows > 0 && p.geo.band_z0 < g.nz - ra && p.geo.band_z0 + p.geo.band_rows > ra) return;
+    auto range = [&](int n, int T, int* lo, int* hi) {
+        *lo = (ra + T - 1) / T;                        // i T >= ra
+        const int top = n - ra - T - 4;                // i T <= top
+        *hi = top < 0 ? 0 : top / T + 1;
+        if (*hi < *lo) *hi = *lo;
+    };
+    range(g.nx, TX, x0, x1);
+    range(g.nz, TZ, z0, z1);
+    if (*x1 <= *x0 || *z1 <= *z0) *x0 = *x1 = *z0 = *z1 = 0;
+}
+
 template <class T>
 hipError_t fwd_step_t(const StepParams& p, int units, bool store, bool born, hipStream_t st) {
     const dim3 grid((p.g.ni + T::TX - 1) / T::TX, (p.g.nzi + T::TZ - 1) / T::TZ, units);
@@ -1453,9 +1776,18 @@ hipError_t fwd_step_t(const StepParams& p, int units, bool store, bool born, hip
 
 template <class T>
 hipError_t fwd_step2_t(const StepParams& p, int units, bool store, hipStream_t st) {
+    StepParams q = p;
+    interior_tiles(p, T::TX, T::TZ, false, &q.in_x0, &q.in_x1, &q.in_z0, &q.in_z1);
+    if (q.in_x1 > q.in_x0) {
+        const dim3 gin(q.in_x1 - q.in_x0, q.in_z1 - q.in_z0, units);
+        void (*fin)(StepParams) = store ? st::k_fwd_step2_in<T, true> : st::k_fwd_step2_in<T, false>;
+        hipLaunchKernelGGL(fin, gin, dim3(st::NT), 0, st, ordered(q, gin));
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
     const dim3 grid((p.g.ni + T::TX - 1) / T::TX, (p.g.nzi + T::TZ - 1) / T::TZ, units);
     void (*fn)(StepParams) = store ? st::k_fwd_step2<T, true> : st::k_fwd_step2<T, false>;
-    hipLaunchKernelGGL(fn, grid, dim3(st::NT), 0, st, ordered(p, grid));
+    hipLaunchKernelGGL(fn, grid, dim3(st::NT), 0, st, ordered(q, grid));
     return hipGetLastError();
 }
 
@@ -1469,8 +1801,16 @@ hipError_t adj_step_t(const StepParams& p, int units, hipStream_t st) {
 
 template <class T>
 hipError_t adj_step2_t(const StepParams& p, int units, hipStream_t st) {
+    StepParams q = p;
+    interior_tiles(p, T::TX, T::TZ, true, &q.in_x0, &q.in_x1, &q.in_z0, &q.in_z1);
+    if (q.in_x1 > q.in_x0) {
+        const dim3 gin(q.in_x1 - q.in_x0, q.in_z1 - q.in_z0, units);
+        hipLaunchKernelGGL(st::k_adj_step2_in<T>, gin, dim3(st::NT), 0, st, ordered(q, gin));
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
     const dim3 grid((p.g.ni + T::TX - 1) / T::TX, (p.g.nzi + T::TZ - 1) / T::TZ, units);
-    hipLaunchKernelGGL(st::k_adj_step2<T>, grid, dim3(st::NT), 0, st, ordered(p, grid));
+    hipLaunchKernelGGL(st::k_adj_step2<T>, grid, dim3(st::NT), 0, st, ordered(q, grid));
     return hipGetLastError();
 }
 

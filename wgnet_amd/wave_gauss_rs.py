@@ -129,7 +129,7 @@ class PyWaveEngine:
         _lib.check(self._lib.wg_set_snapshot_budget(self._h, int(nbytes)))
 
     def set_time_block(self, steps: int) -> None:
-        """Adjoint time steps per HBM-streamed launch (1, the default, or 2);
+        """Adjoint time steps per HBM-streamed launch (1, or 2, the default);
         results are identical either way."""
         _lib.check(self._lib.wg_set_time_block(self._h, int(steps)))
 
