@@ -75,6 +75,27 @@ __device__ __forceinline__ void unit_model(const StepParams& P, int unit, int& b
     s = src_slot(P.geo, unit, unit - b * P.g.shots);
 }
 
+// Tile of a general two-step launch over the tiles outside the lean kernels'
+// rectangle: boundary tile i of a unit is, in order, a tile of the rows above
+// the rectangle, of the rows below it, or of the side columns of its rows.
+__device__ __forceinline__ TileId tile_2step(const StepParams& P) {
+    TileId t = tile_id(P);
+    if (P.bnd_n <= 0) return t;
+    const int gx = P.bgx, i = t.x;
+    const int top = P.in_z0 * gx, bot = (P.bgz - P.in_z1) * gx, iw = P.in_x1 - P.in_x0;
+    if (i < top) {
+        const int z = (int)udiv((unsigned)i, (unsigned)gx, P.m_bgx);
+        return TileId{i - z * gx, z, t.unit};
+    }
+    if (i < top + bot) {
+        const int j = i - top, z = (int)udiv((unsigned)j, (unsigned)gx, P.m_bgx);
+        return TileId{j - z * gx, P.in_z1 + z, t.unit};
+    }
+    const int w = gx - iw, j = i - top - bot, z = (int)udiv((unsigned)j, (unsigned)w, P.m_bw);
+    const int c = j - z * w;
+    return TileId{c < P.in_x0 ? c : c + iw, P.in_z0 + z, t.unit};
+}
+
 #define WG_TILE_CONSTANTS                                              \
     constexpr int TX = TL::TX, TZ = TL::TZ, RZ = TL::RZ, CPT = TL::CPT; \
     constexpr int HX = TL::HX, HZ = TL::HZ;
@@ -506,7 +527,7 @@ __global__ __launch_bounds__(NT) void k_fwd_step2(StepParams P) {
     __shared__ float sAX[BZ * AX];
     __shared__ float sAZ[AZ * BX];
     const Grid& g = P.g;
-    const TileId tile = tile_id(P);
+    const TileId tile = tile_2step(P);
     const int ul = tile.unit;
     const int unit = P.unit0 + ul;
     int b, s;
@@ -544,9 +565,6 @@ __global__ __launch_bounds__(NT) void k_fwd_step2(StepParams P) {
             }
         return;
     }
-    if (tile.x >= P.in_x0 && tile.x < P.in_x1 && tile.z >= P.in_z0 && tile.z < P.in_z1 &&
-        !box_has_source(P, s, xi0, xi0 + TX + 4, zi0, zi0 + TZ + 4, true))
-        return;  // k_fwd_step2_in runs this tile
     // Receivers record u^t (and u^{t-1}) before the update (lib.rs:146-149).
     if (P.traces && tile.x == 0 && tile.z == 0) {
         for (int r = tid; r < g.nrec; r += NT) {
@@ -1063,7 +1081,18 @@ __global__ __launch_bounds__(NT) void k_fwd_step2_in(StepParams P) {
     unit_model(P, unit, b, s);
     if (!unit_active(P.geo, unit)) return;
     const int xi0 = (P.in_x0 + tl.x) * TX, zi0 = (P.in_z0 + tl.z) * TZ;
-    if (box_has_source(P, s, xi0, xi0 + TX + 4, zi0, zi0 + TZ + 4, true)) return;  // k_fwd_step2 runs it
+    // a source corner in the tile or its ring: the injection after each
+    // update (lib.rs:209-215), tested per cell only in such a tile
+    const bool src = box_has_source(P, s, xi0, xi0 + TX + 4, zi0, zi0 + TZ + 4, true);
+    const float f_a = P.wavelet[(size_t)unit * P.wav_stride + P.t];
+    const float f_b = P.wavelet[(size_t)unit * P.wav_stride + P.t + 1];
+    auto inject = [&](int i, float v, float f, float& next) {  // corners read only in such a tile
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const float wc = P.geo.src_w[s * 4 + c];
+            if (P.geo.src_cell[s * 4 + c] == i && wc != 0.0f) next += -(v * v * g.dt2) * wc * f;
+        }
+    };
     const size_t cells = (size_t)g.nx * g.nz;
     const int nx = g.nx, t = P.t;
     auto arr = [&](int a) { return P.state + ((size_t)a * P.units + ul) * cells; };
@@ -1139,7 +1168,9 @@ __global__ __launch_bounds__(NT) void k_fwd_step2_in(StepParams P) {
                                             none, dummy, dummy);
         const float v = pv[k];
         if (STORE) snap0[q0 + k * RZ * g.ni] = 2.0f * v * g.dt2 * w;
-        un[k] = v * v * g.dt2 * w + 2.0f * u - pu[k];
+        float next = v * v * g.dt2 * w + 2.0f * u - pu[k];
+        if (src) inject(i0 + k * RZ * nx, v, f_a, next);
+        un[k] = next;
     }
     float rn[RPT];
 #pragma unroll
@@ -1151,7 +1182,9 @@ __global__ __launch_bounds__(NT) void k_fwd_step2_in(StepParams P) {
             const float w = fwd_w<false, false>(sU, li, AX, u, g.rh, g.rh2, nullptr, none, dummy, dummy, nullptr, BX,
                                                 none, dummy, dummy);
             const float v = rv[r];
-            rn[r] = v * v * g.dt2 * w + 2.0f * u - ru[r];
+            float next = v * v * g.dt2 * w + 2.0f * u - ru[r];
+            if (src) inject((zi0 + 2 + rrz[r]) * nx + (xi0 + 2 + rrx[r]), v, f_a, next);
+            rn[r] = next;
         }
     }
     __syncthreads();  // step A's reads of sU are done
@@ -1171,8 +1204,10 @@ __global__ __launch_bounds__(NT) void k_fwd_step2_in(StepParams P) {
         const float v = pv[k];
         if (STORE) snap1[q0 + k * RZ * g.ni] = 2.0f * v * g.dt2 * w;
         const int i = i0 + k * RZ * nx;
+        float next = v * v * g.dt2 * w + 2.0f * u - ut[k];
+        if (src) inject(i, v, f_b, next);
         O0[i] = u;
-        O1[i] = v * v * g.dt2 * w + 2.0f * u - ut[k];
+        O1[i] = next;
     }
 }
 
@@ -1200,7 +1235,15 @@ __global__ __launch_bounds__(NT) void k_adj_step2_in(StepParams P) {
     unit_model(P, unit, b, s);
     if (!unit_active(P.geo, unit)) return;
     const int xi0 = (P.in_x0 + tl.x) * TX, zi0 = (P.in_z0 + tl.z) * TZ;
-    if (tile_has_source(P, s, xi0, zi0, TX, TZ, false)) return;  // k_adj_step2 runs this tile
+    // a source corner in the tile: its gradient term (wgsl:245-249), tested
+    // per cell only in such a tile
+    const bool src = tile_has_source(P, s, xi0, zi0, TX, TZ, false);
+    auto src_grad = [&](int i, float v, float l, int tt, float& gg) {  // corners read only in such a tile
+        const float f = P.wavelet[(size_t)unit * P.wav_stride + tt];
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+            if (P.geo.src_cell[s * 4 + c] == i) gg += l * (-2.0f * v * g.dt2 * P.geo.src_w[s * 4 + c] * f);
+    };
     const size_t cells = (size_t)g.nx * g.nz;
     const int nx = g.nx, t = P.t;
     auto lam = [&](int k) { return P.state + ((size_t)(k < 2 ? k : k + 8) * P.units + ul) * cells; };
@@ -1292,7 +1335,9 @@ __global__ __launch_bounds__(NT) void k_adj_step2_in(StepParams P) {
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
         step_a(lx, r0 + RZ * k, pl[k], pp[k], pv[k], lt1[k], Vo[k]);
-        gk[k] = pl[k] * ps0[k];
+        float gg = pl[k] * ps0[k];
+        if (src) src_grad(i0 + k * RZ * nx, pv[k], pl[k], t, gg);
+        gk[k] = gg;
     }
     float rV[RPT];
 #pragma unroll
@@ -1318,11 +1363,12 @@ __global__ __launch_bounds__(NT) void k_adj_step2_in(StepParams P) {
         w += adj_axis_int(sV, li, BX, v0, rh);
         const float l1 = lt1[k];
         const float next = w + 2.0f * l1 - pl[k];
-        const float gb = l1 * ps1[k];
+        const int i = i0 + k * RZ * nx;
+        float gb = l1 * ps1[k];
+        if (src) src_grad(i, pv[k], l1, t - 1, gb);
         float gacc = pg[k];
         gacc += gk[k];
         gacc += gb;
-        const int i = i0 + k * RZ * nx;
         grad[i] = gacc;
         O0[i] = l1;
         O1[i] = next;
@@ -1360,7 +1406,7 @@ __global__ __launch_bounds__(NT) void k_adj_step2(StepParams P) {
     __shared__ float sV[AZ * AX];  // V^t on tile + 4 (pitch AX), then V^{t-1} on tile + 2 (pitch BX)
     __shared__ float sS[2 * SAX + 2 * SAZ];
     const Grid& g = P.g;
-    const TileId tile = tile_id(P);
+    const TileId tile = tile_2step(P);
     const int ul = tile.unit;
     const int unit = P.unit0 + ul;
     int b, s;
@@ -1395,9 +1441,6 @@ __global__ __launch_bounds__(NT) void k_adj_step2(StepParams P) {
     const float rh = g.rh;
 
     if (!unit_active(P.geo, unit)) return;  // padding unit; its gradient rows are zeroed by the host
-    if (tile.x >= P.in_x0 && tile.x < P.in_x1 && tile.z >= P.in_z0 && tile.z < P.in_z1 &&
-        !tile_has_source(P, s, xi0, zi0, TX, TZ, false))
-        return;  // k_adj_step2_in runs this tile
 
     // cells of this thread: CPT of the tile (rows tid / TX + k RZ of column
     // tid % TX) and up to RPT of the ring (top and bottom rows, then the left
@@ -1774,21 +1817,39 @@ hipError_t fwd_step_t(const StepParams& p, int units, bool store, bool born, hip
     return hipGetLastError();
 }
 
+// A two-step launch: the lean kernel `lean` over the rectangle of tiles
+// interior_tiles() gives, and the general kernel `full` over the other
+// tiles (all of them when the rectangle is empty).
+template <class T>
+hipError_t two_step(const StepParams& p, int units, bool adjoint, void (*lean)(StepParams), void (*full)(StepParams),
+                    hipStream_t st) {
+    StepParams q = p;
+    const int gx = (p.g.ni + T::TX - 1) / T::TX, gz = (p.g.nzi + T::TZ - 1) / T::TZ;
+    interior_tiles(p, T::TX, T::TZ, adjoint, &q.in_x0, &q.in_x1, &q.in_z0, &q.in_z1);
+    if (q.in_x1 <= q.in_x0) {
+        const dim3 grid(gx, gz, units);
+        hipLaunchKernelGGL(full, grid, dim3(st::NT), 0, st, ordered(q, grid));
+        return hipGetLastError();
+    }
+    const dim3 gin(q.in_x1 - q.in_x0, q.in_z1 - q.in_z0, units);
+    hipLaunchKernelGGL(lean, gin, dim3(st::NT), 0, st, ordered(q, gin));
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const int iw = q.in_x1 - q.in_x0;
+    q.bnd_n = gx * gz - iw * (q.in_z1 - q.in_z0);
+    q.bgx = gx;
+    q.bgz = gz;
+    q.m_bgx = div_magic(gx, (unsigned long long)gx * gz);
+    q.m_bw = div_magic(gx - iw, (unsigned long long)gx * gz);
+    const dim3 gb(q.bnd_n, 1, units);
+    hipLaunchKernelGGL(full, gb, dim3(st::NT), 0, st, ordered(q, gb));
+    return hipGetLastError();
+}
+
 template <class T>
 hipError_t fwd_step2_t(const StepParams& p, int units, bool store, hipStream_t st) {
-    StepParams q = p;
-    interior_tiles(p, T::TX, T::TZ, false, &q.in_x0, &q.in_x1, &q.in_z0, &q.in_z1);
-    if (q.in_x1 > q.in_x0) {
-        const dim3 gin(q.in_x1 - q.in_x0, q.in_z1 - q.in_z0, units);
-        void (*fin)(StepParams) = store ? st::k_fwd_step2_in<T, true> : st::k_fwd_step2_in<T, false>;
-        hipLaunchKernelGGL(fin, gin, dim3(st::NT), 0, st, ordered(q, gin));
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
-    }
-    const dim3 grid((p.g.ni + T::TX - 1) / T::TX, (p.g.nzi + T::TZ - 1) / T::TZ, units);
-    void (*fn)(StepParams) = store ? st::k_fwd_step2<T, true> : st::k_fwd_step2<T, false>;
-    hipLaunchKernelGGL(fn, grid, dim3(st::NT), 0, st, ordered(q, grid));
-    return hipGetLastError();
+    return two_step<T>(p, units, false, store ? st::k_fwd_step2_in<T, true> : st::k_fwd_step2_in<T, false>,
+                       store ? st::k_fwd_step2<T, true> : st::k_fwd_step2<T, false>, st);
 }
 
 template <class T>
@@ -1801,17 +1862,7 @@ hipError_t adj_step_t(const StepParams& p, int units, hipStream_t st) {
 
 template <class T>
 hipError_t adj_step2_t(const StepParams& p, int units, hipStream_t st) {
-    StepParams q = p;
-    interior_tiles(p, T::TX, T::TZ, true, &q.in_x0, &q.in_x1, &q.in_z0, &q.in_z1);
-    if (q.in_x1 > q.in_x0) {
-        const dim3 gin(q.in_x1 - q.in_x0, q.in_z1 - q.in_z0, units);
-        hipLaunchKernelGGL(st::k_adj_step2_in<T>, gin, dim3(st::NT), 0, st, ordered(q, gin));
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
-    }
-    const dim3 grid((p.g.ni + T::TX - 1) / T::TX, (p.g.nzi + T::TZ - 1) / T::TZ, units);
-    hipLaunchKernelGGL(st::k_adj_step2<T>, grid, dim3(st::NT), 0, st, ordered(q, grid));
-    return hipGetLastError();
+    return two_step<T>(p, units, true, st::k_adj_step2_in<T>, st::k_adj_step2<T>, st);
 }
 
 using T0 = st::Tile<64, 16>;

@@ -126,11 +126,16 @@ struct StepParams {
     // tiles of a unit, the shots and nx (0: divide); n / d = mulhi(n, m)
     // exactly for n < 2^32 / d, which the launchers check.
     unsigned m_gx, m_gxy, m_shots, m_nx;
-    // Two-step adjoint: the tiles [in_x0, in_x1) x [in_z0, in_z1) (tile
-    // indices) whose cells and 2-cell ring lie outside the C-PML region and
-    // the receiver rows are run by the lean two-step kernel, unless they hold
-    // a source corner; empty when in_x1 <= in_x0.  Set by the launcher.
+    // Two-step launches: the tiles [in_x0, in_x1) x [in_z0, in_z1) (tile
+    // indices) whose cells and 2-cell ring lie outside the C-PML region (and,
+    // for the adjoint, the receiver rows) are run by the lean two-step kernel;
+    // empty when in_x1 <= in_x0.  Set by the launcher.
     int in_x0, in_x1, in_z0, in_z1;
+    // With bnd_n > 0 a general two-step launch covers only the other tiles of
+    // the full bgx x bgz tiling, bnd_n a unit, numbered by boundary_tile();
+    // m_bgx, m_bw: multipliers of bgx and of the side columns per middle row.
+    int bnd_n, bgx, bgz;
+    unsigned m_bgx, m_bw;
 };
 // vpad[m, z, x] = clamp(vel[b0 + m] at the clamped model cell of padded (x, z))
 hipError_t launch_pad_velocity(const Grid& g, const float* vel, int b0, int nb, float* vpad, hipStream_t st);
