@@ -1215,6 +1215,11 @@ def bench_large(args) -> int:
         },
         "dot_test": {"operator": "wavelet -> traces, 1 model x 20 shots, full nt", "rel": dot_rel},
     }
+    run_cfg = {"workload": "large", "models": B, "shots": S, "grid": f"{nvx}x{nvz}", "tile": args.tile,
+               "time_block": args.time_block}
+    traffic = profile_figure("traffic", adj_kern, "bytes_per_launch", run_cfg)
+    if traffic:
+        out["roofline"]["traffic"], out["roofline"]["traffic_source"] = traffic
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
         nt_cpu = 500
