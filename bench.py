@@ -390,8 +390,8 @@ def refine_summary(args) -> dict:
     """The second half of the metric, refine solves-to-misfit, measured in the
     same run: the network trained in-run on solver pairs, then plain projected
     L-BFGS at the full budget (its final misfit is the target) and the
-    learned arm at a quarter and at half of that budget."""
-    r = run_refine(args, plain_fracs=(1.0,), learned_fracs=(0.25, 0.5), pseudo=False,
+    learned arm at an eighth, a quarter and half of that budget."""
+    r = run_refine(args, plain_fracs=(1.0,), learned_fracs=(0.125, 0.25, 0.5), pseudo=False,
                    other_modes=("first", "replace"), other_fracs=(0.5, 1.0))
     v = r["value"]
     return {
@@ -439,7 +439,7 @@ def refine_lattice_field():
     return render_residual(torch.from_numpy(lat[act])[None], torch.from_numpy(act))[0]
 
 
-def run_refine(args, plain_fracs=(0.25, 0.5, 0.75, 1.0), learned_fracs=(0.25, 0.5, 0.75, 1.0),
+def run_refine(args, plain_fracs=(0.25, 0.5, 0.75, 1.0), learned_fracs=(0.125, 0.25, 0.5, 0.75, 1.0),
                pseudo: bool = True, other_modes=(), other_fracs=(0.5, 1.0)) -> dict:
     """Learned-preconditioned refinement against plain projected L-BFGS:
     solves needed to reach plain L-BFGS's final data misfit.  The network is
