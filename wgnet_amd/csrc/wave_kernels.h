@@ -221,6 +221,7 @@ struct QueueParams {
 };
 bool queue_state_fx(const Grid& g, size_t* fwd_floats, size_t* adj_floats);
 bool launch_queue_fx(const QueueParams& q, bool adjoint, int workgroups, hipStream_t st, hipError_t* err);
+bool launch_queue_fwd_fx(const QueueParams& q, int workgroups, hipStream_t st, hipError_t* err);
 // After a queued loss and gradient: NaN losses and gradients [B, nvis] (grad
 // may be null) when a queue wait timed out (*err != 0).
 hipError_t launch_flag_nan(const int* err, float* loss, int B, float* grad, long long nvis, hipStream_t st);
