@@ -126,7 +126,7 @@ __global__ __launch_bounds__(256) void k_loss_sum_lds(const float* partials, lon
 // visible node; the padded cells folding onto it are visited in row-major
 // order and summed in f64, as the reference's sequential fold does.
 __global__ void k_fold(Grid g, const float* grad_unit, const float* vel, const double* energy,
-                       int B, float* out) {
+                       int B, float* out, CellLayout lay) {
     const int nvis = g.nvx * g.nvz;
     const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= (long long)B * nvis) return;
@@ -139,8 +139,8 @@ __global__ void k_fold(Grid g, const float* grad_unit, const float* vel, const d
     const int zhi = vz == g.nvz - 1 ? g.nz - 1 : vz + g.pad;
     const int xlo = vx == 0 ? 0 : vx + g.pad;
     const int xhi = vx == g.nvx - 1 ? g.nx - 1 : vx + g.pad;
-    const int cells = g.nx * g.nz;
-    const float* gb = grad_unit + (size_t)b * g.shots * cells;
+    const size_t cells = lay.ustride;
+    const float* gb = grad_unit + (size_t)b * g.shots * cells + lay.off;
     double acc = 0.0;
     for (int z = zlo; z <= zhi; ++z) {
         int x = xlo;
@@ -148,7 +148,7 @@ __global__ void k_fold(Grid g, const float* grad_unit, const float* vel, const d
         for (; x + 3 <= xhi; x += 4) {
             float sum_s[4] = {0.0f, 0.0f, 0.0f, 0.0f};
             for (int s = 0; s < g.shots; ++s) {
-                const float* r = gb + (size_t)s * cells + z * g.nx + x;
+                const float* r = gb + (size_t)s * cells + z * lay.pitch + x;
 #pragma unroll
                 for (int u = 0; u < 4; ++u) sum_s[u] += r[u];
             }
@@ -156,7 +156,7 @@ __global__ void k_fold(Grid g, const float* grad_unit, const float* vel, const d
             for (int u = 0; u < 4; ++u) acc += (double)(sum_s[u] * mask);
         }
         for (; x <= xhi; ++x) {
-            const int c = z * g.nx + x;
+            const int c = z * lay.pitch + x;
             float sum_s = 0.0f;
             for (int s = 0; s < g.shots; ++s) sum_s += gb[(size_t)s * cells + c];
             acc += (double)(sum_s * mask);
@@ -264,9 +264,15 @@ hipError_t launch_loss_sum(const float* partials, long long chunks, int B, const
 
 hipError_t launch_fold(const Grid& g, const float* grad_unit, const float* vel, const double* energy,
                        int B, float* grad_vis, hipStream_t st) {
+    return launch_fold(g, grad_unit, vel, energy, B, grad_vis, plain_layout(g), st);
+}
+
+hipError_t launch_fold(const Grid& g, const float* grad_unit, const float* vel, const double* energy,
+                       int B, float* grad_vis, const CellLayout& lay, hipStream_t st) {
     if (B <= 0) return hipSuccess;
     const size_t lds = (size_t)g.nx * g.nz * sizeof(float);
-    if (lds <= kFoldLdsMax) {
+    const bool plain = lay.pitch == g.nx && lay.off == 0 && lay.ustride == (size_t)g.nx * g.nz;
+    if (plain && lds <= kFoldLdsMax) {
         hipError_t e = hipFuncSetAttribute((const void*)k_fold_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
                                            (int)lds);
         if (e != hipSuccess) return e;
@@ -276,7 +282,7 @@ hipError_t launch_fold(const Grid& g, const float* grad_unit, const float* vel, 
     }
     const long long n = (long long)B * g.nvx * g.nvz;
     hipLaunchKernelGGL(k_fold, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, grad_unit, vel,
-                       energy, B, grad_vis);
+                       energy, B, grad_vis, lay);
     return hipGetLastError();
 }
 

@@ -75,6 +75,13 @@ __device__ __forceinline__ void unit_model(const StepParams& P, int unit, int& b
     s = src_slot(P.geo, unit, unit - b * P.g.shots);
 }
 
+// Storage index (z * sp + x) of grid cell `cell` = z * nx + x (receivers and
+// the wavelet gradient; the steps compare source corners by grid cell).
+__device__ __forceinline__ int sidx(const StepParams& P, int cell) {
+    const int z = (int)udiv((unsigned)cell, (unsigned)P.g.nx, P.m_nx);
+    return cell + z * (P.sp - P.g.nx);
+}
+
 // Tile of a general two-step launch over the tiles outside the lean kernels'
 // rectangle: boundary tile i of a unit is, in order, a tile of the rows above
 // the rectangle, of the rows below it, or of the side columns of its rows.
@@ -181,8 +188,8 @@ __device__ __forceinline__ void fwd_interior(const StepParams& P, float* sU, int
                                              int zi0) {
     WG_TILE_CONSTANTS
     const Grid& g = P.g;
-    const size_t cells = (size_t)g.nx * g.nz;
-    const int nx = g.nx;
+    const size_t cells = P.sc;
+    const int nx = P.sp;  // row pitch of the arrays
     const int tid = threadIdx.x, lx = tid % TX, lz0 = tid / TX;
     const float* Ucur = P.state + ((size_t)fwd_u_arr(P.u_cur) * P.units + ul) * cells;
     float* Unxt = P.state + ((size_t)fwd_u_arr(P.u_prev) * P.units + ul) * cells;
@@ -197,8 +204,8 @@ __device__ __forceinline__ void fwd_interior(const StepParams& P, float* sU, int
     }
     stage_interior<TL>(sU, Ucur, nx, xi0, zi0);
     __syncthreads();
-    float* snap = STORE ? P.snaps + ((size_t)(P.t - P.snap_t0) * P.units + ul) * g.inner : nullptr;
-    const int q0 = (zi0 + lz0) * g.ni + xi0 + lx;  // snapshot index of the first cell
+    float* snap = STORE ? P.snaps + ((size_t)(P.t - P.snap_t0) * P.units + ul) * P.sinner : nullptr;
+    const int q0 = (zi0 + lz0) * P.sni + xi0 + lx;  // snapshot index of the first cell
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
         const int lz = lz0 + RZ * k;
@@ -209,7 +216,7 @@ __device__ __forceinline__ void fwd_interior(const StepParams& P, float* sU, int
         const float w = fwd_w<false, false>(sU, li, HX, u, g.rh, g.rh2, nullptr, none, dummy, dummy, nullptr, TX, none,
                                             dummy, dummy);
         const float v = pv[k];
-        if (STORE) snap[q0 + k * RZ * g.ni] = 2.0f * v * g.dt2 * w;
+        if (STORE) snap[q0 + k * RZ * P.sni] = 2.0f * v * g.dt2 * w;
         Unxt[i0 + k * RZ * nx] = v * v * g.dt2 * w + 2.0f * u - pu[k];
     }
 }
@@ -221,8 +228,8 @@ __device__ __forceinline__ void adj_interior(const StepParams& P, float* sV, int
                                              int zi0) {
     WG_TILE_CONSTANTS
     const Grid& g = P.g;
-    const size_t cells = (size_t)g.nx * g.nz;
-    const int nx = g.nx;
+    const size_t cells = P.sc;
+    const int nx = P.sp;  // row pitch of the arrays
     const int tid = threadIdx.x, lx = tid % TX, r0 = tid / TX;
     auto arr = [&](int a) { return P.state + ((size_t)a * P.units + ul) * cells; };
     const float* Lcur = arr(P.lam_cur < 2 ? P.lam_cur : P.lam_cur + 8);
@@ -231,15 +238,15 @@ __device__ __forceinline__ void adj_interior(const StepParams& P, float* sV, int
     const float* vp = P.vpad + (size_t)(b - P.vpad_b0) * cells;
     const int xo = xi0 + lx + 2;
     const int i0 = (zi0 + r0 + 2) * nx + xo;
-    const float* snap = P.snaps + ((size_t)(P.t - P.snap_t0) * P.units + ul) * g.inner;
-    const int q0 = (zi0 + r0) * g.ni + xi0 + lx;
+    const float* snap = P.snaps + ((size_t)(P.t - P.snap_t0) * P.units + ul) * P.sinner;
+    const int q0 = (zi0 + r0) * P.sni + xi0 + lx;
     float pl[CPT], pn[CPT], ps[CPT], pg[CPT];
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
         const int i = i0 + k * RZ * nx;
         pl[k] = Lcur[i];
         pn[k] = Lnxt[i];
-        ps[k] = snap[q0 + k * RZ * g.ni];
+        ps[k] = snap[q0 + k * RZ * P.sni];
         pg[k] = grad[i];
     }
     {
@@ -306,7 +313,7 @@ __global__ __launch_bounds__(NT) void k_fwd_step(StepParams P) {
     const int unit = P.unit0 + ul;
     int b, s;
     unit_model(P, unit, b, s);
-    const size_t cells = (size_t)g.nx * g.nz;
+    const size_t cells = P.sc;
     const int t = P.t, h = P.ps_half;
     auto arr = [&](int a) { return P.state + ((size_t)a * P.units + ul) * cells; };
     float* Ucur = arr(fwd_u_arr(P.u_cur));
@@ -338,7 +345,7 @@ __global__ __launch_bounds__(NT) void k_fwd_step(StepParams P) {
         for (int r = tid; r < g.nrec; r += NT) {
             float acc = 0.0f;
 #pragma unroll
-            for (int c = 0; c < 4; ++c) acc += P.geo.rec_w[r * 4 + c] * Ucur[P.geo.rec_cell[r * 4 + c]];
+            for (int c = 0; c < 4; ++c) acc += P.geo.rec_w[r * 4 + c] * Ucur[sidx(P, P.geo.rec_cell[r * 4 + c])];
             tr[r] = acc;
         }
     }
@@ -370,7 +377,7 @@ __global__ __launch_bounds__(NT) void k_fwd_step(StepParams P) {
     float sw[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-        scell[c] = P.geo.src_cell[s * 4 + c];
+        scell[c] = P.geo.src_cell[s * 4 + c];  // grid cell z * nx + x
         sw[c] = P.geo.src_w[s * 4 + c];
     }
     float pu[CPT], pv[CPT], ppx[CPT], pzx[CPT], ppz[CPT], pzz[CPT];
@@ -379,7 +386,7 @@ __global__ __launch_bounds__(NT) void k_fwd_step(StepParams P) {
         const int z = zi0 + tid / TX + RZ * k + 2;
         pu[k] = pv[k] = ppx[k] = pzx[k] = ppz[k] = pzz[k] = 0.0f;
         if (xo < g.nx - 2 && z < g.nz - 2) {
-            const int i = z * g.nx + xo;
+            const int i = z * P.sp + xo;
             pu[k] = Unxt[i];
             pv[k] = vis_raw(vvis, g, xo, z);
             if (pxo) {
@@ -401,7 +408,7 @@ __global__ __launch_bounds__(NT) void k_fwd_step(StepParams P) {
         for (int k = 0; k < NS; ++k) {
             const int j = tid + k * NT;
             const int x = xi0 + (j % HX), z = zi0 + (j / HX);  // padded coords: halo starts at inner - 2
-            tu[k] = (j < HZ * HX && x < g.nx && z < g.nz) ? Ucur[z * g.nx + x] : 0.0f;
+            tu[k] = (j < HZ * HX && x < g.nx && z < g.nz) ? Ucur[z * P.sp + x] : 0.0f;
         }
 #pragma unroll
         for (int k = 0; k < NS; ++k)
@@ -414,7 +421,7 @@ __global__ __launch_bounds__(NT) void k_fwd_step(StepParams P) {
         for (int k = 0; k < NS; ++k) {
             const int j = tid + k * NT;
             const int x = xi0 + (j % HX), z = zi0 + 2 + (j / HX);
-            ta[k] = (j < TZ * HX && x < g.nx && z < g.nz) ? pax[x] * PXc[z * g.nx + x] : 0.0f;
+            ta[k] = (j < TZ * HX && x < g.nx && z < g.nz) ? pax[x] * PXc[z * P.sp + x] : 0.0f;
         }
 #pragma unroll
         for (int k = 0; k < NS; ++k)
@@ -427,7 +434,7 @@ __global__ __launch_bounds__(NT) void k_fwd_step(StepParams P) {
         for (int k = 0; k < NS; ++k) {
             const int j = tid + k * NT;
             const int x = xi0 + 2 + (j % TX), z = zi0 + (j / TX);
-            ta[k] = (j < HZ * TX && x < g.nx && z < g.nz) ? paz[z] * PZc[z * g.nx + x] : 0.0f;
+            ta[k] = (j < HZ * TX && x < g.nx && z < g.nz) ? paz[z] * PZc[z * P.sp + x] : 0.0f;
         }
 #pragma unroll
         for (int k = 0; k < NS; ++k)
@@ -440,7 +447,7 @@ __global__ __launch_bounds__(NT) void k_fwd_step(StepParams P) {
         const int lz = tid / TX + RZ * k;
         const int x = xo, z = zi0 + lz + 2;
         if (x >= g.nx - 2 || z >= g.nz - 2) continue;
-        const int i = z * g.nx + x;
+        const int i = z * P.sp + x;
         const int li = (lz + 2) * HX + (lx + 2);
         const float u = sU[li];
         const bool px = x < g.region || x >= g.nx - g.region;
@@ -464,19 +471,19 @@ __global__ __launch_bounds__(NT) void k_fwd_step(StepParams P) {
             w = fwd_w<false, false>(sU, li, HX, u, g.rh, g.rh2, AXc, mx, psx, zex, AZc, TX, mz, psz, zez);
         const float vr = pv[k];
         const float v = clampv(vr, g.vmin, g.vmax);
-        const size_t q = (size_t)(z - 2) * g.ni + (x - 2);
-        if (STORE) P.snaps[((size_t)(t - P.snap_t0) * P.units + ul) * g.inner + q] = 2.0f * v * g.dt2 * w;
+        const size_t q = (size_t)(z - 2) * P.sni + (x - 2);
+        if (STORE) P.snaps[((size_t)(t - P.snap_t0) * P.units + ul) * P.sinner + q] = 2.0f * v * g.dt2 * w;
         float next = v * v * g.dt2 * w + 2.0f * u - pu[k];
         if (BORN) {
             const float m = (vr > g.vmin && vr < g.vmax) ? 1.0f : 0.0f;
             next += (vis_raw(P.dvel + (size_t)b * g.nvx * g.nvz, g, x, z) * m) *
-                    P.born_snaps[((size_t)(t - P.snap_t0) * P.units + ul) * g.inner + q];
+                    P.born_snaps[((size_t)(t - P.snap_t0) * P.units + ul) * P.sinner + q];
         }
         // point source after the update (lib.rs:209-215); Born: its derivative
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
             const float wc = sw[c];
-            if (scell[c] == i && wc != 0.0f) {
+            if (scell[c] == z * g.nx + x && wc != 0.0f) {
                 if (BORN) {
                     const float m = (vr > g.vmin && vr < g.vmax) ? 1.0f : 0.0f;
                     const float dv = vis_raw(P.dvel + (size_t)b * g.nvx * g.nvz, g, x, z) * m;
@@ -532,7 +539,7 @@ __global__ __launch_bounds__(NT) void k_fwd_step2(StepParams P) {
     const int unit = P.unit0 + ul;
     int b, s;
     unit_model(P, unit, b, s);
-    const size_t cells = (size_t)g.nx * g.nz;
+    const size_t cells = P.sc;
     const int t = P.t, h = P.ps_half;
     auto arr = [&](int a) { return P.state + ((size_t)a * P.units + ul) * cells; };
     const float* Ucur = arr(fwd_u_arr(P.u_cur));    // u^t
@@ -570,12 +577,12 @@ __global__ __launch_bounds__(NT) void k_fwd_step2(StepParams P) {
         for (int r = tid; r < g.nrec; r += NT) {
             float acc = 0.0f;
 #pragma unroll
-            for (int c = 0; c < 4; ++c) acc += P.geo.rec_w[r * 4 + c] * Ucur[P.geo.rec_cell[r * 4 + c]];
+            for (int c = 0; c < 4; ++c) acc += P.geo.rec_w[r * 4 + c] * Ucur[sidx(P, P.geo.rec_cell[r * 4 + c])];
             P.traces[((size_t)unit * g.nt + t) * g.nrec + r] = acc;
             if (P.trace_prev) {
                 float ap = 0.0f;
 #pragma unroll
-                for (int c = 0; c < 4; ++c) ap += P.geo.rec_w[r * 4 + c] * Uprev[P.geo.rec_cell[r * 4 + c]];
+                for (int c = 0; c < 4; ++c) ap += P.geo.rec_w[r * 4 + c] * Uprev[sidx(P, P.geo.rec_cell[r * 4 + c])];
                 P.traces[((size_t)unit * g.nt + t - 1) * g.nrec + r] = ap;
             }
         }
@@ -590,7 +597,7 @@ __global__ __launch_bounds__(NT) void k_fwd_step2(StepParams P) {
     float sw[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-        scell[c] = P.geo.src_cell[s * 4 + c];
+        scell[c] = P.geo.src_cell[s * 4 + c];  // grid cell z * nx + x
         sw[c] = P.geo.src_w[s * 4 + c];
     }
     // own state of this thread's cells, loaded before the staging so that its
@@ -605,7 +612,7 @@ __global__ __launch_bounds__(NT) void k_fwd_step2(StepParams P) {
         pu[k] = pv[k] = ppx[k] = pzx[k] = ppz[k] = pzz[k] = 0.0f;
         if (inner_cell(lx, lz)) {
             const int x = xi0 + 2 + lx, z = zi0 + 2 + lz;
-            const int i = z * g.nx + x;
+            const int i = z * P.sp + x;
             pu[k] = Uprev[i];
             pv[k] = vis_raw(vvis, g, x, z);
             if (pml_x(x)) {
@@ -642,7 +649,7 @@ __global__ __launch_bounds__(NT) void k_fwd_step2(StepParams P) {
         ru[r] = rv[r] = rpx[r] = rzx[r] = rpz[r] = rzz[r] = 0.0f;
         if (rz != -100 && inner_cell(rx, rz)) {
             const int x = xi0 + 2 + rx, z = zi0 + 2 + rz;
-            const int i = z * g.nx + x;
+            const int i = z * P.sp + x;
             ru[r] = Uprev[i];
             rv[r] = vis_raw(vvis, g, x, z);
             if (pml_x(x)) {
@@ -664,7 +671,7 @@ __global__ __launch_bounds__(NT) void k_fwd_step2(StepParams P) {
         for (int k = 0; k < NS; ++k) {
             const int j = tid + k * NT;
             const int x = xi0 - 2 + (j % AX), z = zi0 - 2 + (j / AX);
-            tu[k] = (j < AZ * AX && x >= 0 && z >= 0 && x < g.nx && z < g.nz) ? Ucur[z * g.nx + x] : 0.0f;
+            tu[k] = (j < AZ * AX && x >= 0 && z >= 0 && x < g.nx && z < g.nz) ? Ucur[z * P.sp + x] : 0.0f;
         }
 #pragma unroll
         for (int k = 0; k < NS; ++k)
@@ -681,7 +688,7 @@ __global__ __launch_bounds__(NT) void k_fwd_step2(StepParams P) {
         for (int k = 0; k < NS; ++k) {
             const int j = tid + k * NT;
             const int x = xi0 - 2 + (j % AX), z = zi0 + (j / AX);
-            ta[k] = (j < BZ * AX && x >= 0 && x < g.nx && z < g.nz) ? pax[x] * PXr[z * g.nx + x] : 0.0f;
+            ta[k] = (j < BZ * AX && x >= 0 && x < g.nx && z < g.nz) ? pax[x] * PXr[z * P.sp + x] : 0.0f;
         }
 #pragma unroll
         for (int k = 0; k < NS; ++k)
@@ -694,7 +701,7 @@ __global__ __launch_bounds__(NT) void k_fwd_step2(StepParams P) {
         for (int k = 0; k < NS; ++k) {
             const int j = tid + k * NT;
             const int x = xi0 + (j % BX), z = zi0 - 2 + (j / BX);
-            ta[k] = (j < AZ * BX && z >= 0 && x < g.nx && z < g.nz) ? paz[z] * PZr[z * g.nx + x] : 0.0f;
+            ta[k] = (j < AZ * BX && z >= 0 && x < g.nx && z < g.nz) ? paz[z] * PZr[z * P.sp + x] : 0.0f;
         }
 #pragma unroll
         for (int k = 0; k < NS; ++k)
@@ -708,7 +715,6 @@ __global__ __launch_bounds__(NT) void k_fwd_step2(StepParams P) {
     auto step_a = [&](int rx, int rz, float up, float vr, float& psx, float& zex, float& psz, float& zez, float& uc,
                       bool snap) -> float {
         const int x = xi0 + 2 + rx, z = zi0 + 2 + rz;
-        const int i = z * g.nx + x;
         const int li = (rz + 4) * AX + (rx + 4);
         const float u = sU[li];
         uc = u;
@@ -735,13 +741,13 @@ __global__ __launch_bounds__(NT) void k_fwd_step2(StepParams P) {
             w = fwd_w<false, false>(sU, li, AX, u, g.rh, g.rh2, AXc, mx, psx, zex, AZc, BX, mz, psz, zez);
         const float v = clampv(vr, g.vmin, g.vmax);
         if (STORE && snap) {
-            const size_t q = (size_t)(z - 2) * g.ni + (x - 2);
-            P.snaps[((size_t)(t - P.snap_t0) * P.units + ul) * g.inner + q] = 2.0f * v * g.dt2 * w;
+            const size_t q = (size_t)(z - 2) * P.sni + (x - 2);
+            P.snaps[((size_t)(t - P.snap_t0) * P.units + ul) * P.sinner + q] = 2.0f * v * g.dt2 * w;
         }
         float next = v * v * g.dt2 * w + 2.0f * u - up;
 #pragma unroll
         for (int c = 0; c < 4; ++c)
-            if (scell[c] == i && sw[c] != 0.0f) next += -(v * v * g.dt2) * sw[c] * f_a;
+            if (scell[c] == z * g.nx + x && sw[c] != 0.0f) next += -(v * v * g.dt2) * sw[c] * f_a;
         return next;
     };
     float ut[CPT], un[CPT];  // u^t and u^{t+1} of the tile cells
@@ -794,7 +800,7 @@ __global__ __launch_bounds__(NT) void k_fwd_step2(StepParams P) {
         const int lz = tid / TX + RZ * k;
         if (!inner_cell(lx, lz)) continue;
         const int x = xo, z = zi0 + 2 + lz;
-        const int i = z * g.nx + x;
+        const int i = z * P.sp + x;
         const int li = (lz + 2) * BX + (lx + 2);
         const float u = sU[li];
         const bool pz = pml_z(z);
@@ -817,13 +823,13 @@ __global__ __launch_bounds__(NT) void k_fwd_step2(StepParams P) {
             w = fwd_w<false, false>(sU, li, BX, u, g.rh, g.rh2, AXc, mx, psx, zex, AZc, TX, mz, psz, zez);
         const float v = clampv(pv[k], g.vmin, g.vmax);
         if (STORE) {
-            const size_t q = (size_t)(z - 2) * g.ni + (x - 2);
-            P.snaps[((size_t)(t1 - P.snap_t0) * P.units + ul) * g.inner + q] = 2.0f * v * g.dt2 * w;
+            const size_t q = (size_t)(z - 2) * P.sni + (x - 2);
+            P.snaps[((size_t)(t1 - P.snap_t0) * P.units + ul) * P.sinner + q] = 2.0f * v * g.dt2 * w;
         }
         float next = v * v * g.dt2 * w + 2.0f * u - ut[k];
 #pragma unroll
         for (int c = 0; c < 4; ++c)
-            if (scell[c] == i && sw[c] != 0.0f) next += -(v * v * g.dt2) * sw[c] * f_b;
+            if (scell[c] == z * g.nx + x && sw[c] != 0.0f) next += -(v * v * g.dt2) * sw[c] * f_b;
         O0[i] = u;
         O1[i] = next;
         if (pxo) {
@@ -841,7 +847,7 @@ __global__ __launch_bounds__(NT) void k_fwd_step2(StepParams P) {
 __global__ __launch_bounds__(NT) void k_fwd_record(StepParams P) {
     const Grid& g = P.g;
     const int ul = blockIdx.x, unit = P.unit0 + ul;
-    const size_t cells = (size_t)g.nx * g.nz;
+    const size_t cells = P.sc;
     const float* U = P.state + ((size_t)fwd_u_arr(P.u_cur) * P.units + ul) * cells;
     float* tr = P.traces + ((size_t)unit * g.nt + P.t) * g.nrec;
     const bool on = unit_active(P.geo, unit);
@@ -849,7 +855,7 @@ __global__ __launch_bounds__(NT) void k_fwd_record(StepParams P) {
         float acc = 0.0f;
         if (on) {
 #pragma unroll
-            for (int c = 0; c < 4; ++c) acc += P.geo.rec_w[r * 4 + c] * U[P.geo.rec_cell[r * 4 + c]];
+            for (int c = 0; c < 4; ++c) acc += P.geo.rec_w[r * 4 + c] * U[sidx(P, P.geo.rec_cell[r * 4 + c])];
         }
         tr[r] = acc;
     }
@@ -870,7 +876,7 @@ __global__ __launch_bounds__(NT) void k_adj_step(StepParams P) {
     const int unit = P.unit0 + ul;
     int b, s;
     unit_model(P, unit, b, s);
-    const size_t cells = (size_t)g.nx * g.nz;
+    const size_t cells = P.sc;
     const int t = P.t, q = P.half;
     // arrays: lam0 lam1 | Px0 Px1 | Zx0 Zx1 | Pz0 Pz1 | Zz0 Zz1 | lam2 lam3; the
     // memory variables are read from half q and written to half 1 - q, and
@@ -908,7 +914,7 @@ __global__ __launch_bounds__(NT) void k_adj_step(StepParams P) {
             const int cell = P.geo.src_cell[s * 4 + c];
             const int cz = cell / g.nx, cx = cell - cz * g.nx;
             const float v = vel_at(vvis, g, cx, cz);
-            acc += (-(v * v * g.dt2) * P.geo.src_w[s * 4 + c]) * Lcur[cell];
+            acc += (-(v * v * g.dt2) * P.geo.src_w[s * 4 + c]) * Lcur[sidx(P, cell)];
         }
         P.dwav[(size_t)unit * g.nt + t] = acc;
     }
@@ -935,7 +941,7 @@ __global__ __launch_bounds__(NT) void k_adj_step(StepParams P) {
     float sw[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-        scell[c] = P.geo.src_cell[s * 4 + c];
+        scell[c] = P.geo.src_cell[s * 4 + c];  // grid cell z * nx + x
         sw[c] = P.geo.src_w[s * 4 + c];
     }
     float pl[CPT], pn[CPT], ps[CPT], pg[CPT], ppx[CPT], pzx[CPT], ppz[CPT], pzz[CPT];
@@ -944,10 +950,10 @@ __global__ __launch_bounds__(NT) void k_adj_step(StepParams P) {
         const int z = zi0 + tid / TX + RZ * k + 2;
         pl[k] = pn[k] = ps[k] = pg[k] = ppx[k] = pzx[k] = ppz[k] = pzz[k] = 0.0f;
         if (xo < g.nx - 2 && z < g.nz - 2) {
-            const int i = z * g.nx + xo;
+            const int i = z * P.sp + xo;
             pl[k] = Lcur[i];
             pn[k] = Lnxt[i];
-            ps[k] = P.snaps[((size_t)(t - P.snap_t0) * P.units + ul) * g.inner + (size_t)(z - 2) * g.ni + (xo - 2)];
+            ps[k] = P.snaps[((size_t)(t - P.snap_t0) * P.units + ul) * P.sinner + (size_t)(z - 2) * P.sni + (xo - 2)];
             pg[k] = grad[i];
             if (pxo) {
                 ppx[k] = PXr[i];
@@ -971,7 +977,7 @@ __global__ __launch_bounds__(NT) void k_adj_step(StepParams P) {
             tl[k] = tv[k] = 0.0f;
             if (j < HZ * HX && x < g.nx && z < g.nz) {
                 tv[k] = vel_at(vvis, g, x, z);
-                tl[k] = Lcur[z * g.nx + x];
+                tl[k] = Lcur[z * P.sp + x];
             }
         }
 #pragma unroll
@@ -986,7 +992,7 @@ __global__ __launch_bounds__(NT) void k_adj_step(StepParams P) {
             const int x = xi0 + hx, z = zi0 + 2 + hz;
             float T = 0.0f, U1 = 0.0f;
             if (x < g.nx && z < g.nz) {
-                const int i = z * g.nx + x;
+                const int i = z * P.sp + x;
                 const float bb = pbx[x];
                 T = (1.0f + bb) * sV[(hz + 2) * HX + hx] + bb * ZXr[i];
                 U1 = pdbx[x] * T + bb * PXr[i];
@@ -1000,7 +1006,7 @@ __global__ __launch_bounds__(NT) void k_adj_step(StepParams P) {
             const int x = xi0 + 2 + hx, z = zi0 + hz;
             float T = 0.0f, U1 = 0.0f;
             if (x < g.nx && z < g.nz) {
-                const int i = z * g.nx + x;
+                const int i = z * P.sp + x;
                 const float bb = pbz[z];
                 T = (1.0f + bb) * sV[hz * HX + hx + 2] + bb * ZZr[i];
                 U1 = pdbz[z] * T + bb * PZr[i];
@@ -1016,7 +1022,7 @@ __global__ __launch_bounds__(NT) void k_adj_step(StepParams P) {
         const int lz = tid / TX + RZ * k;
         const int x = xo, z = zi0 + lz + 2;
         if (x >= g.nx - 2 || z >= g.nz - 2) continue;
-        const int i = z * g.nx + x;
+        const int i = z * P.sp + x;
         const int li = (lz + 2) * HX + (lx + 2);
         const float v0 = sV[li];
         const float rh = g.rh;
@@ -1048,7 +1054,7 @@ __global__ __launch_bounds__(NT) void k_adj_step(StepParams P) {
         }
 #pragma unroll
         for (int c = 0; c < 4; ++c)
-            if (scell[c] == i) {
+            if (scell[c] == z * g.nx + x) {
                 const float v = vel_at(vvis, g, x, z);
                 gk += lam * (-2.0f * v * g.dt2 * sw[c] * f_t);
             }
@@ -1086,15 +1092,15 @@ __global__ __launch_bounds__(NT) void k_fwd_step2_in(StepParams P) {
     const bool src = box_has_source(P, s, xi0, xi0 + TX + 4, zi0, zi0 + TZ + 4, true);
     const float f_a = P.wavelet[(size_t)unit * P.wav_stride + P.t];
     const float f_b = P.wavelet[(size_t)unit * P.wav_stride + P.t + 1];
-    auto inject = [&](int i, float v, float f, float& next) {  // corners read only in such a tile
+    auto inject = [&](int gi, float v, float f, float& next) {  // gi: grid cell; corners read only in such a tile
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
             const float wc = P.geo.src_w[s * 4 + c];
-            if (P.geo.src_cell[s * 4 + c] == i && wc != 0.0f) next += -(v * v * g.dt2) * wc * f;
+            if (P.geo.src_cell[s * 4 + c] == gi && wc != 0.0f) next += -(v * v * g.dt2) * wc * f;
         }
     };
-    const size_t cells = (size_t)g.nx * g.nz;
-    const int nx = g.nx, t = P.t;
+    const size_t cells = P.sc;
+    const int nx = P.sp, t = P.t;  // nx: row pitch of the arrays
     auto arr = [&](int a) { return P.state + ((size_t)a * P.units + ul) * cells; };
     const float* Ucur = arr(fwd_u_arr(P.u_cur));    // u^t
     const float* Uprev = arr(fwd_u_arr(P.u_prev));  // u^{t-1}
@@ -1104,6 +1110,7 @@ __global__ __launch_bounds__(NT) void k_fwd_step2_in(StepParams P) {
     const int tid = threadIdx.x, lx = tid % TX, r0 = tid / TX;
 
     const int i0 = (zi0 + 2 + r0) * nx + xi0 + 2 + lx;
+    const int g0 = (zi0 + 2 + r0) * g.nx + xi0 + 2 + lx;  // its grid cell
     float pu[CPT], pv[CPT];
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
@@ -1154,9 +1161,9 @@ __global__ __launch_bounds__(NT) void k_fwd_step2_in(StepParams P) {
     __syncthreads();
     Pml none{0, 0, 0, 0};
     float dummy = 0.0f;
-    float* snap0 = STORE ? P.snaps + ((size_t)(t - P.snap_t0) * P.units + ul) * g.inner : nullptr;
-    float* snap1 = STORE ? P.snaps + ((size_t)(t + 1 - P.snap_t0) * P.units + ul) * g.inner : nullptr;
-    const int q0 = (zi0 + r0) * g.ni + xi0 + lx;
+    float* snap0 = STORE ? P.snaps + ((size_t)(t - P.snap_t0) * P.units + ul) * P.sinner : nullptr;
+    float* snap1 = STORE ? P.snaps + ((size_t)(t + 1 - P.snap_t0) * P.units + ul) * P.sinner : nullptr;
+    const int q0 = (zi0 + r0) * P.sni + xi0 + lx;
     // Step A (time t) on the tile and its ring: u^{t+1}
     float ut[CPT], un[CPT];
 #pragma unroll
@@ -1167,9 +1174,9 @@ __global__ __launch_bounds__(NT) void k_fwd_step2_in(StepParams P) {
         const float w = fwd_w<false, false>(sU, li, AX, u, g.rh, g.rh2, nullptr, none, dummy, dummy, nullptr, BX,
                                             none, dummy, dummy);
         const float v = pv[k];
-        if (STORE) snap0[q0 + k * RZ * g.ni] = 2.0f * v * g.dt2 * w;
+        if (STORE) snap0[q0 + k * RZ * P.sni] = 2.0f * v * g.dt2 * w;
         float next = v * v * g.dt2 * w + 2.0f * u - pu[k];
-        if (src) inject(i0 + k * RZ * nx, v, f_a, next);
+        if (src) inject(g0 + k * RZ * g.nx, v, f_a, next);
         un[k] = next;
     }
     float rn[RPT];
@@ -1183,7 +1190,7 @@ __global__ __launch_bounds__(NT) void k_fwd_step2_in(StepParams P) {
                                                 none, dummy, dummy);
             const float v = rv[r];
             float next = v * v * g.dt2 * w + 2.0f * u - ru[r];
-            if (src) inject((zi0 + 2 + rrz[r]) * nx + (xi0 + 2 + rrx[r]), v, f_a, next);
+            if (src) inject((zi0 + 2 + rrz[r]) * g.nx + (xi0 + 2 + rrx[r]), v, f_a, next);
             rn[r] = next;
         }
     }
@@ -1202,10 +1209,10 @@ __global__ __launch_bounds__(NT) void k_fwd_step2_in(StepParams P) {
         const float w = fwd_w<false, false>(sU, li, BX, u, g.rh, g.rh2, nullptr, none, dummy, dummy, nullptr, TX,
                                             none, dummy, dummy);
         const float v = pv[k];
-        if (STORE) snap1[q0 + k * RZ * g.ni] = 2.0f * v * g.dt2 * w;
+        if (STORE) snap1[q0 + k * RZ * P.sni] = 2.0f * v * g.dt2 * w;
         const int i = i0 + k * RZ * nx;
         float next = v * v * g.dt2 * w + 2.0f * u - ut[k];
-        if (src) inject(i, v, f_b, next);
+        if (src) inject(g0 + k * RZ * g.nx, v, f_b, next);
         O0[i] = u;
         O1[i] = next;
     }
@@ -1238,14 +1245,14 @@ __global__ __launch_bounds__(NT) void k_adj_step2_in(StepParams P) {
     // a source corner in the tile: its gradient term (wgsl:245-249), tested
     // per cell only in such a tile
     const bool src = tile_has_source(P, s, xi0, zi0, TX, TZ, false);
-    auto src_grad = [&](int i, float v, float l, int tt, float& gg) {  // corners read only in such a tile
+    auto src_grad = [&](int gi, float v, float l, int tt, float& gg) {  // gi: grid cell; corners read only in such a tile
         const float f = P.wavelet[(size_t)unit * P.wav_stride + tt];
 #pragma unroll
         for (int c = 0; c < 4; ++c)
-            if (P.geo.src_cell[s * 4 + c] == i) gg += l * (-2.0f * v * g.dt2 * P.geo.src_w[s * 4 + c] * f);
+            if (P.geo.src_cell[s * 4 + c] == gi) gg += l * (-2.0f * v * g.dt2 * P.geo.src_w[s * 4 + c] * f);
     };
-    const size_t cells = (size_t)g.nx * g.nz;
-    const int nx = g.nx, t = P.t;
+    const size_t cells = P.sc;
+    const int nx = P.sp, t = P.t;  // nx: row pitch of the arrays
     auto lam = [&](int k) { return P.state + ((size_t)(k < 2 ? k : k + 8) * P.units + ul) * cells; };
     const float* Lcur = lam(P.lam_cur);    // lambda^t
     const float* Lprev = lam(P.lam_prev);  // lambda^{t+1}
@@ -1258,13 +1265,14 @@ __global__ __launch_bounds__(NT) void k_adj_step2_in(StepParams P) {
 
     // the thread's tile cells and ring cells, loaded before the staging
     const int i0 = (zi0 + 2 + r0) * nx + xi0 + 2 + lx;
-    const float* sn0 = P.snaps + ((size_t)(t - P.snap_t0) * P.units + ul) * g.inner;
-    const float* sn1 = P.snaps + ((size_t)(t - 1 - P.snap_t0) * P.units + ul) * g.inner;
-    const int q0 = (zi0 + r0) * g.ni + xi0 + lx;
+    const int g0 = (zi0 + 2 + r0) * g.nx + xi0 + 2 + lx;  // its grid cell
+    const float* sn0 = P.snaps + ((size_t)(t - P.snap_t0) * P.units + ul) * P.sinner;
+    const float* sn1 = P.snaps + ((size_t)(t - 1 - P.snap_t0) * P.units + ul) * P.sinner;
+    const int q0 = (zi0 + r0) * P.sni + xi0 + lx;
     float pl[CPT], pp[CPT], ps0[CPT], ps1[CPT], pg[CPT], pv[CPT];
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
-        const int i = i0 + k * RZ * nx, q = q0 + k * RZ * g.ni;
+        const int i = i0 + k * RZ * nx, q = q0 + k * RZ * P.sni;
         pl[k] = Lcur[i];
         pp[k] = Lprev[i];
         ps0[k] = sn0[q];
@@ -1336,7 +1344,7 @@ __global__ __launch_bounds__(NT) void k_adj_step2_in(StepParams P) {
     for (int k = 0; k < CPT; ++k) {
         step_a(lx, r0 + RZ * k, pl[k], pp[k], pv[k], lt1[k], Vo[k]);
         float gg = pl[k] * ps0[k];
-        if (src) src_grad(i0 + k * RZ * nx, pv[k], pl[k], t, gg);
+        if (src) src_grad(g0 + k * RZ * g.nx, pv[k], pl[k], t, gg);
         gk[k] = gg;
     }
     float rV[RPT];
@@ -1365,7 +1373,7 @@ __global__ __launch_bounds__(NT) void k_adj_step2_in(StepParams P) {
         const float next = w + 2.0f * l1 - pl[k];
         const int i = i0 + k * RZ * nx;
         float gb = l1 * ps1[k];
-        if (src) src_grad(i, pv[k], l1, t - 1, gb);
+        if (src) src_grad(g0 + k * RZ * g.nx, pv[k], l1, t - 1, gb);
         float gacc = pg[k];
         gacc += gk[k];
         gacc += gb;
@@ -1411,7 +1419,7 @@ __global__ __launch_bounds__(NT) void k_adj_step2(StepParams P) {
     const int unit = P.unit0 + ul;
     int b, s;
     unit_model(P, unit, b, s);
-    const size_t cells = (size_t)g.nx * g.nz;
+    const size_t cells = P.sc;
     const int t = P.t, h = P.half;
     auto arr = [&](int a) { return P.state + ((size_t)a * P.units + ul) * cells; };
     auto lam = [&](int k) { return arr(k < 2 ? k : k + 8); };
@@ -1456,7 +1464,7 @@ __global__ __launch_bounds__(NT) void k_adj_step2(StepParams P) {
     float sw[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-        scell[c] = P.geo.src_cell[s * 4 + c];
+        scell[c] = P.geo.src_cell[s * 4 + c];  // grid cell z * nx + x
         sw[c] = P.geo.src_w[s * 4 + c];
     }
     float pl[CPT], pp[CPT], ps0[CPT], ps1[CPT], pg[CPT];
@@ -1466,12 +1474,12 @@ __global__ __launch_bounds__(NT) void k_adj_step2(StepParams P) {
         pl[k] = pp[k] = ps0[k] = ps1[k] = pg[k] = 0.0f;
         if (inner_cell(lx, lz)) {
             const int x = xi0 + 2 + lx, z = zi0 + 2 + lz;
-            const int i = z * g.nx + x;
-            const size_t qq = (size_t)(z - 2) * g.ni + (x - 2);
+            const int i = z * P.sp + x;
+            const size_t qq = (size_t)(z - 2) * P.sni + (x - 2);
             pl[k] = Lcur[i];
             pp[k] = Lprev[i];
-            ps0[k] = P.snaps[((size_t)(t - P.snap_t0) * P.units + ul) * g.inner + qq];
-            ps1[k] = P.snaps[((size_t)(t - 1 - P.snap_t0) * P.units + ul) * g.inner + qq];
+            ps0[k] = P.snaps[((size_t)(t - P.snap_t0) * P.units + ul) * P.sinner + qq];
+            ps1[k] = P.snaps[((size_t)(t - 1 - P.snap_t0) * P.units + ul) * P.sinner + qq];
             pg[k] = grad[i];
         }
     }
@@ -1498,7 +1506,7 @@ __global__ __launch_bounds__(NT) void k_adj_step2(StepParams P) {
         rrz[r] = rz;
         rl[r] = rp[r] = 0.0f;
         if (rz != -100 && inner_cell(rx, rz)) {
-            const int i = (zi0 + 2 + rz) * g.nx + (xi0 + 2 + rx);
+            const int i = (zi0 + 2 + rz) * P.sp + (xi0 + 2 + rx);
             rl[r] = Lcur[i];
             rp[r] = Lprev[i];
         }
@@ -1516,7 +1524,7 @@ __global__ __launch_bounds__(NT) void k_adj_step2(StepParams P) {
             tl[k] = tv[k] = 0.0f;
             if (j < AZ * AX && x >= 0 && z >= 0 && x < g.nx && z < g.nz) {
                 tv[k] = vel_at(vvis, g, x, z);
-                tl[k] = Lcur[z * g.nx + x];
+                tl[k] = Lcur[z * P.sp + x];
             }
         }
 #pragma unroll
@@ -1538,7 +1546,7 @@ __global__ __launch_bounds__(NT) void k_adj_step2(StepParams P) {
             const int x = xi0 - 2 + hx, z = zi0 + hz;
             tb[k] = td[k] = tz[k] = tp[k] = 0.0f;
             if (j < SAX && x >= 0 && x < g.nx && z < g.nz) {
-                const int i = z * g.nx + x;
+                const int i = z * P.sp + x;
                 tb[k] = pbx[x];
                 td[k] = pdbx[x];
                 tz[k] = ZXr[i];
@@ -1571,7 +1579,7 @@ __global__ __launch_bounds__(NT) void k_adj_step2(StepParams P) {
             const int x = xi0 + hx, z = zi0 - 2 + hz;
             tb[k] = td[k] = tz[k] = tp[k] = 0.0f;
             if (j < SAZ && z >= 0 && x < g.nx && z < g.nz) {
-                const int i = z * g.nx + x;
+                const int i = z * P.sp + x;
                 tb[k] = pbz[z];
                 td[k] = pdbz[z];
                 tz[k] = ZZr[i];
@@ -1605,7 +1613,7 @@ __global__ __launch_bounds__(NT) void k_adj_step2(StepParams P) {
         l1 = V1 = px = zx = pz = zz = 0.0f;
         const int x = xi0 + 2 + rx, z = zi0 + 2 + rz;
         if (!inner_cell(rx, rz)) return false;
-        const int i = z * g.nx + x;
+        const int i = z * P.sp + x;
         const int li = (rz + 4) * AX + (rx + 4);
         const float v0 = sV[li];
         float w;  // x term, then += z term (0 + x skipped: equal up to the sign of zero)
@@ -1652,11 +1660,10 @@ __global__ __launch_bounds__(NT) void k_adj_step2(StepParams P) {
         gk[k] = 0.0f;
         if (in[k]) {
             const int x = xi0 + 2 + lx, z = zi0 + 2 + lz;
-            const int i = z * g.nx + x;
             float gg = pl[k] * ps0[k];
 #pragma unroll
             for (int c = 0; c < 4; ++c)
-                if (scell[c] == i) {
+                if (scell[c] == z * g.nx + x) {
                     const float v = vel_at(vvis, g, x, z);
                     gg += pl[k] * (-2.0f * v * g.dt2 * sw[c] * f_a);
                 }
@@ -1715,7 +1722,7 @@ __global__ __launch_bounds__(NT) void k_adj_step2(StepParams P) {
         if (!in[k]) continue;
         const int lz = tid / TX + RZ * k;
         const int x = xi0 + 2 + lx, z = zi0 + 2 + lz;
-        const int i = z * g.nx + x;
+        const int i = z * P.sp + x;
         const int li = (lz + 2) * BX + (lx + 2);
         const float v0 = sV[li];
         float w;
@@ -1750,7 +1757,7 @@ __global__ __launch_bounds__(NT) void k_adj_step2(StepParams P) {
         }
 #pragma unroll
         for (int c = 0; c < 4; ++c)
-            if (scell[c] == i) {
+            if (scell[c] == z * g.nx + x) {
                 const float v = vel_at(vvis, g, x, z);
                 gb += l1 * (-2.0f * v * g.dt2 * sw[c] * f_b);
             }
@@ -1910,22 +1917,26 @@ hipError_t launch_adj_step2(const StepParams& p, int units, hipStream_t st) {
 }
 
 namespace st {
-__global__ void k_pad_velocity(Grid g, const float* vel, int b0, int nb, float* vpad) {
+__global__ void k_pad_velocity(Grid g, const float* vel, int b0, int nb, float* vpad, CellLayout lay) {
     const size_t cells = (size_t)g.nx * g.nz, n = cells * nb;
     for (size_t j = blockIdx.x * (size_t)blockDim.x + threadIdx.x; j < n; j += (size_t)gridDim.x * blockDim.x) {
         const int m = (int)(j / cells);
         const int i = (int)(j - (size_t)m * cells);
         const int z = i / g.nx, x = i - z * g.nx;
-        vpad[j] = vel_at(vel + (size_t)(b0 + m) * g.nvx * g.nvz, g, x, z);
+        vpad[(size_t)m * lay.ustride + lay.off + (size_t)z * lay.pitch + x] =
+            vel_at(vel + (size_t)(b0 + m) * g.nvx * g.nvz, g, x, z);
     }
 }
 }  // namespace st
 
-hipError_t launch_pad_velocity(const Grid& g, const float* vel, int b0, int nb, float* vpad, hipStream_t st) {
-    if (nb < 1) return hipErrorInvalidValue;
+hipError_t launch_pad_velocity(const Grid& g, const float* vel, int b0, int nb, float* vpad, const CellLayout& lay,
+                               hipStream_t st) {
+    if (nb < 1 || lay.pitch < g.nx || lay.off < 0 ||
+        lay.ustride < (size_t)lay.off + (size_t)lay.pitch * (g.nz - 1) + g.nx)
+        return hipErrorInvalidValue;
     const size_t n = (size_t)g.nx * g.nz * nb;
     const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
-    hipLaunchKernelGGL(st::k_pad_velocity, dim3(blocks), dim3(256), 0, st, g, vel, b0, nb, vpad);
+    hipLaunchKernelGGL(st::k_pad_velocity, dim3(blocks), dim3(256), 0, st, g, vel, b0, nb, vpad, lay);
     return hipGetLastError();
 }
 

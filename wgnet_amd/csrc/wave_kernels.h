@@ -94,14 +94,14 @@ struct StepParams {
     const float* wavelet;    // [units or 1, nt]
     long long wav_stride;
     float* traces;           // forward: [all units, nt, nrec] or null (no recording)
-    float* snaps;            // [seg_len, units, inner]: written (forward STORE) or read (adjoint)
+    float* snaps;            // [seg_len, units] of sinner floats: written (forward STORE) or read (adjoint)
     int snap_t0, seg_len;    // snapshot of step t is entry t - snap_t0
     float* state;
     int units, unit0, t;
     const float* dvel;       // Born
     const float* born_snaps; // Born: base snapshots, same layout as snaps
     const float* resid;      // adjoint: [all units, nt, nrec]
-    float* grad;             // adjoint: [all units, nx * nz], accumulated
+    float* grad;             // adjoint: [all units] in the storage layout below, accumulated
     float* dwav;             // adjoint: [all units, nt] or null
     int tile;                // tile shape: 0 64x16, 1 128x8, 2 32x32, 3 64x32
     // adjoint state buffers: lambda buffers 0..3 (lambda^t, lambda^{t+1}, and
@@ -136,12 +136,32 @@ struct StepParams {
     // m_bgx, m_bw: multipliers of bgx and of the side columns per middle row.
     int bnd_n, bgx, bgz;
     unsigned m_bgx, m_bw;
+    // Storage layout of the per-unit arrays (state, gradient, vpad): padded
+    // cell (x, z) of unit u at u * sc + z * sp + x from the pointers above,
+    // which include the offset of cell (0, 0) within a unit's block.  On
+    // wide grids the row pitch sp is a multiple of 32 floats and that offset
+    // puts the first inner column at a 128-byte boundary, so that the tiles'
+    // rows are whole cache lines.  Snapshots: inner cell (x, z) of unit u
+    // and step t at (t * units + u) * sinner + (z - 2) * sni + (x - 2).
+    int sp;
+    size_t sc;
+    int sni;
+    size_t sinner;
 };
-// vpad[m, z, x] = clamp(vel[b0 + m] at the clamped model cell of padded (x, z))
-hipError_t launch_pad_velocity(const Grid& g, const float* vel, int b0, int nb, float* vpad, hipStream_t st);
+// Layout of per-unit padded-grid arrays: cell (x, z) of unit u at
+// u * ustride + off + z * pitch + x.  plain_layout(g) is (nx, 0, nx * nz).
+struct CellLayout {
+    int pitch, off;
+    size_t ustride;
+};
+inline CellLayout plain_layout(const Grid& g) { return CellLayout{g.nx, 0, (size_t)g.nx * g.nz}; }
+// vpad[m, z, x] = clamp(vel[b0 + m] at the clamped model cell of padded (x, z)),
+// in layout `lay`
+hipError_t launch_pad_velocity(const Grid& g, const float* vel, int b0, int nb, float* vpad, const CellLayout& lay,
+                               hipStream_t st);
 constexpr int kStreamTiles = 4;
 
-// Arrays of the streamed forward state, each [units, nx * nz]: wavefield
+// Arrays of the streamed forward state, each [units] in the storage layout: wavefield
 // buffers 0..3, psi_x halves (2, 3), psi_z halves (4, 5), zeta_x and zeta_z
 // halves.  Single steps use wavefield buffers 0 and 1, the psi halves by
 // turns and one zeta half: arrays 0..7.
@@ -286,8 +306,12 @@ hipError_t launch_residual(const float* pred, const float* obs,
 hipError_t launch_loss_sum(const float* partials, long long chunks, int B,
                            const double* energy, float* loss, hipStream_t st);
 // grad_vis[b] = fold(mask * sum_s grad_unit[b*S+s]); NaN where energy == 0.
+// grad_unit in layout `lay` (the plain layout unless given).
 hipError_t launch_fold(const Grid& g, const float* grad_unit, const float* vel,
                        const double* energy, int B, float* grad_vis,
                        hipStream_t st);
+hipError_t launch_fold(const Grid& g, const float* grad_unit, const float* vel,
+                       const double* energy, int B, float* grad_vis,
+                       const CellLayout& lay, hipStream_t st);
 
 }  // namespace wg
