@@ -140,7 +140,7 @@ struct wg_engine {
     // storage layout of the streamed per-unit arrays (wg::StepParams::sp ...)
     // and of the snapshots; plain (nx, 0, nx * nz) / (ni, inner) otherwise
     wg::CellLayout slay{};
-    int s_ni = 0;
+    int s_ni = 0, s_soff = 0;  // snapshot row pitch; offset of inner cell (0, 0) in a snapshot block
     size_t s_inner = 0;
     hipStream_t stream = nullptr;  // host-pointer calls
     hipStream_t side = nullptr;    // second stream of the overlapped chunk pipeline
@@ -533,18 +533,17 @@ constexpr int kCkptArrays = 6;         // a checkpoint: u^t, u^{t-1}, psi_x, psi
 constexpr int kMaxStepUnits = 65535;   // units of one step launch (grid z dimension)
 
 // Storage layout of the streamed arrays.  On grids at least kAlignMinNx
-// cells wide the row pitch is rounded up to 32 floats and the arrays start
-// 30 floats into a unit's block, so that padded column 2 (the first inner
-// column, where the tiles' columns start) and every row lie on 128-byte
-// boundaries: the tiles' loads and stores are then whole cache lines (the
-// lean 1000 x 300 step: 160 -> 131 us in a probe).  Snapshot rows are
-// rounded to 32 floats likewise.  Row pitches and per-unit strides are odd
-// multiples of 32 floats: the XCDs stream different units at once, and
-// strides with a larger power of two put their accesses on the same HBM
-// channels.  Narrower grids keep the plain layout (the rounding would cost
-// up to a fifth more memory there).
+// cells wide the row pitch is rounded up to 32 floats and the arrays start a
+// few floats into a unit's block, so that the first column of the lean
+// rectangle (padded column region_adj + 2, where the lean tiles' columns
+// start) and every row lie on 128-byte boundaries: the lean tiles' loads and
+// stores are then whole cache lines (the lean 1000 x 300 step: 160 -> 131 us
+// in a probe).  Snapshot rows are rounded to 32 floats and offset likewise.
+// Row pitches and per-unit strides are odd multiples of 32 floats: the XCDs
+// stream different units at once, and strides with a larger power of two put
+// their accesses on the same HBM channels.  Narrower grids keep the plain
+// layout (the rounding would cost up to a fifth more memory there).
 constexpr int kAlignMinNx = 256;
-constexpr int kStreamColOff = 30;
 
 // The smallest n' >= n that is an odd multiple of 32.
 size_t odd32(size_t n) {
@@ -558,11 +557,14 @@ void set_stream_layout(wg_engine* e) {
     e->slay = wg::plain_layout(g);
     e->s_ni = g.ni;
     e->s_inner = (size_t)g.inner;
+    e->s_soff = 0;
     if (!e->streamed || g.nx < kAlignMinNx) return;
-    const int pitch = (int)odd32((size_t)kStreamColOff + g.nx);
-    e->slay = wg::CellLayout{pitch, kStreamColOff, odd32((size_t)pitch * g.nz)};
+    const int off = (32 - (g.region_adj + 2) % 32) % 32;  // padded column region_adj + 2 at a multiple of 32
+    const int pitch = (int)odd32((size_t)off + g.nx);
+    e->slay = wg::CellLayout{pitch, off, odd32((size_t)pitch * g.nz)};
+    e->s_soff = (32 - g.region_adj % 32) % 32;  // inner column region_adj likewise
     e->s_ni = (int)odd32((size_t)g.ni);
-    e->s_inner = odd32((size_t)e->s_ni * g.nzi);
+    e->s_inner = odd32((size_t)e->s_soff + (size_t)e->s_ni * g.nzi);
 }
 
 StreamPlan plan_stream(wg_engine* e, int B, bool need_snaps) {
@@ -717,7 +719,7 @@ int stream_forward(wg_engine* e, const StreamPlan& P, int units, int unit0, cons
     p.wavelet = wav;
     p.wav_stride = wav_stride;
     p.traces = traces;
-    p.snaps = (float*)e->snaps.p;
+    p.snaps = (float*)e->snaps.p + e->s_soff;
     p.snap_t0 = 0;
     p.seg_len = P.seg_len;
     p.state = (float*)e->fstate.p + e->slay.off;
@@ -746,7 +748,7 @@ int stream_adjoint(wg_engine* e, const StreamPlan& P, int units, int unit0, cons
     LAUNCH(wg::launch_zero((float*)e->astate.p, 12 * units * cells, st));
     LAUNCH(wg::launch_zero(gunit + (size_t)unit0 * cells, units * cells, st));
     wg::StepParams pf = step_params(e, vel, units, unit0);
-    pf.snaps = (float*)e->snaps.p;
+    pf.snaps = (float*)e->snaps.p + e->s_soff;
     pf.seg_len = P.seg_len;
     pf.state = (float*)e->fstate.p + e->slay.off;
     wg::StepParams pa = step_params(e, vel, units, unit0);
@@ -754,7 +756,7 @@ int stream_adjoint(wg_engine* e, const StreamPlan& P, int units, int unit0, cons
     if ((rc0 = stream_pad_velocity(e, vel, units, unit0, st, &pf))) return rc0;
     pa.vpad = pf.vpad;
     pa.vpad_b0 = pf.vpad_b0;
-    pa.snaps = (float*)e->snaps.p;
+    pa.snaps = (float*)e->snaps.p + e->s_soff;
     pa.seg_len = P.seg_len;
     pa.state = (float*)e->astate.p + e->slay.off;
     pa.resid = resid;
@@ -1727,7 +1729,7 @@ int32_t wg_born_dev(wg_engine* e, int32_t B, const float* vel_dev, const float* 
             p.seg_len = P.seg_len;
             p.state = (float*)e->astate.p + e->slay.off;
             p.dvel = dvel_dev;
-            p.born_snaps = (const float*)e->snaps.p;
+            p.born_snaps = (const float*)e->snaps.p + e->s_soff;
             for (int t = 0; t < g.nt; ++t) {  // single steps: wavefield buffers 0 and 1, zeta half 0
                 p.t = t;
                 p.u_cur = t & 1;

@@ -14,6 +14,12 @@
 // 2-cell halo.  Tiles with no C-PML cell along an axis skip that axis's strips
 // and memory variables (they are zero there).  Per-cell arithmetic is
 // stencil.h's, in the reference's order.
+//
+// Most launches are framed: a lean kernel (no memory variables, no per-cell
+// C-PML or bounds tests, the clamped velocity from a padded array) runs the
+// whole tiles of a rectangle that starts region_adj cells in from each side,
+// and a general kernel runs the four bands around it, each band tile
+// clipped to its band.  On 1000 x 300 the rectangle holds 78 % of the cells.
 #include <algorithm>
 
 #include "stencil.h"
@@ -82,25 +88,42 @@ __device__ __forceinline__ int sidx(const StepParams& P, int cell) {
     return cell + z * (P.sp - P.g.nx);
 }
 
-// Tile of a general two-step launch over the tiles outside the lean kernels'
-// rectangle: boundary tile i of a unit is, in order, a tile of the rows above
-// the rectangle, of the rows below it, or of the side columns of its rows.
-__device__ __forceinline__ TileId tile_2step(const StepParams& P) {
-    TileId t = tile_id(P);
-    if (P.bnd_n <= 0) return t;
-    const int gx = P.bgx, i = t.x;
-    const int top = P.in_z0 * gx, bot = (P.bgz - P.in_z1) * gx, iw = P.in_x1 - P.in_x0;
+// The cells a workgroup of a general launch stores: the tile whose first
+// cell is inner cell (xi0, zi0), clipped to inner columns [cx0, cx1) and rows
+// [cz0, cz1).  Unframed (bnd_n <= 0): tile (x, z) of the full tiling, clipped
+// to the inner grid.  Framed: tile i of the four bands around the lean
+// rectangle (rows above it, rows below it, columns left and right of it),
+// clipped to its band, so that every cell is stored by one workgroup.
+struct TileAt {
+    int xi0, zi0, unit;
+    int cx0, cx1, cz0, cz1;
+};
+
+template <int TX, int TZ>
+__device__ __forceinline__ TileAt tile_frame(const StepParams& P) {
+    const TileId t = tile_id(P);
+    const Grid& g = P.g;
+    if (P.bnd_n <= 0) return TileAt{t.x * TX, t.z * TZ, t.unit, 0, g.ni, 0, g.nzi};
+    const int lx1 = P.lx0 + P.nlx * TX, lz1 = P.lz0 + P.nlz * TZ;
+    const int top = P.gxa * P.gzt, bot = P.gxa * P.gzb, left = P.gxl * P.nlz;
+    int i = t.x;
     if (i < top) {
-        const int z = (int)udiv((unsigned)i, (unsigned)gx, P.m_bgx);
-        return TileId{i - z * gx, z, t.unit};
+        const int z = (int)udiv((unsigned)i, (unsigned)P.gxa, P.m_gxa);
+        return TileAt{(i - z * P.gxa) * TX, z * TZ, t.unit, 0, g.ni, 0, P.lz0};
     }
-    if (i < top + bot) {
-        const int j = i - top, z = (int)udiv((unsigned)j, (unsigned)gx, P.m_bgx);
-        return TileId{j - z * gx, P.in_z1 + z, t.unit};
+    i -= top;
+    if (i < bot) {
+        const int z = (int)udiv((unsigned)i, (unsigned)P.gxa, P.m_gxa);
+        return TileAt{(i - z * P.gxa) * TX, lz1 + z * TZ, t.unit, 0, g.ni, lz1, g.nzi};
     }
-    const int w = gx - iw, j = i - top - bot, z = (int)udiv((unsigned)j, (unsigned)w, P.m_bw);
-    const int c = j - z * w;
-    return TileId{c < P.in_x0 ? c : c + iw, P.in_z0 + z, t.unit};
+    i -= bot;
+    if (i < left) {
+        const int z = (int)udiv((unsigned)i, (unsigned)P.gxl, P.m_gxl);
+        return TileAt{(i - z * P.gxl) * TX, P.lz0 + z * TZ, t.unit, 0, P.lx0, P.lz0, lz1};
+    }
+    i -= left;
+    const int z = (int)udiv((unsigned)i, (unsigned)P.gxr, P.m_gxr);
+    return TileAt{lx1 + (i - z * P.gxr) * TX, P.lz0 + z * TZ, t.unit, lx1, g.ni, P.lz0, lz1};
 }
 
 #define WG_TILE_CONSTANTS                                              \
@@ -183,9 +206,11 @@ __device__ __forceinline__ bool tile_has_source(const StepParams& P, int s, int 
     return box_has_source(P, s, xi0 + 2, xi0 + TX + 2, zi0 + 2, zi0 + TZ + 2, nonzero_only);
 }
 
+// src: a source corner of slot s lies in the tile (injected after the
+// update, lib.rs:209-215, with wavelet sample f_t).
 template <class TL, bool STORE>
 __device__ __forceinline__ void fwd_interior(const StepParams& P, float* sU, int ul, int unit, int b, int xi0,
-                                             int zi0) {
+                                             int zi0, bool src = false, int s = 0, float f_t = 0.0f) {
     WG_TILE_CONSTANTS
     const Grid& g = P.g;
     const size_t cells = P.sc;
@@ -217,7 +242,16 @@ __device__ __forceinline__ void fwd_interior(const StepParams& P, float* sU, int
                                             dummy, dummy);
         const float v = pv[k];
         if (STORE) snap[q0 + k * RZ * P.sni] = 2.0f * v * g.dt2 * w;
-        Unxt[i0 + k * RZ * nx] = v * v * g.dt2 * w + 2.0f * u - pu[k];
+        float next = v * v * g.dt2 * w + 2.0f * u - pu[k];
+        if (src) {
+            const int gi = (zi0 + lz + 2) * g.nx + xo;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const float wc = P.geo.src_w[s * 4 + c];
+                if (P.geo.src_cell[s * 4 + c] == gi && wc != 0.0f) next += -(v * v * g.dt2) * wc * f_t;
+            }
+        }
+        Unxt[i0 + k * RZ * nx] = next;
     }
 }
 
@@ -299,6 +333,25 @@ __device__ __forceinline__ void adj_interior(const StepParams& P, float* sV, int
 }
 
 // --------------------------------------------------------------------------
+// Forward step on the tiles of the lean rectangle (a framed launch, beside
+// k_fwd_step over the bands around it).
+// --------------------------------------------------------------------------
+template <class TL, bool STORE>
+__global__ __launch_bounds__(NT) void k_fwd_step_in(StepParams P) {
+    WG_TILE_CONSTANTS
+    __shared__ float sU[HZ * HX];
+    const TileId tl = tile_id(P);
+    const int unit = P.unit0 + tl.unit;
+    int b, s;
+    unit_model(P, unit, b, s);
+    if (!unit_active(P.geo, unit)) return;
+    const int xi0 = P.lx0 + tl.x * TX, zi0 = P.lz0 + tl.z * TZ;
+    const bool src = tile_has_source(P, s, xi0, zi0, TX, TZ, true);
+    const float f_t = P.wavelet[(size_t)unit * P.wav_stride + P.t];
+    fwd_interior<TL, STORE>(P, sU, tl.unit, unit, b, xi0, zi0, src, s, f_t);
+}
+
+// --------------------------------------------------------------------------
 // Forward step (lib.rs:145-220 for one t, every unit of the chunk).
 // --------------------------------------------------------------------------
 template <class TL, bool STORE, bool BORN>
@@ -308,7 +361,7 @@ __global__ __launch_bounds__(NT) void k_fwd_step(StepParams P) {
     __shared__ float sAX[TZ * HX];
     __shared__ float sAZ[HZ * TX];
     const Grid& g = P.g;
-    const TileId tile = tile_id(P);
+    const TileAt tile = tile_frame<TX, TZ>(P);
     const int ul = tile.unit;              // unit within the chunk
     const int unit = P.unit0 + ul;
     int b, s;
@@ -325,6 +378,8 @@ __global__ __launch_bounds__(NT) void k_fwd_step(StepParams P) {
     float* ZX = arr(fwd_zx_arr(P.ze_half));
     float* ZZ = arr(fwd_zz_arr(P.ze_half));
     const float* vvis = P.vel + (size_t)b * g.nvx * g.nvz;
+    // the clamped velocity on the padded grid, or null (then clamped from vvis)
+    const float* vpg = P.vpad ? P.vpad + (size_t)(b - P.vpad_b0) * cells : nullptr;
     const float* pax = P.geo.prof;
     const float* pbx = pax + g.nx;
     const float* pdbx = pbx + g.nx;
@@ -332,15 +387,16 @@ __global__ __launch_bounds__(NT) void k_fwd_step(StepParams P) {
     const float* pbz = paz + g.nz;
     const float* pdbz = pbz + g.nz;
     const int tid = threadIdx.x;
-    const int xi0 = tile.x * TX, zi0 = tile.z * TZ;  // tile origin, inner coords
+    const int xi0 = tile.xi0, zi0 = tile.zi0;  // tile origin, inner coords
+    const bool rec0 = xi0 == 0 && zi0 == 0;  // the workgroup recording the gathers
 
     if (!unit_active(P.geo, unit)) {  // padding unit of an uneven batch
-        if (P.traces && tile.x == 0 && tile.z == 0)
+        if (P.traces && rec0)
             for (int r = tid; r < g.nrec; r += NT) P.traces[((size_t)unit * g.nt + t) * g.nrec + r] = 0.0f;
         return;
     }
     // Receivers record u^t before the update (lib.rs:146-149): one tile per unit.
-    if (P.traces && tile.x == 0 && tile.z == 0) {
+    if (P.traces && rec0) {
         float* tr = P.traces + ((size_t)unit * g.nt + t) * g.nrec;
         for (int r = tid; r < g.nrec; r += NT) {
             float acc = 0.0f;
@@ -356,7 +412,7 @@ __global__ __launch_bounds__(NT) void k_fwd_step(StepParams P) {
         // (the C-PML region is wider than the 2-cell halo, so such a tile and
         // its halo lie inside the grid; tile (0, 0), which records the
         // gathers, is in the region)
-        if (P.vpad && !tpx && !tpz && !tile_has_source(P, s, xi0, zi0, TX, TZ, true)) {
+        if (P.bnd_n <= 0 && P.vpad && !tpx && !tpz && !tile_has_source(P, s, xi0, zi0, TX, TZ, true)) {
             fwd_interior<TL, STORE>(P, sU, ul, unit, b, xi0, zi0);
             return;
         }
@@ -388,7 +444,8 @@ __global__ __launch_bounds__(NT) void k_fwd_step(StepParams P) {
         if (xo < g.nx - 2 && z < g.nz - 2) {
             const int i = z * P.sp + xo;
             pu[k] = Unxt[i];
-            pv[k] = vis_raw(vvis, g, xo, z);
+            if constexpr (BORN) pv[k] = vis_raw(vvis, g, xo, z);  // (the Born mask needs the raw value)
+            else pv[k] = vpg ? vpg[i] : vis_raw(vvis, g, xo, z);
             if (pxo) {
                 ppx[k] = PXc[i];
                 pzx[k] = ZX[i];
@@ -446,7 +503,8 @@ __global__ __launch_bounds__(NT) void k_fwd_step(StepParams P) {
     for (int k = 0; k < CPT; ++k) {
         const int lz = tid / TX + RZ * k;
         const int x = xo, z = zi0 + lz + 2;
-        if (x >= g.nx - 2 || z >= g.nz - 2) continue;
+        // (the clip lies inside the inner grid)
+        if (x - 2 < tile.cx0 || x - 2 >= tile.cx1 || z - 2 < tile.cz0 || z - 2 >= tile.cz1) continue;
         const int i = z * P.sp + x;
         const int li = (lz + 2) * HX + (lx + 2);
         const float u = sU[li];
@@ -534,7 +592,7 @@ __global__ __launch_bounds__(NT) void k_fwd_step2(StepParams P) {
     __shared__ float sAX[BZ * AX];
     __shared__ float sAZ[AZ * BX];
     const Grid& g = P.g;
-    const TileId tile = tile_2step(P);
+    const TileAt tile = tile_frame<TX, TZ>(P);
     const int ul = tile.unit;
     const int unit = P.unit0 + ul;
     int b, s;
@@ -555,6 +613,8 @@ __global__ __launch_bounds__(NT) void k_fwd_step2(StepParams P) {
     const float* ZZr = arr(fwd_zz_arr(P.ze_half));
     float* ZZw = arr(fwd_zz_arr(1 - P.ze_half));
     const float* vvis = P.vel + (size_t)b * g.nvx * g.nvz;
+    // the clamped velocity on the padded grid, or null (then clamped from vvis)
+    const float* vpg = P.vpad ? P.vpad + (size_t)(b - P.vpad_b0) * cells : nullptr;
     const float* pax = P.geo.prof;
     const float* pbx = pax + g.nx;
     const float* pdbx = pbx + g.nx;
@@ -562,10 +622,11 @@ __global__ __launch_bounds__(NT) void k_fwd_step2(StepParams P) {
     const float* pbz = paz + g.nz;
     const float* pdbz = pbz + g.nz;
     const int tid = threadIdx.x;
-    const int xi0 = tile.x * TX, zi0 = tile.z * TZ;  // tile origin; cell (rx, rz) is padded (xi0+2+rx, zi0+2+rz)
+    const int xi0 = tile.xi0, zi0 = tile.zi0;  // tile origin; cell (rx, rz) is padded (xi0+2+rx, zi0+2+rz)
+    const bool rec0 = xi0 == 0 && zi0 == 0;    // the workgroup recording the gathers
 
     if (!unit_active(P.geo, unit)) {  // padding unit of an uneven batch
-        if (P.traces && tile.x == 0 && tile.z == 0)
+        if (P.traces && rec0)
             for (int r = tid; r < g.nrec; r += NT) {
                 P.traces[((size_t)unit * g.nt + t) * g.nrec + r] = 0.0f;
                 if (P.trace_prev) P.traces[((size_t)unit * g.nt + t - 1) * g.nrec + r] = 0.0f;
@@ -573,7 +634,7 @@ __global__ __launch_bounds__(NT) void k_fwd_step2(StepParams P) {
         return;
     }
     // Receivers record u^t (and u^{t-1}) before the update (lib.rs:146-149).
-    if (P.traces && tile.x == 0 && tile.z == 0) {
+    if (P.traces && rec0) {
         for (int r = tid; r < g.nrec; r += NT) {
             float acc = 0.0f;
 #pragma unroll
@@ -590,6 +651,12 @@ __global__ __launch_bounds__(NT) void k_fwd_step2(StepParams P) {
     auto inner_cell = [&](int rx, int rz) {
         const int x = xi0 + 2 + rx, z = zi0 + 2 + rz;
         return x >= 2 && z >= 2 && x < g.nx - 2 && z < g.nz - 2;
+    };
+    // the tile cells this workgroup stores (step A still advances all of
+    // them, as step B reads them)
+    auto own = [&](int rx, int rz) {
+        const int x = xi0 + rx, z = zi0 + rz;
+        return x >= tile.cx0 && x < tile.cx1 && z >= tile.cz0 && z < tile.cz1;
     };
     auto pml_x = [&](int x) { return x < g.region || x >= g.nx - g.region; };
     auto pml_z = [&](int z) { return z < g.region || z >= g.nz - g.region; };
@@ -614,7 +681,7 @@ __global__ __launch_bounds__(NT) void k_fwd_step2(StepParams P) {
             const int x = xi0 + 2 + lx, z = zi0 + 2 + lz;
             const int i = z * P.sp + x;
             pu[k] = Uprev[i];
-            pv[k] = vis_raw(vvis, g, x, z);
+            pv[k] = vpg ? vpg[i] : vis_raw(vvis, g, x, z);
             if (pml_x(x)) {
                 ppx[k] = PXr[i];
                 pzx[k] = ZXr[i];
@@ -651,7 +718,7 @@ __global__ __launch_bounds__(NT) void k_fwd_step2(StepParams P) {
             const int x = xi0 + 2 + rx, z = zi0 + 2 + rz;
             const int i = z * P.sp + x;
             ru[r] = Uprev[i];
-            rv[r] = vis_raw(vvis, g, x, z);
+            rv[r] = vpg ? vpg[i] : vis_raw(vvis, g, x, z);
             if (pml_x(x)) {
                 rpx[r] = PXr[i];
                 rzx[r] = ZXr[i];
@@ -755,7 +822,8 @@ __global__ __launch_bounds__(NT) void k_fwd_step2(StepParams P) {
     for (int k = 0; k < CPT; ++k) {
         const int lz = tid / TX + RZ * k;
         ut[k] = un[k] = 0.0f;
-        if (inner_cell(lx, lz)) un[k] = step_a(lx, lz, pu[k], pv[k], ppx[k], pzx[k], ppz[k], pzz[k], ut[k], true);
+        if (inner_cell(lx, lz))
+            un[k] = step_a(lx, lz, pu[k], pv[k], ppx[k], pzx[k], ppz[k], pzz[k], ut[k], own(lx, lz));
     }
     float rn[RPT];
 #pragma unroll
@@ -798,7 +866,7 @@ __global__ __launch_bounds__(NT) void k_fwd_step2(StepParams P) {
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
         const int lz = tid / TX + RZ * k;
-        if (!inner_cell(lx, lz)) continue;
+        if (!own(lx, lz)) continue;  // (the clip lies inside the inner grid)
         const int x = xo, z = zi0 + 2 + lz;
         const int i = z * P.sp + x;
         const int li = (lz + 2) * BX + (lx + 2);
@@ -894,6 +962,8 @@ __global__ __launch_bounds__(NT) void k_adj_step(StepParams P) {
     float* ZZw = arr(9 - q);
     float* grad = P.grad + (size_t)unit * cells;
     const float* vvis = P.vel + (size_t)b * g.nvx * g.nvz;
+    // the clamped velocity on the padded grid, or null (then clamped from vvis)
+    const float* vpg = P.vpad ? P.vpad + (size_t)(b - P.vpad_b0) * cells : nullptr;
     const float* pax = P.geo.prof;
     const float* pbx = pax + g.nx;
     const float* pdbx = pbx + g.nx;
@@ -976,7 +1046,7 @@ __global__ __launch_bounds__(NT) void k_adj_step(StepParams P) {
             const int x = xi0 + (j % HX), z = zi0 + (j / HX);
             tl[k] = tv[k] = 0.0f;
             if (j < HZ * HX && x < g.nx && z < g.nz) {
-                tv[k] = vel_at(vvis, g, x, z);
+                tv[k] = (vpg ? vpg[z * P.sp + x] : vel_at(vvis, g, x, z));
                 tl[k] = Lcur[z * P.sp + x];
             }
         }
@@ -1055,7 +1125,7 @@ __global__ __launch_bounds__(NT) void k_adj_step(StepParams P) {
 #pragma unroll
         for (int c = 0; c < 4; ++c)
             if (scell[c] == z * g.nx + x) {
-                const float v = vel_at(vvis, g, x, z);
+                const float v = (vpg ? vpg[z * P.sp + x] : vel_at(vvis, g, x, z));
                 gk += lam * (-2.0f * v * g.dt2 * sw[c] * f_t);
             }
         grad[i] = pg[k] + gk;
@@ -1065,11 +1135,12 @@ __global__ __launch_bounds__(NT) void k_adj_step(StepParams P) {
 
 
 // --------------------------------------------------------------------------
-// Two forward steps on the tiles whose cells and 2-cell ring lie outside the
-// C-PML region and hold no source corner (the gathers are recorded by tile
-// (0, 0), which lies in the region): k_fwd_step2's scheme with no memory
-// variables, strips, sources or bounds tests, and the clamped velocity from
-// P.vpad; launched beside k_fwd_step2, which skips these tiles.
+// Two forward steps on the tiles of the lean rectangle, whose cells and
+// 2-cell ring lie outside the C-PML region (the gathers are recorded by the
+// band tile at the grid's corner): k_fwd_step2's scheme with no memory
+// variables, strips or bounds tests, sources tested only in a tile that
+// holds one, and the clamped velocity from P.vpad; launched beside
+// k_fwd_step2 over the bands around the rectangle.
 // --------------------------------------------------------------------------
 template <class TL, bool STORE>
 __global__ __launch_bounds__(NT) void k_fwd_step2_in(StepParams P) {
@@ -1086,7 +1157,7 @@ __global__ __launch_bounds__(NT) void k_fwd_step2_in(StepParams P) {
     int b, s;
     unit_model(P, unit, b, s);
     if (!unit_active(P.geo, unit)) return;
-    const int xi0 = (P.in_x0 + tl.x) * TX, zi0 = (P.in_z0 + tl.z) * TZ;
+    const int xi0 = P.lx0 + tl.x * TX, zi0 = P.lz0 + tl.z * TZ;
     // a source corner in the tile or its ring: the injection after each
     // update (lib.rs:209-215), tested per cell only in such a tile
     const bool src = box_has_source(P, s, xi0, xi0 + TX + 4, zi0, zi0 + TZ + 4, true);
@@ -1219,12 +1290,12 @@ __global__ __launch_bounds__(NT) void k_fwd_step2_in(StepParams P) {
 }
 
 // --------------------------------------------------------------------------
-// Two adjoint steps on the tiles whose cells and 2-cell ring lie outside the
-// C-PML region and the receiver rows and hold no source corner (the bulk of
-// a large grid; P.in_x0 .. P.in_z1, launched beside k_adj_step2, which skips
-// them): k_adj_step2's scheme with no memory variables, strips, receivers or
-// bounds tests, and the clamped velocity from P.vpad.  Same arithmetic in the
-// same order, so the two kernels together equal two single steps bit for bit.
+// Two adjoint steps on the tiles of the lean rectangle, whose cells and
+// 2-cell ring lie outside the C-PML region and the receiver rows (the bulk
+// of a large grid; launched beside k_adj_step2 over the bands around it):
+// k_adj_step2's scheme with no memory variables, strips, receivers or bounds
+// tests, and the clamped velocity from P.vpad.  Same arithmetic in the same
+// order, so the two kernels together equal two single steps bit for bit.
 // --------------------------------------------------------------------------
 template <class TL>
 __global__ __launch_bounds__(NT) void k_adj_step2_in(StepParams P) {
@@ -1241,7 +1312,7 @@ __global__ __launch_bounds__(NT) void k_adj_step2_in(StepParams P) {
     int b, s;
     unit_model(P, unit, b, s);
     if (!unit_active(P.geo, unit)) return;
-    const int xi0 = (P.in_x0 + tl.x) * TX, zi0 = (P.in_z0 + tl.z) * TZ;
+    const int xi0 = P.lx0 + tl.x * TX, zi0 = P.lz0 + tl.z * TZ;
     // a source corner in the tile: its gradient term (wgsl:245-249), tested
     // per cell only in such a tile
     const bool src = tile_has_source(P, s, xi0, zi0, TX, TZ, false);
@@ -1414,7 +1485,7 @@ __global__ __launch_bounds__(NT) void k_adj_step2(StepParams P) {
     __shared__ float sV[AZ * AX];  // V^t on tile + 4 (pitch AX), then V^{t-1} on tile + 2 (pitch BX)
     __shared__ float sS[2 * SAX + 2 * SAZ];
     const Grid& g = P.g;
-    const TileId tile = tile_2step(P);
+    const TileAt tile = tile_frame<TX, TZ>(P);
     const int ul = tile.unit;
     const int unit = P.unit0 + ul;
     int b, s;
@@ -1437,6 +1508,8 @@ __global__ __launch_bounds__(NT) void k_adj_step2(StepParams P) {
     float* ZZw = arr(9 - h);
     float* grad = P.grad + (size_t)unit * cells;
     const float* vvis = P.vel + (size_t)b * g.nvx * g.nvz;
+    // the clamped velocity on the padded grid, or null (then clamped from vvis)
+    const float* vpg = P.vpad ? P.vpad + (size_t)(b - P.vpad_b0) * cells : nullptr;
     const float* pax = P.geo.prof;
     const float* pbx = pax + g.nx;
     const float* pdbx = pbx + g.nx;
@@ -1444,7 +1517,7 @@ __global__ __launch_bounds__(NT) void k_adj_step2(StepParams P) {
     const float* pbz = paz + g.nz;
     const float* pdbz = pbz + g.nz;
     const int tid = threadIdx.x;
-    const int xi0 = tile.x * TX, zi0 = tile.z * TZ;  // tile origin; cell (rx, rz) is padded (xi0+2+rx, zi0+2+rz)
+    const int xi0 = tile.xi0, zi0 = tile.zi0;  // tile origin; cell (rx, rz) is padded (xi0+2+rx, zi0+2+rz)
     const int ra = g.region_adj;
     const float rh = g.rh;
 
@@ -1523,7 +1596,7 @@ __global__ __launch_bounds__(NT) void k_adj_step2(StepParams P) {
             const int x = xi0 - 2 + (j % AX), z = zi0 - 2 + (j / AX);
             tl[k] = tv[k] = 0.0f;
             if (j < AZ * AX && x >= 0 && z >= 0 && x < g.nx && z < g.nz) {
-                tv[k] = vel_at(vvis, g, x, z);
+                tv[k] = (vpg ? vpg[z * P.sp + x] : vel_at(vvis, g, x, z));
                 tl[k] = Lcur[z * P.sp + x];
             }
         }
@@ -1645,7 +1718,7 @@ __global__ __launch_bounds__(NT) void k_adj_step2(StepParams P) {
             for (int e = P.geo.band_off[o]; e < P.geo.band_off[o + 1]; ++e) next += P.geo.ent_w[e] * res_a[P.geo.ent_r[e]];
         }
         l1 = next;
-        const float v = vel_at(vvis, g, x, z);
+        const float v = (vpg ? vpg[z * P.sp + x] : vel_at(vvis, g, x, z));
         V1 = v * v * g.dt2 * next;
         return true;
     };
@@ -1664,7 +1737,7 @@ __global__ __launch_bounds__(NT) void k_adj_step2(StepParams P) {
 #pragma unroll
             for (int c = 0; c < 4; ++c)
                 if (scell[c] == z * g.nx + x) {
-                    const float v = vel_at(vvis, g, x, z);
+                    const float v = (vpg ? vpg[z * P.sp + x] : vel_at(vvis, g, x, z));
                     gg += pl[k] * (-2.0f * v * g.dt2 * sw[c] * f_a);
                 }
             gk[k] = gg;
@@ -1719,9 +1792,10 @@ __global__ __launch_bounds__(NT) void k_adj_step2(StepParams P) {
     const float f_b = P.wavelet[(size_t)unit * P.wav_stride + (t - 1)];
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
-        if (!in[k]) continue;
         const int lz = tid / TX + RZ * k;
         const int x = xi0 + 2 + lx, z = zi0 + 2 + lz;
+        // the cells this workgroup stores (the clip lies inside the inner grid)
+        if (!in[k] || x - 2 < tile.cx0 || x - 2 >= tile.cx1 || z - 2 < tile.cz0 || z - 2 >= tile.cz1) continue;
         const int i = z * P.sp + x;
         const int li = (lz + 2) * BX + (lx + 2);
         const float v0 = sV[li];
@@ -1758,7 +1832,7 @@ __global__ __launch_bounds__(NT) void k_adj_step2(StepParams P) {
 #pragma unroll
         for (int c = 0; c < 4; ++c)
             if (scell[c] == z * g.nx + x) {
-                const float v = vel_at(vvis, g, x, z);
+                const float v = (vpg ? vpg[z * P.sp + x] : vel_at(vvis, g, x, z));
                 gb += l1 * (-2.0f * v * g.dt2 * sw[c] * f_b);
             }
         float gacc = pg[k];
@@ -1781,9 +1855,11 @@ unsigned div_magic(unsigned long long d, unsigned long long n_end) {
     return (unsigned)(((1ull << 32) + d - 1) / d);
 }
 
-StepParams ordered(const StepParams& p, const dim3& grid) {
+// xcd_tiles: tiles a unit of the grid's full tiling, which decides the
+// XCD-aware order (0: those of this launch).
+StepParams ordered(const StepParams& p, const dim3& grid, unsigned xcd_tiles = 0) {
     StepParams q = p;
-    q.xcd_order = grid.x * grid.y >= st::kXcdMinTiles ? 1 : 0;
+    q.xcd_order = (xcd_tiles ? xcd_tiles : grid.x * grid.y) >= st::kXcdMinTiles ? 1 : 0;
     const unsigned long long tiles = (unsigned long long)grid.x * grid.y * grid.z;
     q.m_gx = div_magic(grid.x, tiles);
     q.m_gxy = div_magic((unsigned long long)grid.x * grid.y, tiles);
@@ -1792,71 +1868,80 @@ StepParams ordered(const StepParams& p, const dim3& grid) {
     return q;
 }
 
-// Tile ranges [x0, x1) x [z0, z1) whose cells and 2-cell ring (padded
-// columns [i TX, i TX + TX + 4), rows likewise) lie outside the adjoint C-PML
-// region, when the receiver rows stay outside the region's inner rows; empty
-// without the padded velocity.
-void interior_tiles(const StepParams& p, int TX, int TZ, bool adjoint, int* x0, int* x1, int* z0, int* z1) {
-    *x0 = *x1 = *z0 = *z1 = 0;
-    const Grid& g = p.g;
-    const int ra = adjoint ? g.region_adj : g.region;
-    if (!p.vpad) return;
-    // (the adjoint injects the residual at the receiver rows; the forward
-    // records the gathers in tile (0, 0), which lies in the region)
-    if (adjoint && p.geo.band_rows > 0 && p.geo.band_z0 < g.nz - ra && p.geo.band_z0 + p.geo.band_rows > ra) return;
-    auto range = [&](int n, int T, int* lo, int* hi) {
-        *lo = (ra + T - 1) / T;                        // i T >= ra
-        const int top = n - ra - T - 4;                // i T <= top
-        *hi = top < 0 ? 0 : top / T + 1;
-        if (*hi < *lo) *hi = *lo;
-    };
-    range(g.nx, TX, x0, x1);
-    range(g.nz, TZ, z0, z1);
-    if (*x1 <= *x0 || *z1 <= *z0) *x0 = *x1 = *z0 = *z1 = 0;
+// The lean rectangle of a framed launch (StepParams::lx0 ...): whole TX x TZ
+// tiles from inner cell (ra, ra), ra the adjoint C-PML region, so that the
+// tiles' cells and 2-cell ring (padded [ra, ra + n T + 4)) lie outside every
+// step's C-PML region; empty without the padded velocity, when no tile fits,
+// or when the receiver rows (the adjoint's residual injection) reach into
+// it.  The bands around it get their tile counts and multipliers.  Returns
+// whether the launch is framed.
+bool frame(StepParams& q, int TX, int TZ) {
+    const Grid& g = q.g;
+    const int ra = g.region_adj;
+    q.lx0 = q.lz0 = ra;
+    q.nlx = (g.nx - 2 * ra - 4) / TX;
+    q.nlz = (g.nz - 2 * ra - 4) / TZ;
+    q.bnd_n = 0;
+    const int lz1 = ra + q.nlz * TZ;  // inner rows [ra, lz1): padded [ra + 2, lz1 + 2)
+    // (the lean adjoint also advances the 2-cell ring: padded rows [ra, lz1 + 4))
+    const bool rec_in = q.geo.band_rows > 0 && q.geo.band_z0 < lz1 + 4 && q.geo.band_z0 + q.geo.band_rows > ra;
+    if (!q.vpad || q.nlx <= 0 || q.nlz <= 0 || rec_in) {
+        q.nlx = q.nlz = 0;
+        return false;
+    }
+    const int lx1 = ra + q.nlx * TX;
+    q.gxa = (g.ni + TX - 1) / TX;
+    q.gzt = (ra + TZ - 1) / TZ;
+    q.gzb = (g.nzi - lz1 + TZ - 1) / TZ;
+    q.gxl = (ra + TX - 1) / TX;
+    q.gxr = (g.ni - lx1 + TX - 1) / TX;
+    q.bnd_n = q.gxa * (q.gzt + q.gzb) + (q.gxl + q.gxr) * q.nlz;
+    const unsigned long long n = (unsigned long long)q.bnd_n;
+    q.m_gxa = div_magic(q.gxa, n);
+    q.m_gxl = div_magic(q.gxl, n);
+    q.m_gxr = div_magic(q.gxr, n);
+    return true;
 }
 
+// A framed launch: the lean kernel `lean` over the rectangle frame() gives
+// and the general kernel `full` over the bands around it, or `full` over
+// every tile when the rectangle is empty.
 template <class T>
-hipError_t fwd_step_t(const StepParams& p, int units, bool store, bool born, hipStream_t st) {
-    const dim3 grid((p.g.ni + T::TX - 1) / T::TX, (p.g.nzi + T::TZ - 1) / T::TZ, units);
-    void (*fn)(StepParams) =
-        born ? st::k_fwd_step<T, false, true> : (store ? st::k_fwd_step<T, true, false> : st::k_fwd_step<T, false, false>);
-    hipLaunchKernelGGL(fn, grid, dim3(st::NT), 0, st, ordered(p, grid));
-    return hipGetLastError();
-}
-
-// A two-step launch: the lean kernel `lean` over the rectangle of tiles
-// interior_tiles() gives, and the general kernel `full` over the other
-// tiles (all of them when the rectangle is empty).
-template <class T>
-hipError_t two_step(const StepParams& p, int units, bool adjoint, void (*lean)(StepParams), void (*full)(StepParams),
-                    hipStream_t st) {
+hipError_t framed(const StepParams& p, int units, void (*lean)(StepParams), void (*full)(StepParams),
+                  hipStream_t st) {
     StepParams q = p;
-    const int gx = (p.g.ni + T::TX - 1) / T::TX, gz = (p.g.nzi + T::TZ - 1) / T::TZ;
-    interior_tiles(p, T::TX, T::TZ, adjoint, &q.in_x0, &q.in_x1, &q.in_z0, &q.in_z1);
-    if (q.in_x1 <= q.in_x0) {
+    const unsigned gx = (p.g.ni + T::TX - 1) / T::TX, gz = (p.g.nzi + T::TZ - 1) / T::TZ;
+    if (!frame(q, T::TX, T::TZ)) {
         const dim3 grid(gx, gz, units);
         hipLaunchKernelGGL(full, grid, dim3(st::NT), 0, st, ordered(q, grid));
         return hipGetLastError();
     }
-    const dim3 gin(q.in_x1 - q.in_x0, q.in_z1 - q.in_z0, units);
-    hipLaunchKernelGGL(lean, gin, dim3(st::NT), 0, st, ordered(q, gin));
+    const dim3 gin(q.nlx, q.nlz, units);
+    hipLaunchKernelGGL(lean, gin, dim3(st::NT), 0, st, ordered(q, gin, gx * gz));
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    const int iw = q.in_x1 - q.in_x0;
-    q.bnd_n = gx * gz - iw * (q.in_z1 - q.in_z0);
-    q.bgx = gx;
-    q.bgz = gz;
-    q.m_bgx = div_magic(gx, (unsigned long long)gx * gz);
-    q.m_bw = div_magic(gx - iw, (unsigned long long)gx * gz);
     const dim3 gb(q.bnd_n, 1, units);
-    hipLaunchKernelGGL(full, gb, dim3(st::NT), 0, st, ordered(q, gb));
+    hipLaunchKernelGGL(full, gb, dim3(st::NT), 0, st, ordered(q, gb, gx * gz));
+    return hipGetLastError();
+}
+
+template <class T>
+hipError_t fwd_step_t(const StepParams& p, int units, bool store, bool born, hipStream_t st) {
+    if (!born)
+        return framed<T>(p, units, store ? st::k_fwd_step_in<T, true> : st::k_fwd_step_in<T, false>,
+                         store ? st::k_fwd_step<T, true, false> : st::k_fwd_step<T, false, false>, st);
+    const dim3 grid((p.g.ni + T::TX - 1) / T::TX, (p.g.nzi + T::TZ - 1) / T::TZ, units);
+    StepParams q = p;
+    q.bnd_n = 0;
+    void (*fn)(StepParams) = st::k_fwd_step<T, false, true>;
+    hipLaunchKernelGGL(fn, grid, dim3(st::NT), 0, st, ordered(q, grid));
     return hipGetLastError();
 }
 
 template <class T>
 hipError_t fwd_step2_t(const StepParams& p, int units, bool store, hipStream_t st) {
-    return two_step<T>(p, units, false, store ? st::k_fwd_step2_in<T, true> : st::k_fwd_step2_in<T, false>,
-                       store ? st::k_fwd_step2<T, true> : st::k_fwd_step2<T, false>, st);
+    return framed<T>(p, units, store ? st::k_fwd_step2_in<T, true> : st::k_fwd_step2_in<T, false>,
+                     store ? st::k_fwd_step2<T, true> : st::k_fwd_step2<T, false>, st);
 }
 
 template <class T>
@@ -1869,7 +1954,7 @@ hipError_t adj_step_t(const StepParams& p, int units, hipStream_t st) {
 
 template <class T>
 hipError_t adj_step2_t(const StepParams& p, int units, hipStream_t st) {
-    return two_step<T>(p, units, true, st::k_adj_step2_in<T>, st::k_adj_step2<T>, st);
+    return framed<T>(p, units, st::k_adj_step2_in<T>, st::k_adj_step2<T>, st);
 }
 
 using T0 = st::Tile<64, 16>;

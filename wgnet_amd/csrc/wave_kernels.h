@@ -126,16 +126,17 @@ struct StepParams {
     // tiles of a unit, the shots and nx (0: divide); n / d = mulhi(n, m)
     // exactly for n < 2^32 / d, which the launchers check.
     unsigned m_gx, m_gxy, m_shots, m_nx;
-    // Two-step launches: the tiles [in_x0, in_x1) x [in_z0, in_z1) (tile
-    // indices) whose cells and 2-cell ring lie outside the C-PML region (and,
-    // for the adjoint, the receiver rows) are run by the lean two-step kernel;
-    // empty when in_x1 <= in_x0.  Set by the launcher.
-    int in_x0, in_x1, in_z0, in_z1;
-    // With bnd_n > 0 a general two-step launch covers only the other tiles of
-    // the full bgx x bgz tiling, bnd_n a unit, numbered by boundary_tile();
-    // m_bgx, m_bw: multipliers of bgx and of the side columns per middle row.
-    int bnd_n, bgx, bgz;
-    unsigned m_bgx, m_bw;
+    // Framed launches (set by the launchers): a lean kernel covers the
+    // rectangle of nlx x nlz tiles whose first cell is inner cell (lx0, lz0),
+    // whose cells and 2-cell ring lie outside the adjoint C-PML region and
+    // the receiver rows; with bnd_n > 0 a general kernel covers the rest of
+    // the grid, bnd_n tiles a unit in four bands: the rows above the
+    // rectangle (gxa x gzt tiles), the rows below it (gxa x gzb), then the
+    // columns left (gxl x nlz) and right (gxr x nlz) of it, each tile
+    // storing only its band's cells.  m_*: multipliers of gxa, gxl, gxr.
+    int lx0, lz0, nlx, nlz;
+    int bnd_n, gxa, gzt, gzb, gxl, gxr;
+    unsigned m_gxa, m_gxl, m_gxr;
     // Storage layout of the per-unit arrays (state, gradient, vpad): padded
     // cell (x, z) of unit u at u * sc + z * sp + x from the pointers above,
     // which include the offset of cell (0, 0) within a unit's block.  On
