@@ -72,8 +72,9 @@ def test_phys_stage_equals_per_model_reference_loop():
         loss_r = (field_r - true).abs().mean()
         new_r, misfit_r, scale_r, losses_r, grads_r, obs_r = _reference_term(ref, loss_r, field_r, true, nb, 0.5, dev)
         (gw_r,) = torch.autograd.grad(new_r, w)
-        print(f"step {step} band {band}: loss {float(new):.9g} scale {float(parts['phys_scale']):.6g}", flush=True)
-        assert float(new) == float(new_r)
+        print(f"step {step} band {band}: loss {float(new.detach()):.9g} scale {float(parts['phys_scale']):.6g}",
+              flush=True)
+        assert float(new.detach()) == float(new_r.detach())
         assert float(parts["phys_scale"]) == scale_r
         assert torch.equal(gw, gw_r)
         assert abs(float(parts["phys_misfit"]) - misfit_r) <= 1e-6 * abs(misfit_r)

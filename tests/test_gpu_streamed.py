@@ -224,3 +224,44 @@ def test_streamed_xcd_tile_order_matches_oracle(orc):
     assert np.array_equal(E.simulate(true.reshape(-1)), obs)
     loss3, grad3 = E.evaluate_adjoint(cur.reshape(-1), obs.reshape(-1))
     assert loss3 == loss and np.array_equal(grad3, grad)
+
+
+def test_streamed_sources_inside_lean_rectangle_match_oracle(orc):
+    """Sources deep inside the lean rectangle of the framed launches (a
+    300 x 100 grid, 344 x 144 padded, wide enough for the aligned row
+    layout): the lean kernels inject the source and add its gradient term
+    themselves.  Gathers, loss and gradient equal the oracle; two steps per
+    launch equal single steps bit for bit."""
+    from wgnet_amd.wave_gauss_rs import PyWaveEngine
+
+    nt, nvx, nvz = 400, 300, 100
+    E = PyWaveEngine(2, nt, 0.0008, 15.0, 20, 1000.0, 5000.0, nvx=nvx, nvz=nvz)
+    assert E.streamed
+    E.set_band_split(False)
+    # cells (150, 12) and about (234.6, 15.3): inner cells (170, 32) and
+    # (254.6, 35.3), inside the rectangle from inner cell (26, 26)
+    E.set_sources(np.array([[1500.0, 120.0], [2345.6, 153.3]], np.float32))
+    true = layered(nvz, nvx, 7)
+    cur = true.copy()
+    cur[8:20, 120:240] += 150.0
+    obs = E.simulate(true.reshape(-1))
+    for s, (x, z) in enumerate(E.sources()):
+        g = orc.Geometry(nvx=nvx, nvz=nvz, nt=nt, dt=0.0008, src_x=float(x), src_z=float(z))
+        err = relerr(obs[s], orc.forward_f32(g, true, E.wavelet()))
+        print(f"300x100 deep shot {s}: gather rel-L2 {err:.3e}", flush=True)
+        assert err <= 1e-5
+    loss, grad = E.evaluate_adjoint(cur.reshape(-1), obs.reshape(-1))
+    rl, rg = orc.evaluate_adjoint_f32(orc.Geometry(nvx=nvx, nvz=nvz, nt=nt, dt=0.0008), E.sources(), cur,
+                                      E.wavelet(), obs)
+    print(f"300x100 deep: loss {loss:.9e} vs {rl:.9e}; grad rel-L2 {relerr(grad, rg):.3e}", flush=True)
+    assert rl > 0.0 and np.linalg.norm(rg) > 0.0
+    assert abs(loss - rl) <= 1e-6 * rl
+    assert relerr(grad, rg) <= 1e-6
+    E.set_time_block(1)
+    loss1, grad1 = E.evaluate_adjoint(cur.reshape(-1), obs.reshape(-1))
+    assert loss1 == loss and np.array_equal(grad1, grad)
+    E.set_time_block(2)
+    E.set_forward_time_block(2)
+    assert np.array_equal(E.simulate(true.reshape(-1)), obs)
+    loss2, grad2 = E.evaluate_adjoint(cur.reshape(-1), obs.reshape(-1))
+    assert loss2 == loss and np.array_equal(grad2, grad)
