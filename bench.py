@@ -204,7 +204,7 @@ def main() -> int:
                     "1 128x8, 2 32x32, 3 64x32")
     ap.add_argument("--time-block", type=int, choices=(1, 2), default=2,
                     help="large workload: adjoint steps per streamed launch")
-    ap.add_argument("--fwd-time-block", type=int, choices=(1, 2), default=1,
+    ap.add_argument("--fwd-time-block", type=int, choices=(1, 2), default=2,
                     help="large workload: forward steps per streamed launch")
     ap.add_argument("--time-split", type=int, default=4,
                     help="default workload: time parts per unit of the resident work queue (0: chunk pipeline)")

@@ -295,7 +295,7 @@ int32_t wg_set_stream_tile(wg_engine *e, int32_t tile);
  * single steps. */
 int32_t wg_set_time_block(wg_engine *e, int32_t steps);
 /* Temporal blocking of the HBM-streamed forward (no reference counterpart):
- * time steps per launch, 1 (default) or 2.  A two-step launch recomputes a
+ * time steps per launch, 1 or 2 (default).  A two-step launch recomputes a
  * 2-cell ring around each tile and reads the wavefield, velocity and memory
  * variables and writes them once per two steps (snapshots every step).
  * Gathers, snapshots, checkpoints and everything computed from them are

@@ -165,6 +165,7 @@ def test_time_blocked_forward_is_bit_identical(nvx, nvz, nt, budget):
     E = PyWaveEngine(3, nt, 0.001, 15.0, 20, 1000.0, 5000.0, nvx=nvx, nvz=nvz)
     assert E.streamed
     E.set_band_split(False)
+    E.set_forward_time_block(1)  # the reference: single forward steps
     if budget:
         E.set_snapshot_budget(budget)
     w = (nvx - 1) * 10.0
@@ -198,6 +199,8 @@ def test_streamed_xcd_tile_order_matches_oracle(orc):
     nt, nvx, nvz = 200, 600, 200
     E = PyWaveEngine(2, nt, 0.0008, 15.0, 20, 1000.0, 5000.0, nvx=nvx, nvz=nvz)
     assert E.streamed
+    E.set_forward_time_block(1)
+    E.set_time_block(1)
     nx, nz, _ = E.dims()
     assert ((nx - 4 + 63) // 64) * ((nz - 4 + 15) // 16) >= 128
     E.set_sources(np.array([[1000.0, 10.0], [4321.0, 10.0]], np.float32))
@@ -238,6 +241,7 @@ def test_streamed_sources_inside_lean_rectangle_match_oracle(orc):
     E = PyWaveEngine(2, nt, 0.0008, 15.0, 20, 1000.0, 5000.0, nvx=nvx, nvz=nvz)
     assert E.streamed
     E.set_band_split(False)
+    E.set_forward_time_block(1)
     # cells (150, 12) and about (234.6, 15.3): inner cells (170, 32) and
     # (254.6, 35.3), inside the rectangle from inner cell (26, 26)
     E.set_sources(np.array([[1500.0, 120.0], [2345.6, 153.3]], np.float32))

@@ -148,7 +148,7 @@ struct wg_engine {
     int cus = 256;                 // compute units of the device
     bool overlap = true;           // overlap chunk forwards with the previous chunk's adjoint
     int time_block = 2;            // adjoint steps per streamed launch (1 or 2)
-    int fwd_block = 1;             // forward steps per streamed launch (1 or 2)
+    int fwd_block = 2;             // forward steps per streamed launch (1 or 2)
     int qparts = 4;                // time parts per unit of the resident work queues (0: chunk pipeline)
     int qspin = 1 << 24;           // polls before a work-queue wait gives up
     // row bands: grids too large for one workgroup whose units split into

@@ -134,7 +134,7 @@ class PyWaveEngine:
         _lib.check(self._lib.wg_set_time_block(self._h, int(steps)))
 
     def set_forward_time_block(self, steps: int) -> None:
-        """Forward time steps per HBM-streamed launch (1, the default, or 2);
+        """Forward time steps per HBM-streamed launch (1, or 2, the default);
         results are identical either way."""
         _lib.check(self._lib.wg_set_forward_time_block(self._h, int(steps)))
 
