@@ -132,10 +132,12 @@ struct StepParams {
     // the receiver rows; with bnd_n > 0 a general kernel covers the rest of
     // the grid, bnd_n tiles a unit in four bands: the rows above the
     // rectangle (gxa x gzt tiles), the rows below it (gxa x gzb), then the
-    // columns left (gxl x nlz) and right (gxr x nlz) of it, each tile
+    // columns left (gxl x gzs) and right (gxr x gzs) of it, each tile
     // storing only its band's cells.  m_*: multipliers of gxa, gxl, gxr.
-    int lx0, lz0, nlx, nlz;
-    int bnd_n, gxa, gzt, gzb, gxl, gxr;
+    // The band kernel's tiles may differ from the lean kernel's: [lx0, lx1)
+    // x [lz0, lz1) is the rectangle, the side bands are gzs tile rows.
+    int lx0, lz0, nlx, nlz, lx1, lz1;
+    int bnd_n, gxa, gzt, gzb, gxl, gxr, gzs;
     unsigned m_gxa, m_gxl, m_gxr;
     // Storage layout of the per-unit arrays (state, gradient, vpad): padded
     // cell (x, z) of unit u at u * sc + z * sp + x from the pointers above,
