@@ -156,6 +156,8 @@ struct wg_engine {
     // exchanged every step) instead of the per-step HBM-streamed kernels
     int bands = 0;                 // bands per unit (0: the grid does not split)
     int band_bounds[wg::kMaxBands + 1] = {};
+    int rt_bands = 0;              // the runtime-geometry kernels' split (0: none fits)
+    int rt_bounds[wg::kMaxBands + 1] = {};
     int band_on = 1;               // 0: streamed kernels, 1: band kernels (compiled where the grid has them), 2: runtime-geometry band kernels
     DevBuf bxbuf[2], bqflags[2];   // per band-launch slot (one per stream in flight): exchange words; [next, err]
     DevBuf mtable, mnext, mslots;  // one-launch batches led by this engine: group table, [counter, slot flags..., err], snapshot slots
@@ -414,14 +416,29 @@ int small_bands(const wg_engine* e, int units, int bounds[wg::kMaxBands + 1]) {
 }
 
 // Buffers and counters of a band launch over `units` units in split (P,
-// bounds) (P = 0: the engine's split), in buffer slot `slot` (band launches
-// that may run at the same time use different slots): exchange words
-// (zeroed: no step has tag 0) and [next, err].  Clears err when `clear_err`.
+// bounds) (P = 0: the engine's split for that many units), in buffer slot
+// `slot` (band launches that may run at the same time use different slots):
+// exchange words (zeroed: no step has tag 0) and [next, err].  Clears err
+// when `clear_err`.  The split of a call depends only on its unit count, so a
+// chunk's forward and adjoint run the same one (the results do not depend on
+// the split).
 int band_setup(wg_engine* e, int units, bool clear_err, hipStream_t st, wg::BandLaunch* bl, int P = 0,
                const int* bounds = nullptr, int slot = 0) {
+    int call_bounds[wg::kMaxBands + 1];
     if (P == 0) {
         P = e->bands;
         bounds = e->band_bounds;
+        wg::Geometry gh{};
+        gh.band_z0 = e->band_z0;
+        gh.band_rows = e->band_rows;
+        int q = 0;
+        if (e->band_on == 2 && e->rt_bands > 1) {
+            P = e->rt_bands;
+            bounds = e->rt_bounds;
+        } else if (e->band_on == 1 && wg::band_split_call_fx(e->g, gh, units, e->cus, &q, call_bounds)) {
+            P = q;
+            bounds = call_bounds;
+        }
     }
     DevBuf& xb = e->bxbuf[slot];
     DevBuf& qf = e->bqflags[slot];
@@ -1123,8 +1140,13 @@ int32_t wg_engine_create(const wg_config* cfg, wg_engine** out) {
         gh.band_rows = e->band_rows;
         gh.n_ent = (int)e->ent_r.size();
         int P = 0;
-        if (wg::band_split_fx(g, gh, &P, e->band_bounds) || wg::band_plan(g, gh, lds_limit, &P, e->band_bounds))
+        if (wg::band_plan(g, gh, lds_limit, &P, e->rt_bounds)) e->rt_bands = P;
+        if (wg::band_split_fx(g, gh, &P, e->band_bounds)) {
             e->bands = P;
+        } else if (e->rt_bands > 1) {
+            e->bands = e->rt_bands;
+            for (int q = 0; q <= e->rt_bands; ++q) e->band_bounds[q] = e->rt_bounds[q];
+        }
     }
     e->prof.resize(3 * (g.nx + g.nz));
     wg_cpml_profiles(g.nx, c.h, c.dt, c.pml, c.vmax, c.peak_hz, e->prof.data(), e->prof.data() + g.nx,
@@ -1568,7 +1590,7 @@ int32_t wg_loss_grad_groups_dev(int32_t n, const wg_group_args* ga, void* stream
     }
     // longest tasks first: step count times the kind's forward + adjoint step
     // time (us, measured: 70 x 70 whole unit, 100 x 100 band, 141 x 141 band)
-    const double step_us[3] = {17.6, 14.9, 13.6};
+    const double step_us[3] = {17.6, 14.9, 15.2};
     std::vector<int> idx;
     for (int i = 0; i < n; ++i)
         if (ga[i].B > 0) idx.push_back(i);

@@ -256,6 +256,10 @@ size_t adjoint_lds_bytes_fx(const Grid& g, const Geometry& geo);
 // (false for another grid, or receivers outside its top band); the launchers
 // return false when the grid or the split b is not a compiled one.
 bool band_split_fx(const Grid& g, const Geometry& geo, int* P, int bounds[kMaxBands + 1]);
+// The compiled split of one call over `units` units: band_split_fx's, or a
+// split into more bands where that call ends sooner on `cus` compute units.
+bool band_split_call_fx(const Grid& g, const Geometry& geo, int units, int cus, int* P,
+                        int bounds[kMaxBands + 1]);
 // The 70 x 70 model (pml 20) as 4 or 2 bands when `units` units of that many
 // bands fit `cus` compute units (one band workgroup per compute unit): the
 // split, or false (another grid, or too many units).
