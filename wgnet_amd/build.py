@@ -25,10 +25,13 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libwgnet_hip.so")
 SOURCES = ["wave_kernels.hip", "wave_kernels_fixed.hip", "wave_kernels_fixed_fwd.hip", "wave_kernels_band.hip",
-           "stream_kernels.hip", "misfit_kernels.hip", "precond.hip", "engine.hip"]
+           "wave_kernels_fused.hip", "stream_kernels.hip", "misfit_kernels.hip", "precond.hip", "engine.hip"]
 # Per-source flags: the resident 70 x 70 forward bodies run faster with the
-# machine scheduler's occupancy-first strategy (the adjoint's spill with it).
-SOURCE_FLAGS = {"wave_kernels_fixed_fwd.hip": ["-mllvm", "--amdgpu-sched-strategy=iterative-maxocc"]}
+# machine scheduler's occupancy-first strategy (the adjoint's spill with it),
+# the one-launch uneven batch with the max-ilp strategy (the band kernels of
+# small batches are slower with it).
+SOURCE_FLAGS = {"wave_kernels_fixed_fwd.hip": ["-mllvm", "--amdgpu-sched-strategy=iterative-maxocc"],
+                "wave_kernels_fused.hip": ["-mllvm", "--amdgpu-sched-strategy=max-ilp"]}
 ARCH = os.environ.get("WGNET_OFFLOAD_ARCH", "gfx950")
 FLAGS = [
     "-O3",
