@@ -259,7 +259,9 @@ int32_t wg_engine_streamed(const wg_engine *e);
  * neighbour band per step through global memory.  wg_engine_bands returns
  * the bands per unit (0: the grid does not split, or fits one workgroup).
  * The 100 x 100 and 141 x 141 grids (pml 20) have band kernels compiled for
- * their geometry and split (2 and 4 bands); other grids run runtime-geometry
+ * their geometry and split (2 and 3 bands; a 141 x 141 call runs 4 where its
+ * unit count fills the compute units in fewer rounds that way, and
+ * wg_engine_bands returns the default 3); other grids run runtime-geometry
  * band kernels.  Results equal the HBM-streamed kernels' bit for bit either
  * way.  The band kernels serve simulate and the loss and gradient when a
  * model's snapshots fit the snapshot budget; the Born operator, the wavelet
