@@ -200,8 +200,8 @@ def main() -> int:
     ap.add_argument("--jumps", type=int, default=1)
     ap.add_argument("--lanes", type=int, default=0, help="uneven workload: streams the groups run on "
                     "(0: one per group)")
-    ap.add_argument("--tile", type=int, default=0, help="streamed tile shape (large workload): 0 64x16, "
-                    "1 128x8, 2 32x32, 3 64x32")
+    ap.add_argument("--tile", type=int, default=0, help="streamed tile shape (large workload): 0 64x16 "
+                    "(64x32 lean tiles in the two-step forward), 1 128x8, 2 32x32, 3 64x32")
     ap.add_argument("--time-block", type=int, choices=(1, 2), default=2,
                     help="large workload: adjoint steps per streamed launch")
     ap.add_argument("--fwd-time-block", type=int, choices=(1, 2), default=2,

@@ -277,9 +277,10 @@ int32_t wg_engine_streamed(const wg_engine *e);
 int32_t wg_engine_bands(const wg_engine *e);
 int32_t wg_set_band_split(wg_engine *e, int32_t on);
 /* Tile shapes of the HBM-streamed kernels (no reference counterpart): a
- * 256-thread workgroup computes 64x16 (0, default), 128x8 (1), 32x32 (2) or
- * 64x32 (3) cells.  Results do not depend on the shape; an autotuner times
- * them per grid and sets the fastest. */
+ * 256-thread workgroup computes 64x16 (0, default; the two-step forward's
+ * lean interior tiles are 64x32 in it), 128x8 (1), 32x32 (2) or 64x32 (3)
+ * cells.  Results do not depend on the shape; an autotuner times them per
+ * grid and sets the fastest. */
 int32_t wg_stream_tiles(void);
 /* Overlapped chunk pipeline of the resident loss/gradient (no reference
  * counterpart; default on): a batch larger than one unit per compute unit is

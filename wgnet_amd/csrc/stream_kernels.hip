@@ -1918,13 +1918,13 @@ bool frame(StepParams& q, int TX, int TZ, int TXb, int TZb) {
 // A framed launch: the lean kernel `lean` over the rectangle frame() gives
 // and the general kernel `full` over the bands around it, or `full` over
 // every tile when the rectangle is empty.
-template <class T, class TB>
+template <class T, class TB, class TF = T>
 hipError_t framed(const StepParams& p, int units, void (*lean)(StepParams), void (*full)(StepParams),
                   void (*full_b)(StepParams), hipStream_t st) {
     StepParams q = p;
     const unsigned gx = (p.g.ni + T::TX - 1) / T::TX, gz = (p.g.nzi + T::TZ - 1) / T::TZ;
-    if (!frame(q, T::TX, T::TZ, TB::TX, TB::TZ)) {
-        const dim3 grid(gx, gz, units);
+    if (!frame(q, T::TX, T::TZ, TB::TX, TB::TZ)) {  // unframed: the general kernel on TF tiles
+        const dim3 grid((p.g.ni + TF::TX - 1) / TF::TX, (p.g.nzi + TF::TZ - 1) / TF::TZ, units);
         hipLaunchKernelGGL(full, grid, dim3(st::NT), 0, st, ordered(q, grid));
         return hipGetLastError();
     }
@@ -1961,12 +1961,13 @@ hipError_t fwd_step_t(const StepParams& p, int units, bool store, bool born, hip
     return hipGetLastError();
 }
 
-template <class T>
+// Lean tiles of T, and the general kernel of an unframed launch on TF tiles.
+template <class T, class TF = T>
 hipError_t fwd_step2_t(const StepParams& p, int units, bool store, hipStream_t st) {
-    return framed<T, BandTile<T>>(p, units, store ? st::k_fwd_step2_in<T, true> : st::k_fwd_step2_in<T, false>,
-                                  store ? st::k_fwd_step2<T, true> : st::k_fwd_step2<T, false>,
-                                  store ? st::k_fwd_step2<BandTile<T>, true> : st::k_fwd_step2<BandTile<T>, false>,
-                                  st);
+    return framed<T, BandTile<T>, TF>(p, units, store ? st::k_fwd_step2_in<T, true> : st::k_fwd_step2_in<T, false>,
+                                      store ? st::k_fwd_step2<TF, true> : st::k_fwd_step2<TF, false>,
+                                      store ? st::k_fwd_step2<BandTile<T>, true> : st::k_fwd_step2<BandTile<T>, false>,
+                                      st);
 }
 
 template <class T>
@@ -2006,7 +2007,11 @@ hipError_t launch_fwd_step2(const StepParams& p, int units, bool store, hipStrea
         case 1: return fwd_step2_t<T1>(p, units, store, st);
         case 2: return fwd_step2_t<T2>(p, units, store, st);
         case 3: return fwd_step2_t<T3>(p, units, store, st);
-        default: return fwd_step2_t<T0>(p, units, store, st);
+        // the default's lean tiles are 64 x 32 in the two-step forward (its
+        // call 0.446 against 0.459 ms at 1000 x 300 x 160 units), 64 x 16
+        // for the general kernel of an unframed launch (181-185 against
+        // 101-103 VGPRs at 64 x 32)
+        default: return fwd_step2_t<T3, T0>(p, units, store, st);
     }
 }
 
